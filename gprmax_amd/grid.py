@@ -1,0 +1,348 @@
+"""The grid object ``G``: the data contract between model build and the solve
+loop (gprMax/grid.py:80-272).
+
+Attribute names, shapes and dtypes follow the reference so that a solve
+function written against the reference's ``G`` reads this one unchanged:
+
+* ``ID``      uint32 [6, nx+1, ny+1, nz+1], component order Ex..Hz, initialised
+  to 1 (free_space);
+* ``Ex``..``Hz`` float32 [nx+1, ny+1, nz+1], z fastest;
+* ``updatecoeffsE/H`` float32 [nmat, 5];
+* ``updatecoeffsdispersive`` complex64 [nmat, 3*maxpoles];
+* ``pmls`` in the order x0, y0, z0, xmax, ymax, zmax.
+
+The geometry helpers below only cover what the solve path's tests and
+workloads need: boxes, rigidly painted voxels and the edge build of
+gprMax/yee_cell_build_ext.pyx:110-257 that turns cell materials into edge IDs.
+"""
+
+from collections import OrderedDict
+
+import numpy as np
+
+from .constants import complextype, floattype
+from .materials import Material, default_materials, process_materials
+from .pml import CFS, build_pmls, default_pml_thickness
+from .receivers import Rx
+from .snapshots import Snapshot
+from .sources import HertzianDipole, MagneticDipole, VoltageSource
+from .utilities import GeneralError, courant_dt, iterations_for
+from .waveforms import Waveform
+
+
+class FDTDGrid:
+    """Holds everything the solve loop reads and writes."""
+
+    def __init__(self):
+        self.title = ''
+        self.messages = False
+        self.progressbars = False
+        self.nthreads = 0
+        self.gpu = None
+        self.nx = self.ny = self.nz = 0
+        self.dx = self.dy = self.dz = 0
+        self.dt = 0
+        self.mode = None
+        self.iterations = 0
+        self.timewindow = 0
+        self.pmlthickness = default_pml_thickness(10)
+        self.cfs = []
+        self.pmls = []
+        self.pmlformulation = 'HORIPML'
+        self.materials = default_materials()
+        self.waveforms = []
+        self.voltagesources = []
+        self.hertziandipoles = []
+        self.magneticdipoles = []
+        self.transmissionlines = []
+        self.rxs = []
+        self.snapshots = []
+        self.maxpoles = 0
+        self.updatecoeffsdispersive = None
+        # float32/complex64 by default; float64/complex128 reproduces the
+        # reference's double-precision build (gprMax/constants.py:43-50).
+        self.floattype = floattype
+        self.complextype = complextype
+
+    # ------------------------------------------------------------------ arrays
+    def initialise_geometry_arrays(self):
+        """solid (cell material), per-cell rigid flag and ID (gprMax/grid.py:152-167).
+
+        The reference tracks rigidity per cell edge slot; every writer sets or
+        clears all slots of a cell at once, so one flag per cell for E and one
+        for H is equivalent.
+        """
+        self.solid = np.ones((self.nx, self.ny, self.nz), dtype=np.uint32)
+        self.rigidcellE = np.zeros((self.nx, self.ny, self.nz), dtype=bool)
+        self.rigidcellH = np.zeros((self.nx, self.ny, self.nz), dtype=bool)
+        self.ID = np.ones((6, self.nx + 1, self.ny + 1, self.nz + 1), dtype=np.uint32)
+
+    def initialise_field_arrays(self):
+        shape = (self.nx + 1, self.ny + 1, self.nz + 1)
+        for name in ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz'):
+            setattr(self, name, np.zeros(shape, dtype=self.floattype))
+
+    def initialise_dispersive_arrays(self):
+        shape = (self.maxpoles, self.nx + 1, self.ny + 1, self.nz + 1)
+        for name in ('Tx', 'Ty', 'Tz'):
+            setattr(self, name, np.zeros(shape, dtype=self.complextype))
+
+    @property
+    def ncells(self):
+        return self.nx * self.ny * self.nz
+
+
+# ---------------------------------------------------------------------- build
+class ModelBuilder:
+    """Programmatic construction of a grid, in the order run_model uses
+    (gprMax/model_build_run.py:110-281).
+
+    Usage: create, add materials / waveforms / sources / receivers / geometry,
+    then call :meth:`finalise` which builds PMLs, edge IDs and coefficients.
+    """
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, time_window, pml_cells=10,
+                 pml_formulation='HORIPML', cfs=None, dt=None, precision='single'):
+        G = FDTDGrid()
+        if precision == 'double':
+            G.floattype, G.complextype = np.float64, np.complex128
+        elif precision != 'single':
+            raise GeneralError('precision must be single or double')
+        G.nx, G.ny, G.nz = int(nx), int(ny), int(nz)
+        G.dx, G.dy, G.dz = float(dx), float(dy), float(dz)
+        G.dt, G.mode = courant_dt(G.nx, G.ny, G.nz, G.dx, G.dy, G.dz)
+        if dt is not None:
+            G.dt = dt
+        if G.mode != '3D':
+            raise GeneralError('only 3D models are supported by this solver core (mode {})'.format(G.mode))
+        G.iterations, G.timewindow = iterations_for(time_window, G.dt)
+        if isinstance(pml_cells, dict):
+            G.pmlthickness = OrderedDict((k, int(pml_cells.get(k, 0))) for k in default_pml_thickness())
+        else:
+            G.pmlthickness = default_pml_thickness(int(pml_cells))
+        for key, n in G.pmlthickness.items():
+            size = {'x': G.nx, 'y': G.ny, 'z': G.nz}[key[0]]
+            if 2 * n >= size:
+                raise GeneralError('#pml_cells has too many cells for the domain size')
+        G.pmlformulation = pml_formulation
+        G.cfs = list(cfs) if cfs else []
+        G.initialise_geometry_arrays()
+        self.G = G
+
+    # materials and waveforms ------------------------------------------------
+    def add_material(self, er, se, mr, sm, name):
+        m = Material(len(self.G.materials), name)
+        m.er, m.se, m.mr, m.sm = float(er), float(se), float(mr), float(sm)
+        if m.se == float('inf'):
+            m.averagable = False
+        self.G.materials.append(m)
+        return m
+
+    def add_waveform(self, kind, amp, freq, name):
+        w = Waveform(ID=name, type=kind, amp=amp, freq=freq)
+        self.G.waveforms.append(w)
+        return w
+
+    # sources and receivers ----------------------------------------------------
+    def _place(self, src, pol, cell, waveform, start, stop):
+        G = self.G
+        src.polarisation = pol
+        src.xcoord, src.ycoord, src.zcoord = (int(c) for c in cell)
+        src.xcoordorigin, src.ycoordorigin, src.zcoordorigin = src.xcoord, src.ycoord, src.zcoord
+        src.ID = '{}({},{},{})'.format(type(src).__name__, *cell)
+        src.waveformID = waveform
+        if start is None:
+            src.start, src.stop = 0, G.timewindow
+        else:
+            src.start = start
+            src.stop = min(stop, G.timewindow)
+        src.calculate_waveform_values(G)
+        return src
+
+    def add_hertzian_dipole(self, pol, cell, waveform, start=None, stop=None):
+        h = HertzianDipole()
+        h.dl = {'x': self.G.dx, 'y': self.G.dy, 'z': self.G.dz}[pol]
+        self.G.hertziandipoles.append(self._place(h, pol, cell, waveform, start, stop))
+        return h
+
+    def add_magnetic_dipole(self, pol, cell, waveform, start=None, stop=None):
+        m = MagneticDipole()
+        self.G.magneticdipoles.append(self._place(m, pol, cell, waveform, start, stop))
+        return m
+
+    def add_voltage_source(self, pol, cell, resistance, waveform, start=None, stop=None):
+        v = VoltageSource()
+        v.resistance = float(resistance)
+        self.G.voltagesources.append(self._place(v, pol, cell, waveform, start, stop))
+        return v
+
+    def add_rx(self, cell, name=None):
+        rx = Rx(ID=name or 'Rx({},{},{})'.format(*cell), xcoord=int(cell[0]),
+                ycoord=int(cell[1]), zcoord=int(cell[2]))
+        rx.allocate(self.G.iterations, ftype=self.G.floattype)
+        self.G.rxs.append(rx)
+        return rx
+
+    def add_snapshot(self, xs, ys, zs, xf, yf, zf, dx, dy, dz, time, filename='snap'):
+        s = Snapshot(xs, ys, zs, xf, yf, zf, dx, dy, dz, time, filename)
+        self.G.snapshots.append(s)
+        return s
+
+    # geometry ---------------------------------------------------------------
+    def box(self, xs, xf, ys, yf, zs, zf, material, averaging=True):
+        """#box over cells [xs, xf) x [ys, yf) x [zs, zf)
+        (gprMax/geometry_primitives_ext.pyx:618-690)."""
+        G = self.G
+        averaging = averaging and material.averagable
+        n = material.numID
+        G.solid[xs:xf, ys:yf, zs:zf] = n
+        G.rigidcellE[xs:xf, ys:yf, zs:zf] = not averaging
+        G.rigidcellH[xs:xf, ys:yf, zs:zf] = not averaging
+        if not averaging:
+            G.ID[0, xs:xf, ys:yf + 1, zs:zf + 1] = n
+            G.ID[1, xs:xf + 1, ys:yf, zs:zf + 1] = n
+            G.ID[2, xs:xf + 1, ys:yf + 1, zs:zf] = n
+            G.ID[3, xs:xf + 1, ys:yf, zs:zf] = n
+            G.ID[4, xs:xf, ys:yf + 1, zs:zf] = n
+            G.ID[5, xs:xf, ys:yf, zs:zf + 1] = n
+
+    def paint_voxels(self, mask, numids):
+        """Rigid (non-averaged) voxels painted in (i, j, k) order with the
+        semantics of build_voxel (gprMax/geometry_primitives_ext.pyx:314-376).
+
+        ``mask`` bool [nx, ny, nz] selects cells; ``numids`` uint32 [nx, ny, nz]
+        (or a scalar) gives each selected cell's material.  A voxel writes its
+        four x-edges into components 0 and 3, its four y-edges into 1 and 4 and
+        its four z-edges into 2 and 5.  Where voxels share an edge the one later
+        in (i, j, k) order wins, which is reproduced by writing the four
+        neighbour offsets in increasing voxel order.
+        """
+        G = self.G
+        numids = np.broadcast_to(np.asarray(numids, dtype=np.uint32), mask.shape)
+        G.solid[mask] = numids[mask]
+        G.rigidcellE[mask] = True
+        G.rigidcellH[mask] = True
+        # For the x-edge (i, j, k) the touching voxels in paint order are
+        # (i, j-1, k-1) < (i, j-1, k) < (i, j, k-1) < (i, j, k); a voxel at
+        # (i, j', k') writes edge (i, j'+a, k'+b) for a, b in {0, 1}.  Writing
+        # the offsets (a, b) = (1,1), (1,0), (0,1), (0,0) in that order makes
+        # the latest voxel win.  y- and z-edges are analogous.
+        order = ((1, 1), (1, 0), (0, 1), (0, 0))
+        nx, ny, nz = mask.shape
+        for comps, axes in (((0, 3), (1, 2)), ((1, 4), (0, 2)), ((2, 5), (0, 1))):
+            for a, b in order:
+                off = [0, 0, 0]
+                off[axes[0]] = a
+                off[axes[1]] = b
+                dst = (slice(off[0], off[0] + nx), slice(off[1], off[1] + ny), slice(off[2], off[2] + nz))
+                for c in comps:
+                    view = G.ID[c][dst]
+                    np.copyto(view, numids, where=mask)
+
+    # finalise ---------------------------------------------------------------
+    def finalise(self):
+        """PMLs, edge build, voltage-source materials, coefficients, receivers
+        (gprMax/model_build_run.py:187-281)."""
+        G = self.G
+        if any(v > 0 for v in G.pmlthickness.values()):
+            if not G.cfs:
+                G.cfs = [CFS()]
+            build_pmls(G)
+        else:
+            G.pmls = []
+        build_electric_components(G)
+        build_magnetic_components(G)
+        for vs in G.voltagesources:
+            vs.create_material(G)
+        process_materials(G)
+        for rx in G.rxs:
+            if not rx.outputs:
+                rx.allocate(G.iterations, ftype=G.floattype)
+        return G
+
+
+# ------------------------------------------------------------- edge build
+def _average_material(G, names, numIDs, cache, electric):
+    """Find or create the averaged material for one edge
+    (gprMax/yee_cell_build_ext.pyx:31-107).
+
+    A candidate matches when, for every constituent name, it holds that name as
+    many times as the required ID does.  For H edges the reference also accepts
+    any material whose ID holds both names an even number of times, which
+    includes materials that hold neither; that rule is kept.
+    """
+    required = '+'.join(names)
+    hit = cache.get(required)
+    if hit is not None:
+        return hit
+    found = None
+    for m in G.materials:
+        same = all(m.ID.count(n) == required.count(n) for n in names)
+        if not electric and not same:
+            same = m.ID.count(names[0]) % 2 == 0 and m.ID.count(names[1]) % 2 == 0
+        if same:
+            found = m.numID
+            break
+    if found is None:
+        found = len(G.materials)
+        m = Material(found, required)
+        m.type = 'dielectric-smoothed'
+        parts = [G.materials[n] for n in numIDs]
+        m.er = np.mean([p.er for p in parts], axis=0)
+        m.se = np.mean([p.se for p in parts], axis=0)
+        m.mr = np.mean([p.mr for p in parts], axis=0)
+        m.sm = np.mean([p.sm for p in parts], axis=0)
+        G.materials.append(m)
+    cache[required] = found
+    return found
+
+
+def _edge_build(G, comp, ranges, cell_offsets, rigidcells, electric):
+    """Set ID[comp] over ``ranges`` from the materials of the cells around each
+    edge, in the reference's (i, j, k) loop order for new materials."""
+    (i0, i1), (j0, j1), (k0, k1) = ranges
+    if i1 <= i0 or j1 <= j0 or k1 <= k0:
+        return
+    sl = []
+    rig = np.zeros((i1 - i0, j1 - j0, k1 - k0), dtype=bool)
+    for (di, dj, dk) in cell_offsets:
+        box = (slice(i0 + di, i1 + di), slice(j0 + dj, j1 + dj), slice(k0 + dk, k1 + dk))
+        sl.append(G.solid[box])
+        rig |= rigidcells[box]
+    same = np.ones_like(rig)
+    for s in sl[1:]:
+        same &= (s == sl[0])
+    target = G.ID[comp, i0:i1, j0:j1, k0:k1]
+    uniform = same & ~rig
+    target[uniform] = sl[0][uniform]
+    mixed = np.argwhere(~same & ~rig)  # C order = the reference's loop order
+    if mixed.size == 0:
+        return
+    cache = {}
+    names_of = [m.ID for m in G.materials]
+    for (a, b, c) in mixed:
+        ids = [int(s[a, b, c]) for s in sl]
+        if len(names_of) != len(G.materials):
+            names_of = [m.ID for m in G.materials]
+        names = [names_of[n] for n in ids]
+        target[a, b, c] = _average_material(G, names, ids, cache, electric)
+
+
+def build_electric_components(G):
+    """Edge IDs of Ex, Ey, Ez (gprMax/yee_cell_build_ext.pyx:110-185)."""
+    nx, ny, nz = G.nx, G.ny, G.nz
+    # cell offsets in the reference's numID1..numID4 order
+    _edge_build(G, 0, ((0, nx), (1, ny), (1, nz)),
+                ((0, 0, 0), (0, -1, 0), (0, -1, -1), (0, 0, -1)), G.rigidcellE, True)
+    _edge_build(G, 1, ((1, nx), (0, ny), (1, nz)),
+                ((0, 0, 0), (-1, 0, 0), (-1, 0, -1), (0, 0, -1)), G.rigidcellE, True)
+    _edge_build(G, 2, ((1, nx), (1, ny), (0, nz)),
+                ((0, 0, 0), (-1, 0, 0), (-1, -1, 0), (0, -1, 0)), G.rigidcellE, True)
+
+
+def build_magnetic_components(G):
+    """Face IDs of Hx, Hy, Hz (gprMax/yee_cell_build_ext.pyx:188-257)."""
+    nx, ny, nz = G.nx, G.ny, G.nz
+    _edge_build(G, 3, ((1, nx), (0, ny), (0, nz)), ((0, 0, 0), (-1, 0, 0)), G.rigidcellH, False)
+    _edge_build(G, 4, ((0, nx), (1, ny), (0, nz)), ((0, 0, 0), (0, -1, 0)), G.rigidcellH, False)
+    _edge_build(G, 5, ((0, nx), (0, ny), (1, nz)), ((0, 0, 0), (0, 0, -1)), G.rigidcellH, False)
