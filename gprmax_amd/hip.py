@@ -1,0 +1,239 @@
+"""ctypes binding of libgprmax_hip.so (C ABI in include/gpx.h).
+
+The library must have been built (``gprmax_amd.build.build()``); loading fails
+loudly otherwise -- there is no CPU fallback on this path.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from .pml import DIRECTION_CODE
+from .sources import device_source_table
+from .receivers import rx_coords
+from .utilities import GeneralError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, 'libgprmax_hip.so')
+
+PART = {'store_rx': 0, 'snapshot': 1, 'h_main': 2, 'h_pml': 3, 'h_src': 4,
+        'e_main': 5, 'e_pml': 6, 'e_src': 7, 'e_disp_b': 8}
+SRC_KIND = {'hertzian': 0, 'magnetic': 1, 'voltage': 2}
+
+
+class GpxDesc(ctypes.Structure):
+    _fields_ = [('nx', ctypes.c_int), ('ny', ctypes.c_int), ('nz', ctypes.c_int),
+                ('iterations', ctypes.c_int), ('nmat', ctypes.c_int), ('maxpoles', ctypes.c_int),
+                ('pml_formulation', ctypes.c_int), ('device', ctypes.c_int)]
+
+
+# Exported symbols and their argument types (include/gpx.h).
+_P, _I, _F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+SIGNATURES = {
+    'gpx_create': ([ctypes.POINTER(GpxDesc), ctypes.POINTER(_P)], _I),
+    'gpx_destroy': ([_P], _I),
+    'gpx_last_error': ([_P], ctypes.c_char_p),
+    'gpx_upload_id': ([_P, _P], _I),
+    'gpx_upload_coeffs': ([_P, _P, _P], _I),
+    'gpx_upload_dispersive': ([_P, _P], _I),
+    'gpx_add_pml': ([_P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _P, _P, ctypes.POINTER(_I)], _I),
+    'gpx_set_sources': ([_P, _I, _I, _P, _P, _P], _I),
+    'gpx_set_receivers': ([_P, _I, _P], _I),
+    'gpx_add_snapshot': ([_P] + [_I] * 10 + [ctypes.POINTER(_I)], _I),
+    'gpx_run': ([_P, _I, _I], _I),
+    'gpx_run_timed': ([_P, _I, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
+    'gpx_part': ([_P, _I, _I, _I], _I),
+    'gpx_set_launch': ([_P, _I, _I], _I),
+    'gpx_synchronize': ([_P], _I),
+    'gpx_download_rx': ([_P, _P], _I),
+    'gpx_download_fields': ([_P, _P], _I),
+    'gpx_upload_fields': ([_P, _P], _I),
+    'gpx_download_pml_phi': ([_P, _I, _I, _P], _I),
+    'gpx_upload_pml_phi': ([_P, _I, _I, _P], _I),
+    'gpx_download_dispersive_T': ([_P, _P], _I),
+    'gpx_upload_dispersive_T': ([_P, _P], _I),
+    'gpx_download_snapshot': ([_P, _I, _P], _I),
+    'gpx_device_info': ([_I, ctypes.c_char_p, _I, ctypes.POINTER(ctypes.c_size_t),
+                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I)], _I),
+    'gpx_memory_bytes': ([_P], ctypes.c_size_t),
+}
+
+_lib = None
+
+
+def library():
+    """Load libgprmax_hip.so once and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise GeneralError('libgprmax_hip.so is not built; run gprmax_amd.build.build() '
+                           '(the HIP path has no CPU fallback)')
+    lib = ctypes.CDLL(LIBRARY)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _arr(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+class HipSolver:
+    """One model resident on one GPU, built from a grid ``G``.
+
+    Uploads ``G.ID``, the coefficient tables, the PML slabs (in ``G.pmls``
+    order), the sources, receivers and snapshots; then steps on the device.
+    """
+
+    def __init__(self, G, device=0):
+        self.lib = library()
+        self.G = G
+        if G.floattype != np.float32:
+            raise GeneralError('the HIP solver computes in float32')
+        desc = GpxDesc(G.nx, G.ny, G.nz, G.iterations, len(G.updatecoeffsE), G.maxpoles,
+                       0 if G.pmlformulation == 'HORIPML' else 1, device)
+        ctx = ctypes.c_void_p()
+        rc = self.lib.gpx_create(ctypes.byref(desc), ctypes.byref(ctx))
+        if rc:
+            raise GeneralError('gpx_create failed ({}): {}'.format(rc, self.lib.gpx_last_error(None).decode()))
+        self.ctx = ctx
+        self.device = device
+        self.ntot = (G.nx + 1) * (G.ny + 1) * (G.nz + 1)
+        self._check(self.lib.gpx_upload_id(ctx, _arr(G.ID, np.uint32)[1]))
+        e, pe = _arr(G.updatecoeffsE, np.float32)
+        h, ph = _arr(G.updatecoeffsH, np.float32)
+        self._check(self.lib.gpx_upload_coeffs(ctx, pe, ph))
+        if G.maxpoles > 0:
+            d = np.ascontiguousarray(G.updatecoeffsdispersive, dtype=np.complex64)
+            self._check(self.lib.gpx_upload_dispersive(ctx, d.ctypes.data_as(ctypes.c_void_p)))
+        for pml in G.pmls:
+            eR = np.ascontiguousarray(np.stack([pml.ERA, pml.ERB, pml.ERE, pml.ERF]), dtype=np.float32)
+            hR = np.ascontiguousarray(np.stack([pml.HRA, pml.HRB, pml.HRE, pml.HRF]), dtype=np.float32)
+            slab = ctypes.c_int()
+            self._check(self.lib.gpx_add_pml(ctx, DIRECTION_CODE[pml.direction], pml.xs, pml.xf, pml.ys,
+                                             pml.yf, pml.zs, pml.zf, len(pml.CFS), float(pml.d),
+                                             eR.ctypes.data_as(ctypes.c_void_p),
+                                             hR.ctypes.data_as(ctypes.c_void_p), ctypes.byref(slab)))
+        for kind, sources in (('hertzian', G.hertziandipoles), ('magnetic', G.magneticdipoles),
+                              ('voltage', G.voltagesources)):
+            if sources:
+                info, fac, waves = device_source_table(sources, G)
+                self._check(self.lib.gpx_set_sources(ctx, SRC_KIND[kind], len(sources),
+                                                     _arr(info, np.int32)[1], _arr(fac, np.float32)[1],
+                                                     _arr(waves, np.float32)[1]))
+        if G.rxs:
+            self._check(self.lib.gpx_set_receivers(ctx, len(G.rxs), _arr(rx_coords(G), np.int32)[1]))
+        for snap in G.snapshots:
+            idx = ctypes.c_int()
+            self._check(self.lib.gpx_add_snapshot(ctx, snap.xs, snap.ys, snap.zs, snap.dx, snap.dy, snap.dz,
+                                                  snap.nx, snap.ny, snap.nz, snap.time, ctypes.byref(idx)))
+
+    # -- plumbing ----------------------------------------------------------
+    def _check(self, rc):
+        if rc:
+            raise GeneralError('HIP solver error {}: {}'.format(rc, self.lib.gpx_last_error(self.ctx).decode()))
+
+    def close(self):
+        if getattr(self, 'ctx', None):
+            self.lib.gpx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping ----------------------------------------------------------
+    def set_launch(self, xchunk=0, block=0):
+        self._check(self.lib.gpx_set_launch(self.ctx, xchunk, block))
+
+    def run(self, it0=0, n=None):
+        n = self.G.iterations - it0 if n is None else n
+        self._check(self.lib.gpx_run(self.ctx, it0, n))
+
+    def run_timed(self, it0=0, n=None, per_kernel=False):
+        """Run and return (ms_total, ms_h_mean, ms_e_mean) from HIP events."""
+        n = self.G.iterations - it0 if n is None else n
+        t, h, e = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.gpx_run_timed(self.ctx, it0, n, int(per_kernel), ctypes.byref(t),
+                                           ctypes.byref(h), ctypes.byref(e)))
+        return t.value, h.value, e.value
+
+    def part(self, name, arg=0, it=0):
+        self._check(self.lib.gpx_part(self.ctx, PART[name], arg, it))
+
+    def synchronize(self):
+        self._check(self.lib.gpx_synchronize(self.ctx))
+
+    # -- state ---------------------------------------------------------------
+    def download_fields(self):
+        out = np.empty((6, self.G.nx + 1, self.G.ny + 1, self.G.nz + 1), dtype=np.float32)
+        self._check(self.lib.gpx_download_fields(self.ctx, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def upload_fields(self, fields6):
+        a, p = _arr(fields6, np.float32)
+        if a.size != 6 * self.ntot:
+            raise GeneralError('upload_fields expects 6 x (nx+1)(ny+1)(nz+1) values')
+        self._check(self.lib.gpx_upload_fields(self.ctx, p))
+
+    def download_rx(self):
+        out = np.zeros((6, self.G.iterations, max(len(self.G.rxs), 1)), dtype=np.float32)
+        if self.G.rxs:
+            self._check(self.lib.gpx_download_rx(self.ctx, out.ctypes.data_as(ctypes.c_void_p)))
+        return out[:, :, :len(self.G.rxs)]
+
+    def download_phi(self, slab, which):
+        pml = self.G.pmls[slab]
+        out = np.empty(pml.phi_shapes()[which], dtype=np.float32)
+        self._check(self.lib.gpx_download_pml_phi(self.ctx, slab, which, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def upload_phi(self, slab, which, values):
+        pml = self.G.pmls[slab]
+        a, p = _arr(values, np.float32)
+        if a.shape != tuple(pml.phi_shapes()[which]):
+            raise GeneralError('Phi shape mismatch')
+        self._check(self.lib.gpx_upload_pml_phi(self.ctx, slab, which, p))
+
+    def download_T(self):
+        G = self.G
+        out = np.empty((3, G.maxpoles, G.nx + 1, G.ny + 1, G.nz + 1), dtype=np.complex64)
+        self._check(self.lib.gpx_download_dispersive_T(self.ctx, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def upload_T(self, T):
+        a = np.ascontiguousarray(T, dtype=np.complex64)
+        self._check(self.lib.gpx_upload_dispersive_T(self.ctx, a.ctypes.data_as(ctypes.c_void_p)))
+
+    def download_snapshot(self, index):
+        s = self.G.snapshots[index]
+        out = np.empty((6, s.nx, s.ny, s.nz), dtype=np.float32)
+        self._check(self.lib.gpx_download_snapshot(self.ctx, index, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def memory_bytes(self):
+        return int(self.lib.gpx_memory_bytes(self.ctx))
+
+
+def device_info(device=0):
+    lib = library()
+    name = ctypes.create_string_buffer(256)
+    total, free, cus = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+    rc = lib.gpx_device_info(device, name, 256, ctypes.byref(total), ctypes.byref(free), ctypes.byref(cus))
+    if rc:
+        raise GeneralError('gpx_device_info failed: {}'.format(lib.gpx_last_error(None).decode()))
+    return {'name': name.value.decode(), 'totalmem': total.value, 'freemem': free.value, 'cus': cus.value}

@@ -1,0 +1,139 @@
+/*
+ * gpx: C ABI of the MI355X FDTD solve path (libgprmax_hip.so).
+ *
+ * A gpx_ctx owns every device buffer of one model: the six field arrays, the
+ * material-index array ID, the coefficient tables, PML auxiliary arrays,
+ * dispersive auxiliary arrays, sources, receivers and snapshots.  Host arrays
+ * passed in are borrowed for the duration of the call only and copied
+ * synchronously.  All layouts are the reference's:
+ *
+ *   fields  float32 [nx+1][ny+1][nz+1], z fastest          (gprMax/grid.py:175-181)
+ *   ID      uint32  [6][nx+1][ny+1][nz+1], Ex,Ey,Ez,Hx,Hy,Hz (gprMax/grid.py:167)
+ *   coeffs  float32 [nmat][5] = (CA,CBx,CBy,CBz,srce) / (DA,DBx,DBy,DBz,srcm)
+ *                                                           (gprMax/materials.py:200-201)
+ *   disp    complex64 as float32 pairs [nmat][3*P][2] = (e0*eqt2, eqt, zt) per pole
+ *                                                           (gprMax/materials.py:204-208)
+ *   PML R   float32 [4][order][thickness] = RA, RB, RE, RF  (gprMax/pml.py:230-274)
+ *   PML Phi float32 [order][A][B][C], the padded shapes of gprMax/pml.py:202-219
+ *   rx buf  float32 [6][iterations][nrx]                    (gprMax/receivers.py:62)
+ *
+ * Every function returns 0 on success, a positive hipError_t for a HIP
+ * failure or a negative GPX_E* code for a bad argument or state; the message
+ * is available from gpx_last_error(ctx) (or gpx_last_error(NULL) for
+ * gpx_create failures).  Nothing throws across the ABI.  A ctx is used from
+ * one host thread at a time; all its work goes to one HIP stream.
+ */
+#ifndef GPX_H
+#define GPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPX_EARG   (-1) /* invalid argument */
+#define GPX_ESTATE (-2) /* call not valid in the current state */
+#define GPX_ELIMIT (-3) /* exceeds a compiled-in limit */
+
+typedef struct gpx_ctx gpx_ctx;
+
+typedef struct gpx_desc {
+    int nx, ny, nz;       /* grid size in cells (G.nx, G.ny, G.nz) */
+    int iterations;       /* G.iterations */
+    int nmat;             /* rows of updatecoeffsE/H */
+    int maxpoles;         /* Material.maxpoles; 0 when no dispersive material */
+    int pml_formulation;  /* 0 = HORIPML, 1 = MRIPML (G.pmlformulation) */
+    int device;           /* HIP device ordinal */
+} gpx_desc;
+
+/* Sub-steps of one iteration, for gpx_part (gprMax/model_build_run.py:424-470). */
+enum gpx_part_id {
+    GPX_PART_STORE_RX = 0,   /* store_outputs, before the H update            */
+    GPX_PART_SNAPSHOT = 1,   /* store snapshot `arg`                          */
+    GPX_PART_H_MAIN = 2,     /* update_magnetic                               */
+    GPX_PART_H_PML = 3,      /* PML.update_magnetic of slab `arg`             */
+    GPX_PART_H_SRC = 4,      /* magnetic dipoles                              */
+    GPX_PART_E_MAIN = 5,     /* update_electric, or dispersive part A         */
+    GPX_PART_E_PML = 6,      /* PML.update_electric of slab `arg`             */
+    GPX_PART_E_SRC = 7,      /* voltage sources then Hertzian dipoles         */
+    GPX_PART_E_DISP_B = 8    /* dispersive part B                             */
+};
+
+/* Source kinds for gpx_set_sources. */
+enum gpx_source_kind { GPX_SRC_HERTZIAN = 0, GPX_SRC_MAGNETIC = 1, GPX_SRC_VOLTAGE = 2 };
+
+/* Context lifetime.  Replaces the device set-up of solve_gpu
+ * (gprMax/model_build_run.py:477-583). */
+int gpx_create(const gpx_desc *desc, gpx_ctx **out);
+int gpx_destroy(gpx_ctx *ctx);
+const char *gpx_last_error(const gpx_ctx *ctx);
+
+/* Model data.  Replace the host->device copies of grid.py:251-272,
+ * model_build_run.py:512-519 and 540-549. */
+int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID);
+int gpx_upload_coeffs(gpx_ctx *ctx, const float *coeffsE, const float *coeffsH);
+int gpx_upload_dispersive(gpx_ctx *ctx, const float *disp_complex);
+
+/* PML slabs, added in the order they are applied (x0, y0, z0, xmax, ymax,
+ * zmax; gprMax/pml.py:307-335).  direction: 0 xminus, 1 yminus, 2 zminus,
+ * 3 xplus, 4 yplus, 5 zplus.  Returns the slab index through *slab. */
+int gpx_add_pml(gpx_ctx *ctx, int direction, int xs, int xf, int ys, int yf, int zs, int zf,
+                int order, float d, const float *eR, const float *hR, int *slab);
+
+/* Sources of one kind (gprMax/sources.py:235-283).  info int32 [n][6] =
+ * (i, j, k, pol, first_it, last_it); pol + 4 marks a hard voltage source.
+ * factors float32 [n][2]; waves float32 [n][iterations]. */
+int gpx_set_sources(gpx_ctx *ctx, int kind, int n, const int32_t *info, const float *factors,
+                    const float *waves);
+
+/* Receivers (gprMax/receivers.py:45-68): coords int32 [n][3]. */
+int gpx_set_receivers(gpx_ctx *ctx, int n, const int32_t *coords);
+
+/* Snapshots (gprMax/snapshots.py:170-201): start cell, step, output size,
+ * and the iteration+1 at which it is taken.  Returns its index. */
+int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, int dy, int dz,
+                     int snx, int sny, int snz, int time, int *index);
+
+/* The time loop: iterations [it0, it0 + n) with all sub-steps fused into one
+ * H pass and one E pass per iteration (replaces model_build_run.py:590-696).
+ * Enqueued on the ctx stream; returns without waiting. */
+int gpx_run(gpx_ctx *ctx, int it0, int n);
+
+/* As gpx_run, and reports the elapsed device time measured with HIP events
+ * on the ctx stream, plus the mean duration of the H and E passes when
+ * per_kernel is non-zero (events between every launch).  Waits. */
+int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
+                  float *ms_h, float *ms_e);
+
+/* One unfused sub-step (enum gpx_part_id) at iteration it. */
+int gpx_part(gpx_ctx *ctx, int part, int arg, int it);
+
+/* Launch shape of the H and E passes: planes per chunk along x (0 = auto)
+ * and threads per block (64, 128, 256, 512 or 1024; 0 = auto). */
+int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
+
+int gpx_synchronize(gpx_ctx *ctx);
+
+/* State transfer (synchronous). */
+int gpx_download_rx(gpx_ctx *ctx, float *out);
+int gpx_download_fields(gpx_ctx *ctx, float *out6);
+int gpx_upload_fields(gpx_ctx *ctx, const float *in6);
+int gpx_download_pml_phi(gpx_ctx *ctx, int slab, int which, float *out);
+int gpx_upload_pml_phi(gpx_ctx *ctx, int slab, int which, const float *in);
+int gpx_download_dispersive_T(gpx_ctx *ctx, float *out_complex);
+int gpx_upload_dispersive_T(gpx_ctx *ctx, const float *in_complex);
+int gpx_download_snapshot(gpx_ctx *ctx, int index, float *out6);
+
+/* Device query for the drop-in GPU object (gprMax/utilities.py:341-366). */
+int gpx_device_info(int device, char *name, int namelen, size_t *totalmem, size_t *freemem,
+                    int *cus);
+/* Bytes of device memory held by ctx. */
+size_t gpx_memory_bytes(const gpx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPX_H */

@@ -1,0 +1,57 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+REF_TREE = os.path.join(ROOT, 'oracle', '_ref', 'exact')
+
+
+def pytest_configure(config):
+    config.addinivalue_line('markers', 'gpu: needs an AMD GPU (MI355X) and the built HIP library')
+    config.addinivalue_line('markers', 'slow: runs for more than a few seconds on the CPU')
+
+
+def gpu_present():
+    return os.path.exists('/dev/kfd')
+
+
+def pytest_collection_modifyitems(config, items):
+    if gpu_present():
+        return
+    skip = pytest.mark.skip(reason='no GPU device in this environment')
+    for item in items:
+        if 'gpu' in item.keywords:
+            item.add_marker(skip)
+
+
+@pytest.fixture(scope='session')
+def oracle_lib():
+    import numpy as np
+    import oracle
+    return oracle.load(np.float32)
+
+
+@pytest.fixture(scope='session')
+def ref_modules():
+    """The reference's own Cython kernels compiled into oracle/_ref (only in a
+    container where the reference is mounted)."""
+    if not os.path.isdir(REF_TREE):
+        pytest.skip('reference kernels not built (oracle/build_ref.py)')
+    import importlib
+    sys.path.insert(0, REF_TREE)
+    try:
+        mods = {
+            'fields': importlib.import_module('gprMax.fields_updates_ext'),
+            'snap': importlib.import_module('gprMax.snapshots_ext'),
+        }
+        for field in ('electric', 'magnetic'):
+            for form in ('HORIPML', 'MRIPML'):
+                mods[(field, form)] = importlib.import_module(
+                    'gprMax.pml_updates.pml_updates_{}_{}_ext'.format(field, form))
+    finally:
+        sys.path.remove(REF_TREE)
+    return mods

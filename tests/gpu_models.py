@@ -1,0 +1,57 @@
+"""Small seeded models that exercise every feature of the solve path at once:
+random materials on every edge, PML on all six faces, all source kinds,
+receivers, snapshots and optional dispersion."""
+
+import numpy as np
+
+from gprmax_amd.grid import ModelBuilder
+from gprmax_amd.materials import add_dispersion
+from gprmax_amd.pml import CFS
+
+
+def rich_model(seed=0, shape=(26, 22, 19), iterations=40, pml=4, order=1, formulation='HORIPML',
+               poles=0, kind='debye', nmat=9, snapshots=True):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    cfs = None
+    if order == 2:
+        a, b = CFS(), CFS()
+        b.alpha.max = 0.05
+        b.kappa.scalingprofile, b.kappa.min, b.kappa.max = 'quadratic', 1, 3
+        cfs = [a, b]
+    bld = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, iterations, pml_cells=pml,
+                       pml_formulation=formulation, cfs=cfs)
+    mats = []
+    for n in range(nmat):
+        m = bld.add_material(rng.uniform(1, 9), rng.uniform(0, 0.05), 1, 0, 'm{}'.format(n))
+        if poles and n % 2 == 0:
+            if kind == 'debye':
+                add_dispersion(m, 'debye', list(rng.uniform(0.5, 5, poles)), list(10 ** rng.uniform(-11, -9, poles)))
+            elif kind == 'lorentz':
+                add_dispersion(m, 'lorentz', list(rng.uniform(0.5, 5, poles)), list(rng.uniform(1e9, 5e9, poles)),
+                               list(rng.uniform(1e8, 5e8, poles)))
+            else:
+                add_dispersion(m, 'drude', None, list(rng.uniform(1e9, 3e9, poles)), list(rng.uniform(1e9, 5e9, poles)))
+        mats.append(m)
+    mask = rng.random((nx, ny, nz)) < 0.7
+    ids = rng.integers(2, 2 + nmat, (nx, ny, nz)).astype(np.uint32)
+    bld.paint_voxels(mask, ids)
+    # a PEC block
+    bld.box(nx // 2 - 2, nx // 2 + 1, 3, 6, nz // 2, nz // 2 + 3, bld.G.materials[0], averaging=False)
+    bld.add_waveform('ricker', 1, 20e9, 'w1')
+    bld.add_waveform('gaussiandot', 2.0, 15e9, 'w2')
+    c = (nx // 2, ny // 2, nz // 2)
+    bld.add_hertzian_dipole('z', c, 'w1')
+    bld.add_hertzian_dipole('x', (c[0] - 3, c[1] + 2, c[2] - 1), 'w2', start=5e-12, stop=40e-12)
+    bld.add_hertzian_dipole('z', c, 'w2')          # two sources on one edge
+    bld.add_magnetic_dipole('y', (c[0] + 2, c[1] - 3, c[2]), 'w1')
+    bld.add_voltage_source('y', (c[0] - 4, c[1] - 2, c[2] + 2), 50.0, 'w2')
+    bld.add_voltage_source('x', (c[0] + 4, c[1] + 3, c[2] - 2), 0.0, 'w1')
+    for r in range(5):
+        bld.add_rx((int(rng.integers(pml, nx - pml)), int(rng.integers(pml, ny - pml)),
+                    int(rng.integers(pml, nz - pml))))
+    bld.add_rx(c)
+    if snapshots:
+        bld.add_snapshot(0, 0, 0, nx, ny, nz, 1, 1, 1, iterations // 2)
+        bld.add_snapshot(2, 3, 1, 2 + 2 * 5, 3 + 3 * 4, 1 + 2 * 6, 2, 3, 2, iterations - 1)
+    return bld.finalise()

@@ -10,16 +10,20 @@ from gprmax_amd.materials import add_dispersion
 
 
 def _base(precision='single', n=100, time_window=3e-9, pml_cells=10, pml_formulation='HORIPML', cfs=None):
-    return ModelBuilder(n, n, n, 0.001, 0.001, 0.001, time_window, pml_cells=pml_cells,
-                        pml_formulation=pml_formulation, cfs=cfs, precision=precision)
+    b = ModelBuilder(n, n, n, 0.001, 0.001, 0.001, time_window, pml_cells=pml_cells,
+                     pml_formulation=pml_formulation, cfs=cfs, precision=precision)
+    # source at the centre, receiver 20% of the side further along each axis
+    b.src = (n // 2,) * 3
+    b.rx = (n // 2 + n // 5,) * 3
+    return b
 
 
 def hertzian_dipole_fs(precision='single', **kw):
     """tests/models_basic/hertzian_dipole_fs/hertzian_dipole_fs.in"""
     b = _base(precision, **kw)
     b.add_waveform('gaussiandot', 1, 1e9, 'myWave')
-    b.add_hertzian_dipole('z', (50, 50, 50), 'myWave')
-    b.add_rx((70, 70, 70))
+    b.add_hertzian_dipole('z', b.src, 'myWave')
+    b.add_rx(b.rx)
     return b.finalise()
 
 
@@ -27,33 +31,33 @@ def magnetic_dipole_fs(precision='single', **kw):
     """tests/models_basic/magnetic_dipole_fs/magnetic_dipole_fs.in"""
     b = _base(precision, **kw)
     b.add_waveform('gaussiandot', 1, 1e9, 'myWave')
-    b.add_magnetic_dipole('z', (50, 50, 50), 'myWave')
-    b.add_rx((70, 70, 70))
+    b.add_magnetic_dipole('z', b.src, 'myWave')
+    b.add_rx(b.rx)
     return b.finalise()
 
 
-def hertzian_dipole_hs(precision='single', **kw):
+def hertzian_dipole_hs(precision='single', n=100, **kw):
     """tests/models_basic/hertzian_dipole_hs/hertzian_dipole_hs.in: a box of
     er=8 over z in [0, 50) cells, averaged at its surface."""
-    b = _base(precision, **kw)
+    b = _base(precision, n=n, **kw)
     hs = b.add_material(8, 0, 1, 0, 'half_space')
     b.add_waveform('gaussiandot', 1, 1e9, 'myWave')
-    b.add_hertzian_dipole('z', (50, 50, 50), 'myWave')
-    b.add_rx((70, 70, 70))
-    b.box(0, 100, 0, 100, 0, 50, hs, averaging=True)
+    b.add_hertzian_dipole('z', b.src, 'myWave')
+    b.add_rx(b.rx)
+    b.box(0, n, 0, n, 0, n // 2, hs, averaging=True)
     return b.finalise()
 
 
-def hertzian_dipole_dispersive(precision='single', **kw):
+def hertzian_dipole_dispersive(precision='single', n=100, **kw):
     """tests/models_basic/hertzian_dipole_dispersive/hertzian_dipole_dispersive.in:
     water (er_inf 4.9, one Debye pole of 75.2 and 9.231 ps) filling the domain."""
-    b = _base(precision, **kw)
+    b = _base(precision, n=n, **kw)
     water = b.add_material(4.9, 0, 1, 0, 'myWater')
     add_dispersion(water, 'debye', [75.2], [9.231e-12])
     b.add_waveform('gaussiandot', 1, 1e9, 'myWave')
-    b.add_hertzian_dipole('z', (50, 50, 50), 'myWave')
-    b.add_rx((70, 70, 70))
-    b.box(0, 100, 0, 100, 0, 100, water, averaging=True)
+    b.add_hertzian_dipole('z', b.src, 'myWave')
+    b.add_rx(b.rx)
+    b.box(0, n, 0, n, 0, n, water, averaging=True)
     return b.finalise()
 
 

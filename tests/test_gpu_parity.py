@@ -1,0 +1,230 @@
+"""HIP path against the CPU oracle, through the C ABI.
+
+Sub-steps are compared one by one from the same random state, then whole
+runs.  Both sides compute in float32 without fused multiply-add and in the
+same association order, so the expected agreement is bit for bit
+(np.array_equal, which treats -0 and +0 as equal).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import solver as osolver
+from gpu_models import rich_model
+from models import BASIC_MODELS
+
+pytestmark = pytest.mark.gpu
+
+COMPS = ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
+
+
+def _hip():
+    from gprmax_amd.hip import HipSolver
+    return HipSolver
+
+
+def _random_state(G, seed):
+    """Random fields, Phi and T on the oracle grid; returns the arrays."""
+    rng = np.random.default_rng(seed)
+    osolver.initialise_state(G)
+    for c in COMPS:
+        getattr(G, c)[...] = rng.standard_normal(getattr(G, c).shape).astype(np.float32)
+    for pml in G.pmls:
+        for name in ('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2'):
+            a = getattr(pml, name)
+            a[...] = rng.standard_normal(a.shape).astype(np.float32)
+    if G.maxpoles:
+        real = np.all(G.updatecoeffsdispersive.imag == 0)
+        # T stays exactly zero on edges whose material has no poles; a
+        # physical state never holds anything else there.
+        has_poles = np.any(G.updatecoeffsdispersive != 0, axis=1)
+        for c, name in enumerate(('Tx', 'Ty', 'Tz')):
+            a = getattr(G, name)
+            re = rng.standard_normal(a.shape)
+            a[...] = re if real else re + 1j * rng.standard_normal(a.shape)
+            a[:, ~has_poles[G.ID[c]]] = 0
+
+
+def _push_state(s, G):
+    s.upload_fields(np.stack([getattr(G, c) for c in COMPS]))
+    for n, pml in enumerate(G.pmls):
+        for w, name in enumerate(('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2')):
+            s.upload_phi(n, w, getattr(pml, name))
+    if G.maxpoles:
+        s.upload_T(np.stack([G.Tx, G.Ty, G.Tz]))
+
+
+def _assert_state_equal(s, G, what):
+    F = s.download_fields()
+    for n, c in enumerate(COMPS):
+        assert np.array_equal(F[n], getattr(G, c)), (what, c, np.abs(F[n] - getattr(G, c)).max())
+    for n, pml in enumerate(G.pmls):
+        for w, name in enumerate(('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2')):
+            assert np.array_equal(s.download_phi(n, w), getattr(pml, name)), (what, pml.ID, name)
+    if G.maxpoles:
+        T = s.download_T()
+        for n, name in enumerate(('Tx', 'Ty', 'Tz')):
+            assert np.array_equal(T[n], getattr(G, name)), (what, name)
+
+
+CASES = {
+    'horipml1': dict(order=1, formulation='HORIPML'),
+    'horipml2': dict(order=2, formulation='HORIPML'),
+    'mripml1': dict(order=1, formulation='MRIPML'),
+    'mripml2': dict(order=2, formulation='MRIPML'),
+    'debye3': dict(poles=3, kind='debye'),
+    'debye1': dict(poles=1, kind='debye'),
+    'lorentz2': dict(poles=2, kind='lorentz'),
+    'drude1': dict(poles=1, kind='drude'),
+}
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_each_substep_matches_oracle(case):
+    """update_magnetic, every H-PML slab, magnetic sources, update_electric /
+    dispersive A, every E-PML slab, electric sources, dispersive B, receivers
+    and snapshots, each from the same random state."""
+    G = rich_model(seed=7, **CASES[case])
+    lib = oracle.load(np.float32)
+    f = lib.funcs
+    it = 9
+    with _hip()(G) as s:
+        steps = [('h_main', 0, lambda: f['update_magnetic'](G.nx, G.ny, G.nz, 4, oracle.ptr(G.updatecoeffsH),
+                                                            oracle.ptr(G.ID), *(oracle.ptr(getattr(G, c)) for c in COMPS)))]
+        for n, pml in enumerate(G.pmls):
+            steps.append(('h_pml', n, (lambda p=pml: osolver._pml_call(lib, G, p, 1, 4))))
+        steps.append(('h_src', 0, lambda: [osolver.magnetic_dipole_update(m, it, G) for m in G.magneticdipoles]))
+        if G.maxpoles:
+            steps.append(('e_main', 0, lambda: f['update_electric_dispersive_A'](
+                G.nx, G.ny, G.nz, 4, G.maxpoles, oracle.ptr(G.updatecoeffsE), oracle.ptr(G.updatecoeffsdispersive),
+                oracle.ptr(G.ID), oracle.ptr(G.Tx), oracle.ptr(G.Ty), oracle.ptr(G.Tz),
+                *(oracle.ptr(getattr(G, c)) for c in COMPS))))
+        else:
+            steps.append(('e_main', 0, lambda: f['update_electric'](G.nx, G.ny, G.nz, 4, oracle.ptr(G.updatecoeffsE),
+                                                                    oracle.ptr(G.ID), *(oracle.ptr(getattr(G, c)) for c in COMPS))))
+        for n, pml in enumerate(G.pmls):
+            steps.append(('e_pml', n, (lambda p=pml: osolver._pml_call(lib, G, p, 0, 4))))
+
+        def esrc():
+            for v in G.voltagesources:
+                osolver.voltage_source_update(v, it, G)
+            for h in G.hertziandipoles:
+                osolver.hertzian_dipole_update(h, it, G)
+        steps.append(('e_src', 0, esrc))
+        if G.maxpoles:
+            steps.append(('e_disp_b', 0, lambda: f['update_electric_dispersive_B'](
+                G.nx, G.ny, G.nz, 4, G.maxpoles, oracle.ptr(G.updatecoeffsdispersive), oracle.ptr(G.ID),
+                oracle.ptr(G.Tx), oracle.ptr(G.Ty), oracle.ptr(G.Tz), oracle.ptr(G.Ex), oracle.ptr(G.Ey),
+                oracle.ptr(G.Ez))))
+        for n, (part, arg, cpu) in enumerate(steps):
+            _random_state(G, 100 + n)
+            _push_state(s, G)
+            s.part(part, arg, it)
+            cpu()
+            _assert_state_equal(s, G, '{} {}'.format(part, arg))
+
+        # receivers: the sample at iteration it is the current state
+        _random_state(G, 999)
+        _push_state(s, G)
+        s.part('store_rx', 0, it)
+        buf = s.download_rx()
+        for r, rx in enumerate(G.rxs):
+            for c, name in enumerate(COMPS):
+                assert buf[c, it, r] == getattr(G, name)[rx.xcoord, rx.ycoord, rx.zcoord]
+        # snapshots
+        for n, snap in enumerate(G.snapshots):
+            s.part('snapshot', n, it)
+            osolver.store_snapshot(lib, snap, G)
+            out = s.download_snapshot(n)
+            for c in range(6):
+                assert np.array_equal(out[c], snap.fields[c]), ('snapshot', n, c)
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_fused_run_matches_oracle(case):
+    """All iterations with the fused passes: final state, receiver traces and
+    snapshots equal to the CPU solve."""
+    G = rich_model(seed=3, **CASES[case])
+    Gc = rich_model(seed=3, **CASES[case])
+    with _hip()(G) as s:
+        s.run(0, G.iterations)
+        F = s.download_fields()
+        rxb = s.download_rx()
+        snaps = [s.download_snapshot(n) for n in range(len(G.snapshots))]
+        phis = [[s.download_phi(n, w) for w in range(4)] for n in range(len(G.pmls))]
+        T = s.download_T() if G.maxpoles else None
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    for n, c in enumerate(COMPS):
+        assert np.array_equal(F[n], getattr(Gc, c)), c
+    for r, rx in enumerate(Gc.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
+    for n, snap in enumerate(Gc.snapshots):
+        for c in range(6):
+            assert np.array_equal(snaps[n][c], snap.fields[c]), ('snapshot', n, c)
+    for n, pml in enumerate(Gc.pmls):
+        for w, name in enumerate(('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2')):
+            assert np.array_equal(phis[n][w], getattr(pml, name)), (pml.ID, name)
+    if T is not None:
+        for n, name in enumerate(('Tx', 'Ty', 'Tz')):
+            assert np.array_equal(T[n], getattr(Gc, name)), name
+
+
+@pytest.mark.parametrize('xchunk,block', [(1, 64), (7, 128), (0, 256), (300, 512), (5, 1024)])
+def test_launch_shapes_do_not_change_results(xchunk, block):
+    G = rich_model(seed=5, poles=2, order=2)
+    ref = None
+    with _hip()(G) as s:
+        s.set_launch(xchunk, block)
+        s.run(0, G.iterations)
+        out = s.download_fields()
+    with _hip()(G) as s:
+        s.run(0, G.iterations)
+        ref = s.download_fields()
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize('name', ['hertzian_dipole_fs', 'hertzian_dipole_dispersive', 'magnetic_dipole_fs'])
+def test_reference_models_match_oracle_and_golden(name):
+    """The reference's 100^3 test models, 1559 iterations, through solve_hip."""
+    from gprmax_amd.solver import solve_hip
+    G = BASIC_MODELS[name]()
+    tsolve, mem = solve_hip(1, 1, G)
+    assert tsolve > 0 and mem > 0
+    ours = np.stack([G.rxs[0].outputs[c] for c in COMPS])
+    Gc = BASIC_MODELS[name]()
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    cpu = np.stack([Gc.rxs[0].outputs[c] for c in COMPS])
+    assert np.array_equal(ours, cpu)
+    ref = np.load('tests/golden/{}_ref.npz'.format(name))['traces']
+    if name == 'hertzian_dipole_dispersive':
+        for c in range(5):
+            shift = 1 if c >= 3 else 0
+            a, b = (ours[c][shift:], ref[c][:len(ref[c]) - shift])
+            assert np.abs(a - b).max() / np.abs(b).max() < 1e-6
+
+
+def test_config1_freespace_ricker_within_tolerance():
+    """100^3 free space, 10-cell PML, z-dipole with a 1.5 GHz Ricker, Ez receiver
+    20 cells along x, 500 iterations: |GPU - CPU| <= 1e-4 of the peak |Ez| (the
+    stated tolerance; the expected result is exact equality)."""
+    from gprmax_amd.workloads import freespace_ricker
+    G = freespace_ricker()
+    Gc = freespace_ricker()
+    from gprmax_amd.solver import solve_hip
+    solve_hip(1, 1, G)
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    a, b = G.rxs[0].outputs['Ez'], Gc.rxs[0].outputs['Ez']
+    peak = np.abs(b).max()
+    assert peak > 0
+    assert np.abs(a - b).max() <= 1e-4 * peak
+    assert np.array_equal(a, b)
+
+
+def test_hip_rejects_bad_input():
+    from gprmax_amd.utilities import GeneralError
+    G = rich_model(seed=1, iterations=4, snapshots=False)
+    G.ID[0, 0, 0, 0] = len(G.updatecoeffsE) + 5
+    with pytest.raises(GeneralError, match='material index'):
+        _hip()(G)

@@ -1,0 +1,184 @@
+"""Host model layer: time step, coefficients, PML profiles, waveforms, ID build.
+
+Each check is against a closed form or an independent brute force, not a
+retyping of the code under test.
+"""
+
+import numpy as np
+import pytest
+
+from gprmax_amd.constants import c, e0, m0
+from gprmax_amd.grid import ModelBuilder
+from gprmax_amd.materials import Material
+from gprmax_amd.pml import CFS
+from gprmax_amd.utilities import courant_dt, iterations_for, round_value
+from gprmax_amd.waveforms import Waveform
+
+
+def test_dt_and_iterations_match_reference_files():
+    # The reference's receiver files record dt and Iterations for 1 mm cells
+    # and a 3 ns window (tests/models_basic/*/*.out attributes).
+    ref = np.load('tests/golden/hertzian_dipole_fs_ref.npz')
+    dt, mode = courant_dt(100, 100, 100, 1e-3, 1e-3, 1e-3)
+    assert mode == '3D'
+    assert dt == float(ref['dt'])
+    assert iterations_for(3e-9, dt)[0] == int(ref['iterations'])
+    # 2-D files fix the 2-D Courant limit too
+    ref2 = np.load('tests/golden/2D_EzHxHy_ref.npz')
+    assert courant_dt(100, 100, 1, 1e-3, 1e-3, 1e-3)[0] == float(ref2['dt'])
+    # dt never exceeds the Courant limit and is within 1e-15 of it
+    raw = 1 / (c * np.sqrt(3) / 1e-3)
+    assert dt <= raw and raw - dt < 1e-26
+
+
+def test_round_value_half_down():
+    assert round_value(2.5) == 2 and round_value(3.5) == 3 and round_value(2.51) == 3
+    assert round_value(1.23456789, 3) == 1.234
+
+
+def test_free_space_and_pec_coefficients_closed_form():
+    b = ModelBuilder(20, 20, 20, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+    G = b.finalise()
+    dt = G.dt
+    # free space: CA = 1, CB = dt/(e0 dx), DA = 1, DB = dt/(m0 dx)
+    assert np.isclose(G.updatecoeffsE[1, 0], 1.0, rtol=0, atol=0)
+    assert np.isclose(G.updatecoeffsE[1, 1], dt / (e0 * 1e-3), rtol=1e-7)
+    assert np.isclose(G.updatecoeffsE[1, 4], dt / e0, rtol=1e-7)
+    assert np.isclose(G.updatecoeffsH[1, 1], dt / (m0 * 1e-3), rtol=1e-7)
+    assert np.all(G.updatecoeffsE[0] == 0)          # pec
+    assert np.isclose(G.updatecoeffsH[0, 0], 1.0)   # pec still updates H
+
+
+def test_lossy_coefficient_closed_form():
+    b = ModelBuilder(20, 20, 20, 2e-3, 2e-3, 2e-3, 10, pml_cells=0)
+    m = b.add_material(6.0, 0.02, 1, 0, 'soil')
+    G = b.finalise()
+    dt = G.dt
+    EA = e0 * 6 / dt + 0.01
+    EB = e0 * 6 / dt - 0.01
+    assert np.isclose(G.updatecoeffsE[m.numID, 0], EB / EA, rtol=1e-7)
+    assert np.isclose(G.updatecoeffsE[m.numID, 2], 1 / (EA * 2e-3), rtol=1e-7)
+
+
+def test_debye_coefficients_closed_form():
+    # One Debye pole: eqt = exp(-dt/tau), zt = (w/q)(1-eqt)/dt with w = de/tau, q = -1/tau
+    b = ModelBuilder(20, 20, 20, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+    m = b.add_material(4.9, 0, 1, 0, 'water')
+    from gprmax_amd.materials import add_dispersion
+    add_dispersion(m, 'debye', [75.2], [9.231e-12])
+    G = b.finalise()
+    dt, tau = G.dt, 9.231e-12
+    eqt = np.exp(-dt / tau)
+    zt = (75.2 / tau) / (-1 / tau) * (1 - eqt) / dt
+    row = G.updatecoeffsdispersive[m.numID]
+    assert np.isclose(row[1].real, eqt, rtol=1e-6) and row[1].imag == 0
+    assert np.isclose(row[2].real, zt, rtol=1e-5) and row[2].imag == 0
+    assert np.isclose(row[0].real, e0 * np.exp(-dt / (2 * tau)), rtol=1e-6)
+    # non-dispersive materials of a dispersive model have zero pole terms
+    assert np.all(G.updatecoeffsdispersive[1] == 0)
+    # complex permittivity at DC: er_inf + de
+    assert np.isclose(m.calculate_er(1e-3).real, 4.9 + 75.2, rtol=1e-9)
+
+
+def test_pml_profiles_closed_form():
+    cfs = CFS()
+    cfs.sigma.max = 2.0
+    E, H = cfs.calculate_values(10, cfs.sigma)
+    # quartic: sigma_E(p) = smax (p/t)^4, sigma_H(p) = smax ((p+1/2)/t)^4
+    p = np.arange(10)
+    assert np.allclose(E, 2.0 * (p / 10) ** 4, rtol=1e-12)
+    assert np.allclose(H, 2.0 * ((p + 0.5) / 10) ** 4, rtol=1e-12)
+    kE, kH = cfs.calculate_values(10, cfs.kappa)
+    assert np.all(kE == 1) and np.all(kH == 1)
+
+
+def test_pml_sigma_max_from_first_slab_only():
+    # x0 slab sits on er=8 cells for half its first layer; every other slab
+    # must reuse that sigma max (shared CFS object).
+    b = ModelBuilder(30, 30, 30, 1e-3, 1e-3, 1e-3, 10, pml_cells=5)
+    hs = b.add_material(8, 0, 1, 0, 'hs')
+    b.box(0, 30, 0, 30, 0, 15, hs, averaging=True)
+    G = b.finalise()
+    from gprmax_amd.constants import z0
+    smax = 0.8 * 5 / (z0 * 1e-3 * np.sqrt((8 + 1) / 2))
+    assert np.isclose(G.cfs[0].sigma.max, smax, rtol=1e-12)
+    assert [p.ID for p in G.pmls] == ['x0', 'y0', 'z0', 'xmax', 'ymax', 'zmax']
+    # HORIPML order 1, default CFS: RF = 2 sigma dt / (2 e0 + dt sigma)
+    sig = smax * ((np.arange(5) + 0.0) / 5) ** 4
+    rf = 2 * sig * G.dt / (2 * e0 + G.dt * sig)
+    for pml in G.pmls:
+        assert np.allclose(pml.ERF[0], rf, rtol=1e-6)
+
+
+def test_ricker_closed_form_and_integral():
+    w = Waveform('r', 'ricker', 1.0, 1.5e9)
+    f = 1.5e9
+    chi, zeta = np.sqrt(2) / f, np.pi ** 2 * f ** 2
+    t = np.linspace(0, 3e-9, 301)
+    v = np.array([w.calculate_value(x, 1e-12) for x in t])
+    d = t - chi
+    assert np.allclose(v, (1 - 2 * zeta * d ** 2) * np.exp(-zeta * d ** 2), rtol=1e-12, atol=1e-15)
+    # the Ricker pulse has zero mean
+    tt = np.linspace(0, 4 * chi, 20001)
+    vv = np.array([w.calculate_value(x, 1e-12) for x in tt])
+    assert abs(np.trapezoid(vv, tt)) < 1e-3 * chi
+
+
+def test_gaussiandot_is_derivative_of_gaussian():
+    g = Waveform('g', 'gaussian', 1.0, 1e9)
+    gd = Waveform('gd', 'gaussiandot', 1.0, 1e9)
+    h = 1e-15
+    for x in np.linspace(0.2e-9, 2e-9, 13):
+        fd = (g.calculate_value(x + h, 1e-12) - g.calculate_value(x - h, 1e-12)) / (2 * h)
+        assert np.isclose(gd.calculate_value(x, 1e-12), fd, rtol=1e-5, atol=1e-3)
+
+
+def _brute_voxels(nx, ny, nz, mask, numids):
+    """build_voxel (gprMax/geometry_primitives_ext.pyx:337-376) one voxel at a time."""
+    ID = np.ones((6, nx + 1, ny + 1, nz + 1), dtype=np.uint32)
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                if not mask[i, j, k]:
+                    continue
+                n = numids[i, j, k]
+                for c in (0, 3):
+                    for (a, b) in ((0, 0), (1, 1), (1, 0), (0, 1)):
+                        ID[c, i, j + a, k + b] = n
+                for c in (1, 4):
+                    for (a, b) in ((0, 0), (1, 1), (1, 0), (0, 1)):
+                        ID[c, i + a, j, k + b] = n
+                for c in (2, 5):
+                    for (a, b) in ((0, 0), (1, 1), (1, 0), (0, 1)):
+                        ID[c, i + a, j + b, k] = n
+    return ID
+
+
+def test_voxel_painting_matches_brute_force():
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 7, 6, 5
+    b = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, 5, pml_cells=0)
+    for n in range(5):
+        b.add_material(2 + n, 0, 1, 0, 'm%d' % n)
+    mask = rng.random((nx, ny, nz)) < 0.6
+    ids = rng.integers(2, 7, (nx, ny, nz)).astype(np.uint32)
+    b.paint_voxels(mask, ids)
+    assert np.array_equal(b.G.ID, _brute_voxels(nx, ny, nz, mask, ids))
+
+
+def test_averaged_half_space_edges():
+    # hertzian_dipole_hs: edges on the z=50 interface average four cells
+    # (two of each material), H faces there average two.
+    b = ModelBuilder(20, 20, 20, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+    hs = b.add_material(8, 0, 1, 0, 'half_space')
+    b.box(0, 20, 0, 20, 0, 10, hs, averaging=True)
+    G = b.finalise()
+    names = {m.numID: m for m in G.materials}
+    ex = names[int(G.ID[0, 5, 5, 10])]
+    assert ex.er == pytest.approx(4.5) and ex.ID.count('half_space') == 2
+    assert int(G.ID[0, 5, 5, 9]) == hs.numID and int(G.ID[0, 5, 5, 11]) == 1
+    assert int(G.ID[1, 5, 5, 10]) == ex.numID
+    assert int(G.ID[2, 5, 5, 9]) == hs.numID
+    # The reference's magnetic match accepts a material holding neither name an
+    # even number of times (zero), so the first such material (pec) is taken.
+    assert int(G.ID[5, 5, 5, 10]) == 0

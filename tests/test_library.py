@@ -1,0 +1,54 @@
+"""The C-ABI library loads and exports every symbol include/gpx.h declares.
+No compute call is made (this runs without a GPU)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, 'include', 'gpx.h')).read()
+    return sorted(set(re.findall(r'\b(gpx_[A-Za-z_]+)\s*\(', text)))
+
+
+def test_header_and_binding_agree():
+    from gprmax_amd.hip import SIGNATURES
+    assert declared_functions() == sorted(SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    from gprmax_amd.hip import LIBRARY, library
+    if not os.path.exists(LIBRARY):
+        pytest.fail('libgprmax_hip.so is not built (run __graft_entry__.build())')
+    lib = library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_create_rejects_bad_descriptors_before_touching_a_device():
+    from gprmax_amd.hip import GpxDesc, library
+    lib = library()
+    ctx = ctypes.c_void_p()
+    bad = [GpxDesc(1, 10, 10, 5, 2, 0, 0, 0),     # 2-D grid
+           GpxDesc(10, 10, 10, 0, 2, 0, 0, 0),    # no iterations
+           GpxDesc(10, 10, 10, 5, 2, 9, 0, 0),    # too many poles
+           GpxDesc(10, 10, 10, 5, 2, 0, 7, 0),    # unknown PML formulation
+           GpxDesc(10, 10, 10, 5, 5000, 0, 0, 0)]  # table larger than LDS
+    for d in bad:
+        rc = lib.gpx_create(ctypes.byref(d), ctypes.byref(ctx))
+        assert rc < 0, (d.nx, d.iterations, d.maxpoles)
+        assert lib.gpx_last_error(None)
+    assert lib.gpx_create(None, ctypes.byref(ctx)) < 0
+
+
+def test_oracle_and_product_are_separate():
+    """The product package never imports the oracle."""
+    pkg = os.path.join(ROOT, 'gprmax_amd')
+    for name in os.listdir(pkg):
+        if name.endswith('.py'):
+            src = open(os.path.join(pkg, name)).read()
+            assert 'import oracle' not in src and 'from oracle' not in src, name

@@ -1,0 +1,283 @@
+"""Pinning the CPU oracle before it is trusted as the checker.
+
+* bit for bit against the reference's own Cython kernels compiled in this
+  container (oracle/build_ref.py), when they are built;
+* bit for bit against the independent NumPy slicing twin;
+* against the reference's golden receiver traces (v3.0.13 / v3.1.0b1 files);
+* against the closed-form Hertzian dipole field;
+* invariants: zero-weight extra poles change nothing, real Debye poles keep
+  the imaginary part of T at zero, PML reflection is small.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import numpy_twin, ptr
+from oracle.solver import solve_cpu
+from analytical import hertzian_dipole_fs as analytical_dipole
+from models import BASIC_MODELS
+from gprmax_amd.pml import DIRECTIONS, PML
+
+NX, NY, NZ, NMAT = 23, 19, 17, 7
+COMPS = ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
+
+
+def _state(seed):
+    rng = np.random.default_rng(seed)
+    F = [rng.standard_normal((NX + 1, NY + 1, NZ + 1)).astype(np.float32) for _ in range(6)]
+    ID = rng.integers(0, NMAT, (6, NX + 1, NY + 1, NZ + 1)).astype(np.uint32)
+    cE = rng.uniform(-1, 1, (NMAT, 5)).astype(np.float32)
+    cH = rng.uniform(-1, 1, (NMAT, 5)).astype(np.float32)
+    return rng, F, ID, cE, cH
+
+
+def _boxes(t):
+    return {'xminus': (0, t, 0, NY, 0, NZ), 'xplus': (NX - t, NX, 0, NY, 0, NZ),
+            'yminus': (0, NX, 0, t, 0, NZ), 'yplus': (0, NX, NY - t, NY, 0, NZ),
+            'zminus': (0, NX, 0, NY, 0, t), 'zplus': (0, NX, 0, NY, NZ - t, NZ)}
+
+
+class _G:
+    dx = dy = dz = 0.5
+    cfs = None
+
+
+def _phi_shapes(direction, box, order):
+    xs, xf, ys, yf, zs, zf = box
+    p = PML(_G, direction=direction, xs=xs, xf=xf, ys=ys, yf=yf, zs=zs, zf=zf)
+    p.CFS = [None] * order
+    return p.phi_shapes()
+
+
+# ------------------------------------------------------------ vs NumPy twin
+def test_field_updates_match_numpy_twin(oracle_lib):
+    f = oracle_lib.funcs
+    _, F, ID, cE, cH = _state(1)
+    A = [a.copy() for a in F]
+    f['update_electric'](NX, NY, NZ, 4, ptr(cE), ptr(ID), *(ptr(a) for a in F))
+    numpy_twin.update_electric(NX, NY, NZ, cE, ID, *A)
+    assert all(np.array_equal(a, b) for a, b in zip(A, F))
+    f['update_magnetic'](NX, NY, NZ, 4, ptr(cH), ptr(ID), *(ptr(a) for a in F))
+    numpy_twin.update_magnetic(NX, NY, NZ, cH, ID, *A)
+    assert all(np.array_equal(a, b) for a, b in zip(A, F))
+
+
+@pytest.mark.parametrize('formulation', [0, 1])
+@pytest.mark.parametrize('order', [1, 2])
+def test_pml_slabs_match_numpy_twin(oracle_lib, formulation, order):
+    f = oracle_lib.funcs
+    rng, F, ID, cE, cH = _state(10 + 2 * formulation + order)
+    t = 5
+    for dname, box in _boxes(t).items():
+        d = DIRECTIONS.index(dname)
+        shapes = _phi_shapes(dname, box, order)
+        R = [rng.uniform(0.5, 1.5, (order, t)).astype(np.float32) for _ in range(4)]
+        for field, coeffs, s1, s2 in ((0, cE, shapes[0], shapes[1]), (1, cH, shapes[2], shapes[3])):
+            P1 = rng.standard_normal(s1).astype(np.float32)
+            P2 = rng.standard_normal(s2).astype(np.float32)
+            Q1, Q2 = P1.copy(), P2.copy()
+            A = {n: a.copy() for n, a in zip(COMPS, F)}
+            f['pml_update'](field, formulation, order, d, *box, 4, NX, NY, NZ, ptr(coeffs), ptr(ID),
+                            *(ptr(a) for a in F), ptr(P1), ptr(P2), *(ptr(r) for r in R),
+                            oracle_lib.real(0.37))
+            numpy_twin.pml_update(field, formulation, d, box, coeffs, ID, A, Q1, Q2, *R, 0.37)
+            assert all(np.array_equal(A[n], a) for n, a in zip(COMPS, F)), (dname, field)
+            assert np.array_equal(P1, Q1) and np.array_equal(P2, Q2), (dname, field)
+
+
+# ------------------------------------------------ vs the reference's kernels
+def test_field_updates_match_reference_kernels(oracle_lib, ref_modules):
+    f = oracle_lib.funcs
+    fu = ref_modules['fields']
+    _, F, ID, cE, cH = _state(2)
+    A = [a.copy() for a in F]
+    fu.update_electric(NX, NY, NZ, 4, cE, ID, *A)
+    f['update_electric'](NX, NY, NZ, 4, ptr(cE), ptr(ID), *(ptr(a) for a in F))
+    assert all(np.array_equal(a, b) for a, b in zip(A, F))
+    fu.update_magnetic(NX, NY, NZ, 4, cH, ID, *A)
+    f['update_magnetic'](NX, NY, NZ, 4, ptr(cH), ptr(ID), *(ptr(a) for a in F))
+    assert all(np.array_equal(a, b) for a, b in zip(A, F))
+
+
+@pytest.mark.parametrize('P', [1, 3])
+def test_dispersive_updates_match_reference_kernels(oracle_lib, ref_modules, P):
+    f = oracle_lib.funcs
+    fu = ref_modules['fields']
+    rng, F, ID, cE, _ = _state(3 + P)
+    disp = (rng.uniform(-1, 1, (NMAT, 3 * P)) + 1j * rng.uniform(-1, 1, (NMAT, 3 * P))).astype(np.complex64)
+    shape = (P, NX + 1, NY + 1, NZ + 1)
+    T = [(rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64) for _ in range(3)]
+    A, TA = [a.copy() for a in F], [t.copy() for t in T]
+    if P == 1:
+        fu.update_electric_dispersive_1pole_A(NX, NY, NZ, 4, cE, disp, ID, *TA, *A)
+        fu.update_electric_dispersive_1pole_B(NX, NY, NZ, 4, disp, ID, *TA, *A[:3])
+    else:
+        fu.update_electric_dispersive_multipole_A(NX, NY, NZ, 4, P, cE, disp, ID, *TA, *A)
+        fu.update_electric_dispersive_multipole_B(NX, NY, NZ, 4, P, disp, ID, *TA, *A[:3])
+    f['update_electric_dispersive_A'](NX, NY, NZ, 4, P, ptr(cE), ptr(disp), ptr(ID),
+                                      *(ptr(t) for t in T), *(ptr(a) for a in F))
+    f['update_electric_dispersive_B'](NX, NY, NZ, 4, P, ptr(disp), ptr(ID), *(ptr(t) for t in T),
+                                      *(ptr(a) for a in F[:3]))
+    assert all(np.array_equal(a, b) for a, b in zip(A, F))
+    assert all(np.array_equal(a, b) for a, b in zip(TA, T))
+
+
+@pytest.mark.parametrize('formulation', ['HORIPML', 'MRIPML'])
+@pytest.mark.parametrize('order', [1, 2])
+def test_pml_slabs_match_reference_kernels(oracle_lib, ref_modules, formulation, order):
+    f = oracle_lib.funcs
+    rng, F, ID, cE, cH = _state(20 + order)
+    t = 5
+    form = 0 if formulation == 'HORIPML' else 1
+    for dname, box in _boxes(t).items():
+        shapes = _phi_shapes(dname, box, order)
+        R = [rng.uniform(0.5, 1.5, (order, t)).astype(np.float32) for _ in range(4)]
+        for field, fname, coeffs, s1, s2 in ((0, 'electric', cE, shapes[0], shapes[1]),
+                                              (1, 'magnetic', cH, shapes[2], shapes[3])):
+            mod = ref_modules[(fname, formulation)]
+            P1 = rng.standard_normal(s1).astype(np.float32)
+            P2 = rng.standard_normal(s2).astype(np.float32)
+            Q1, Q2 = P1.copy(), P2.copy()
+            A = [a.copy() for a in F]
+            getattr(mod, 'order{}_{}'.format(order, dname))(*box, 4, coeffs, ID, *A, Q1, Q2, *R, 0.37)
+            f['pml_update'](field, form, order, DIRECTIONS.index(dname), *box, 4, NX, NY, NZ, ptr(coeffs),
+                            ptr(ID), *(ptr(a) for a in F), ptr(P1), ptr(P2), *(ptr(r) for r in R),
+                            oracle_lib.real(0.37))
+            assert all(np.array_equal(a, b) for a, b in zip(A, F)), (dname, field)
+            assert np.array_equal(P1, Q1) and np.array_equal(P2, Q2), (dname, field)
+
+
+def test_snapshot_average_matches_reference_kernel(oracle_lib, ref_modules):
+    _, F, _, _, _ = _state(4)
+    xs, ys, zs, dx, dy, dz = 2, 3, 1, 2, 1, 3
+    snx, sny, snz = 8, 9, 4
+    out = [np.zeros((snx, sny, snz), np.float32) for _ in range(6)]
+    oracle_lib.funcs['snapshot_fields'](NX, NY, NZ, xs, ys, zs, dx, dy, dz, snx, sny, snz,
+                                        *(ptr(a) for a in F), *(ptr(o) for o in out))
+    sl = (slice(xs, xs + snx * dx + dx, dx), slice(ys, ys + sny * dy + dy, dy), slice(zs, zs + snz * dz + dz, dz))
+    slices = [np.ascontiguousarray(a[sl]) for a in F]
+    ref = [np.zeros((snx, sny, snz), np.float32) for _ in range(6)]
+    ref_modules['snap'].calculate_snapshot_fields(snx, sny, snz, *slices, *ref)
+    assert all(np.array_equal(a, b) for a, b in zip(out, ref))
+
+
+# --------------------------------------------------------- golden traces
+def _trace(G):
+    return np.stack([G.rxs[0].outputs[c] for c in COMPS])
+
+
+def _maxerr(a, b, shift=0):
+    """max |a - b| / max |b|; shift > 0 compares a[n + shift] with b[n]."""
+    if shift > 0:
+        a, b = a[shift:], b[:-shift]
+    elif shift < 0:
+        a, b = a[:shift], b[-shift:]
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope='module')
+def solved():
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            G = BASIC_MODELS[name]()
+            solve_cpu(1, 1, G, nthreads=8)
+            cache[name] = G
+        return cache[name]
+    return get
+
+
+# The reference files predate the current receiver timing: they sample H
+# after its update, one half step later than the current loop does
+# (gprMax/model_build_run.py:424-428 stores before update_magnetic), so H is
+# compared one sample shifted.  Their PML also predates the HORIPML
+# formulation, which leaves a residual of a few 1e-3 where the PML reflection
+# reaches the receiver; inside lossy water (dispersive model) reflections are
+# damped and the traces agree to float32 noise.
+GOLDEN = {
+    'hertzian_dipole_dispersive': (1e-6, (0, 1, 2, 3, 4)),
+    'hertzian_dipole_fs': (3.5e-3, (0, 1, 2, 3, 4)),
+    'hertzian_dipole_hs': (2.5e-3, (0, 1, 2, 3, 4)),
+}
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize('name', list(GOLDEN))
+def test_oracle_matches_reference_traces(solved, name):
+    tol, comps = GOLDEN[name]
+    G = solved(name)
+    ref = np.load('tests/golden/{}_ref.npz'.format(name))['traces']
+    ours = _trace(G)
+    for c in comps:
+        err = _maxerr(ours[c], ref[c], shift=1 if c >= 3 else 0)
+        assert err < tol, (name, COMPS[c], err)
+
+
+@pytest.mark.slow
+def test_oracle_matches_magnetic_dipole_trace(solved):
+    # v3.1.0b1 file: the source is sampled half a step apart from the current
+    # code, a ~1e-2 difference for this pulse width; the field pattern and
+    # polarisation must still agree.
+    G = solved('magnetic_dipole_fs')
+    ref = np.load('tests/golden/magnetic_dipole_fs_ref.npz')['traces']
+    ours = _trace(G)
+    for c in (0, 1, 3, 4, 5):
+        assert _maxerr(ours[c], ref[c], shift=-1 if c < 3 else 0) < 1.5e-2, COMPS[c]
+
+
+@pytest.mark.slow
+def test_oracle_matches_analytical_dipole(solved):
+    G = solved('hertzian_dipole_fs')
+    an = analytical_dipole(G.iterations, G.dt, (G.dx, G.dy, G.dz), (0.02, 0.02, 0.02)).T
+    ours = _trace(G)
+    # ~1% for Ez and ~0.5% for Ex, Ey (docs/source/comparisons_analytical.rst)
+    assert _maxerr(ours[2], an[2]) < 1.1e-2
+    assert _maxerr(ours[0], an[0]) < 5.5e-3
+    assert _maxerr(ours[1], an[1]) < 5.5e-3
+    # ~0.25% for H once sampled on the same half step
+    assert _maxerr(ours[3], an[3], shift=1) < 3e-3
+    assert _maxerr(ours[4], an[4], shift=1) < 3e-3
+
+
+# ----------------------------------------------------------- invariants
+def test_extra_zero_poles_change_nothing():
+    from models import hertzian_dipole_dispersive
+    from gprmax_amd.materials import process_materials
+    G1 = hertzian_dipole_dispersive(n=30, time_window=120, pml_cells=5)
+    G3 = hertzian_dipole_dispersive(n=30, time_window=120, pml_cells=5)
+    # pad the water to three poles of zero weight: coefficients of poles 2, 3 are 0
+    water = next(m for m in G3.materials if m.ID == 'myWater')
+    water.poles = 3
+    water.deltaer = [75.2, 0.0, 0.0]
+    water.tau = [9.231e-12, 1e-9, 1e-9]
+    process_materials(G3)
+    solve_cpu(1, 1, G1, nthreads=4)
+    solve_cpu(1, 1, G3, nthreads=4)
+    assert np.array_equal(_trace(G1), _trace(G3))
+    assert np.all(G3.Tx[1:] == 0) and np.all(G3.Tx.imag == 0)
+
+
+@pytest.mark.slow
+def test_pml_reflection_is_small():
+    # Same source and receiver offset in a 40^3 domain with a 10-cell PML and
+    # in a domain large enough that no reflection returns within the window.
+    from gprmax_amd.grid import ModelBuilder
+
+    def run(n, pml):
+        b = ModelBuilder(n, n, n, 1e-3, 1e-3, 1e-3, 260, pml_cells=pml)
+        b.add_waveform('gaussiandot', 1, 4e9, 'w')
+        c = n // 2
+        b.add_hertzian_dipole('z', (c, c, c), 'w')
+        b.add_rx((c + 6, c + 4, c + 2))
+        G = b.finalise()
+        solve_cpu(1, 1, G, nthreads=8)
+        return _trace(G)
+
+    small = run(40, 10)
+    big = run(200, 0)
+    err = _maxerr(small[2], big[2])
+    assert err < 2e-2, err            # about -40 dB or better
+    wrong = run(40, 0)                # no PML: the PEC box reflects fully
+    assert _maxerr(wrong[2], big[2]) > 5 * err
