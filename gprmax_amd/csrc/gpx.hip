@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -56,9 +57,9 @@ struct SlabArgs {
     int org[3];      // slab origin (xs, ys, zs) for the Phi index
     int d1[3], d2[3];// Phi1/Phi2 dims without the order axis
     int t, order;
+    int roff;        // offset of this slab's [4][order][t] table in the LDS R area
     float d;
     float *phi1, *phi2;
-    const float *R;  // [4][order][t]
 };
 
 struct SrcArgs {
@@ -71,15 +72,18 @@ struct SrcArgs {
 struct StepArgs {
     int nx, ny, nz;
     int plane;           // (ny+1)*(nz+1)
-    int nseg, nblocks, per_xcd, xc;
+    int nseg, nitems, per_xcd, xc;  // work items: nseg plane segments x x-chunks of xc planes
     float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
-    const uint32_t *ID;  // [6][ntot]
+    const void *ID;      // [6][ntot] of the compact index type
     long long ntot;
     const float *coef;   // [nmat][5]
+    const float *R;      // all slabs' R tables for this field, concatenated
+    int nR;
     const float *disp;   // [nmat][3P] (real) or [nmat][3P][2] (complex)
     const uint8_t *dispflag;
     float *T[3];         // [P][ntot] or [P][ntot][2]
     int nmat;
+    int form;            // 0 HORIPML, 1 MRIPML
     int nslab;
     SlabArgs slab[MAX_SLABS];
     int flags;
@@ -88,57 +92,123 @@ struct StepArgs {
     int nrx;
     const int *rxc;      // [nrx][3]
     float *rxbuf;        // [6][iterations][nrx]
+    // Per x-plane flags: bit 0 a receiver, bit 1 a magnetic source, bit 2 an
+    // electric source on the plane; bits 8..13 slabs whose E box holds the
+    // plane, bits 16..21 the same for H.
+    const uint32_t *planeflag;
+    // Cell split.  The interior box [ix0,ix1) x [iy0,iy1) x [iz0,iz1) holds
+    // cells with every neighbour inside the grid, every update applying and no
+    // PML slab; the interior kernel covers it.  The boundary kernel covers
+    // the rest: full planes for x outside [ix0, ix1) (items_a1 items below
+    // ix0, items_a2 above), and for x inside, the plane cells of the frame
+    // list (plane offsets j*(nz+1)+k outside the interior rectangle).
+    int box[6];
+    int nseg_a, nseg_b, items_a1, items_a2;
+    const int *frame;
+    int nframe;
 };
+
+constexpr uint32_t PF_RX = 1, PF_SRCH = 2, PF_SRCE = 4;
+constexpr int PF_SLAB_E = 8, PF_SLAB_H = 16;
+
+// Slabs whose box, for one field, contains lane (j, k) in y and z; the x part
+// comes per plane from the plane flags.
+__device__ __forceinline__ uint32_t lane_slab_mask(const StepArgs &a, int j, int k)
+{
+    uint32_t m = 0;
+    for (int s = 0; s < a.nslab; ++s) {
+        const SlabArgs &sl = a.slab[s];
+        if (j >= sl.lo[1] && j < sl.hi[1] && k >= sl.lo[2] && k < sl.hi[2]) m |= 1u << s;
+    }
+    return m;
+}
+
+// Streamed arrays are read and written with buffer instructions: a scalar
+// resource per array (base address in SGPRs), the lane's byte offset within
+// the plane in a VGPR and the plane's byte offset in an SGPR, so no
+// per-access vector address arithmetic is needed.  Offsets are 32-bit: an
+// array holds fewer than 2^30 floats.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7fffffff, 0x00020000);
+}
+// The b32 buffer builtins move raw 32-bit words; floats pass through as bits.
+__device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+template <class IDT>
+__device__ __forceinline__ uint32_t bld_id(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    if (sizeof(IDT) == 1) return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
+    if (sizeof(IDT) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
+// Loads and stores at a uniform base pointer plus a 32-bit lane byte offset.
+template <class T>
+__device__ __forceinline__ T ld(const T *base, uint32_t boff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void st(T *base, uint32_t boff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
 
 // ---------------------------------------------------------------- PML
 // One corrected component (gprMax/pml_updates/*_ext.pyx).  R = [RA, RB, RE, RF]
-// each [order][t]; idx/off index Phi order 0 and the order-1 offset.
-template <int FORM>
-__device__ __forceinline__ float pml_correct(const SlabArgs &s, int p, float F, bool plus_sign,
-                                             float c, float dd, float *phi, long long idx,
-                                             long long off)
+// each [order][t] in LDS; i1/off index Phi order 0 and the order-1 offset.
+__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
+                                             float F, bool plus_sign, float c, float dd,
+                                             float *phi, uint32_t i1, uint32_t off)
 {
     const int ot = s.order * s.t;
-    const float *R = s.R;
     float corr;
-    if (FORM == 0) {  // HORIPML
+    if (form == 0) {  // HORIPML
         if (s.order == 1) {
-            float RA01 = R[p] - 1.0f;
-            float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            float ph = phi[idx];
+            const float RA01 = R[p] - 1.0f;
+            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
+            const float ph = phi[i1];
             corr = c * (RA01 * dd + RB0 * ph);
-            phi[idx] = RE0 * ph - RF0 * dd;
+            phi[i1] = RE0 * ph - RF0 * dd;
         } else {
-            float RA0 = R[p], RA1 = R[s.t + p];
-            float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
-            float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
-            float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
-            float RA01 = RA0 * RA1 - 1.0f;
-            float p0 = phi[idx], p1 = phi[idx + off];
+            const float RA0 = R[p], RA1 = R[s.t + p];
+            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
+            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
+            const float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
+            const float RA01 = RA0 * RA1 - 1.0f;
+            const float p0 = phi[i1], p1 = phi[i1 + off];
             corr = c * (RA01 * dd + RA1 * RB0 * p0 + RB1 * p1);
-            phi[idx + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
-            phi[idx] = RE0 * p0 - RF0 * dd;
+            phi[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
+            phi[i1] = RE0 * p0 - RF0 * dd;
         }
     } else {  // MRIPML
         if (s.order == 1) {
-            float IRA = 1.0f / R[p];
-            float IRA1 = IRA - 1.0f;
-            float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            float RC0 = IRA * RB0 * RF0;
-            float ph = phi[idx];
+            const float IRA = 1.0f / R[p];
+            const float IRA1 = IRA - 1.0f;
+            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
+            const float RC0 = IRA * RB0 * RF0;
+            const float ph = phi[i1];
             corr = c * (IRA1 * dd - IRA * ph);
-            phi[idx] = RE0 * ph + RC0 * dd - RC0 * ph;
+            phi[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
         } else {
-            float IRA = 1.0f / (R[p] + R[s.t + p]);
-            float IRA1 = IRA - 1.0f;
-            float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
-            float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
-            float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
-            float p0 = phi[idx], p1 = phi[idx + off];
-            float psi = RB0 * p0 + RB1 * p1;
+            const float IRA = 1.0f / (R[p] + R[s.t + p]);
+            const float IRA1 = IRA - 1.0f;
+            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
+            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
+            const float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
+            const float p0 = phi[i1], p1 = phi[i1 + off];
+            const float psi = RB0 * p0 + RB1 * p1;
             corr = c * (IRA1 * dd - IRA * psi);
-            phi[idx + off] = RE1 * p1 + RC1 * (dd - psi);
-            phi[idx] = RE0 * p0 + RC0 * (dd - psi);
+            phi[i1 + off] = RE1 * p1 + RC1 * (dd - psi);
+            phi[i1] = RE0 * p0 + RC0 * (dd - psi);
         }
     }
     return plus_sign ? F + corr : F - corr;
@@ -152,132 +222,282 @@ __device__ __forceinline__ bool in_box(const SlabArgs &s, int i, int j, int k)
 
 // Profile index and the two Phi element indices of cell (i, j, k).
 __device__ __forceinline__ void slab_index(const SlabArgs &s, int i, int j, int k, int &p,
-                                           long long &i1, long long &i2, long long &o1,
-                                           long long &o2)
+                                           uint32_t &i1, uint32_t &i2, uint32_t &o1, uint32_t &o2)
 {
     int c[3] = {i - s.org[0], j - s.org[1], k - s.org[2]};
-    int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
+    const int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
     p = s.minus ? s.pend - along : along - s.pend;
     c[s.axis] = p;
-    i1 = ((long long)c[0] * s.d1[1] + c[1]) * s.d1[2] + c[2];
-    i2 = ((long long)c[0] * s.d2[1] + c[1]) * s.d2[2] + c[2];
-    o1 = (long long)s.d1[0] * s.d1[1] * s.d1[2];
-    o2 = (long long)s.d2[0] * s.d2[1] * s.d2[2];
+    i1 = ((uint32_t)c[0] * s.d1[1] + c[1]) * s.d1[2] + c[2];
+    i2 = ((uint32_t)c[0] * s.d2[1] + c[1]) * s.d2[2] + c[2];
+    o1 = (uint32_t)s.d1[0] * s.d1[1] * s.d1[2];
+    o2 = (uint32_t)s.d2[0] * s.d2[1] * s.d2[2];
 }
 
-// Block -> (plane segment, x chunk) with consecutive segments on one XCD.
+// Persistent grid: blocks are dealt to the 8 XCDs round robin, so block b is
+// renumbered (b mod 8) * per_xcd + b / 8; in each round an XCD then works on a
+// contiguous run of items, i.e. neighbouring plane segments that share rows
+// through its L2.
 __device__ __forceinline__ int logical_block(const StepArgs &a)
 {
-    int b = blockIdx.x;
+    const int b = blockIdx.x;
     return (b & 7) * a.per_xcd + (b >> 3);
 }
 
+// Copy a small table to LDS with four loads in flight per thread.
+// Slab descriptions go to LDS so the PML branch reads them into VGPRs when a
+// lane needs them instead of holding them in SGPRs for the whole loop.
+__device__ __forceinline__ void stage_slabs(SlabArgs *dst, const StepArgs &a)
+{
+    const int words = (int)(sizeof(SlabArgs) / 4) * a.nslab;
+    const int *src = reinterpret_cast<const int *>(a.slab);
+    int *d = reinterpret_cast<int *>(dst);
+    for (int q = threadIdx.x; q < words; q += blockDim.x) d[q] = src[q];
+}
+
+__device__ __forceinline__ void stage(float *dst, const float *src, int n)
+{
+    const int B = blockDim.x;
+    for (int q = threadIdx.x; q < n; q += 4 * B) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = q + u * B < n ? src[q + u * B] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q + u * B < n) dst[q + u * B] = v[u];
+    }
+}
+
+// ---------------------------------------------------------------- work items
+struct Cell {
+    int xa, xb;   // x range of the item
+    int j, k;     // the lane's cell in the plane
+    bool live;
+};
+
+// Map (item, lane) to an x range and a plane cell; see StepArgs for the split.
+template <bool BND>
+__device__ __forceinline__ Cell item_cell(const StepArgs &a, int item)
+{
+    Cell c;
+    const int lane = threadIdx.x, B = blockDim.x;
+    const int NZ1 = a.nz + 1;
+    if (!BND) {
+        const int Kin = a.box[5] - a.box[4];
+        const int ncell = (a.box[3] - a.box[2]) * Kin;
+        const int seg = item % a.nseg, chunk = item / a.nseg;
+        const int t = seg * B + lane;
+        c.xa = a.box[0] + chunk * a.xc;
+        c.xb = min(c.xa + a.xc, a.box[1]);
+        c.live = t < ncell;
+        const int q = c.live ? t : 0;
+        c.j = a.box[2] + q / Kin;
+        c.k = a.box[4] + q % Kin;
+        return c;
+    }
+    int t;
+    if (item < a.items_a1 + a.items_a2) {
+        const bool lower = item < a.items_a1;
+        const int r = lower ? item : item - a.items_a1;
+        const int seg = r % a.nseg_a, chunk = r / a.nseg_a;
+        c.xa = (lower ? 0 : a.box[1]) + chunk * a.xc;
+        c.xb = min(c.xa + a.xc, lower ? a.box[0] : a.nx + 1);
+        t = seg * B + lane;
+        c.live = t < a.plane;
+        if (!c.live) t = 0;
+    } else {
+        const int r = item - a.items_a1 - a.items_a2;
+        const int seg = r % a.nseg_b, chunk = r / a.nseg_b;
+        c.xa = a.box[0] + chunk * a.xc;
+        c.xb = min(c.xa + a.xc, a.box[1]);
+        const int q = seg * B + lane;
+        c.live = q < a.nframe;
+        t = c.live ? a.frame[q] : 0;
+    }
+    c.j = t / NZ1;
+    c.k = t - c.j * NZ1;
+    return c;
+}
+
+// Coefficients in LDS: (CA, CBx, CBy, CBz) or (DA, DBx, DBy, DBz) as one
+// float4 per material, then srce/srcm as one float per material.
+__device__ __forceinline__ void stage_coef(float4 *c4, float *c1, const float *src, int nmat)
+{
+    for (int m = threadIdx.x; m < nmat; m += blockDim.x) {
+        const float *r = src + 5 * m;
+        c4[m] = make_float4(r[0], r[1], r[2], r[3]);
+        c1[m] = r[4];
+    }
+}
+
 // ---------------------------------------------------------------- H pass
-template <int FORM>
+struct HLoad {
+    float ex, ey1, ez1, hx, hy, hz, ezj, exj, eyk, exk;
+    uint32_t m3, m4, m5;
+};
+struct ELoad {
+    float hx, hy, hz, ex, ey, ez, hzj, hxj, hyk, hxk;
+    uint32_t m0, m1, m2;
+};
+
+// update_magnetic (gprMax/fields_updates_ext.pyx:404-412), then each H-PML
+// slab in order, then magnetic dipoles; receivers sample the state at the
+// start of the iteration.
+template <class IDT, bool BND>
 __global__ void k_h_pass(StepArgs a)
 {
-    extern __shared__ float lds[];
-    const int l = logical_block(a);
-    if (l >= a.nblocks) return;
-    const int nc = a.nmat * 5;
-    for (int n = threadIdx.x; n < nc; n += blockDim.x) lds[n] = a.coef[n];
+    extern __shared__ float4 lds4[];
+    __shared__ SlabArgs lslab[MAX_SLABS];
+    const int nm = a.nmat;
+    const float4 *lc = lds4;
+    const float *ls = reinterpret_cast<const float *>(lds4 + nm);
+    const float *lR = ls + nm;
+    stage_coef(lds4, const_cast<float *>(ls), a.coef, nm);
+    if (BND) {
+        stage(const_cast<float *>(lR), a.R, a.nR);
+        stage_slabs(lslab, a);
+    }
     __syncthreads();
 
-    const int seg = l % a.nseg, chunk = l / a.nseg;
-    const int t = seg * blockDim.x + threadIdx.x;
-    if (t >= a.plane) return;
-    const int NZ1 = a.nz + 1;
-    const int j = t / NZ1, k = t - j * NZ1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int i0 = chunk * a.xc;
-    const int i1 = min(i0 + a.xc, nx + 1);
-    const long long P = a.plane, N = a.ntot;
+    const int NZ1 = nz + 1;
+    const size_t N = a.ntot;
     const int flags = a.flags;
-    const uint32_t *ID = a.ID;
+    const IDT *ID = static_cast<const IDT *>(a.ID);
+    const rsrc_t rEx = rsrc(a.Ex), rEy = rsrc(a.Ey), rEz = rsrc(a.Ez);
+    const rsrc_t rHx = rsrc(a.Hx), rHy = rsrc(a.Hy), rHz = rsrc(a.Hz);
+    const rsrc_t rI3 = rsrc(ID + 3 * N), rI4 = rsrc(ID + 4 * N), rI5 = rsrc(ID + 5 * N);
+    const uint32_t P4 = (uint32_t)a.plane * 4u, Pid = (uint32_t)a.plane * (uint32_t)sizeof(IDT);
 
-    long long f = (long long)i0 * P + t;
-    float ey = a.Ey[f], ez = a.Ez[f];
-    for (int i = i0; i < i1; ++i, f += P) {
-        const float ex = a.Ex[f];
-        float ey1 = 0.f, ez1 = 0.f;
-        if (i < nx) {
-            ey1 = a.Ey[f + P];
-            ez1 = a.Ez[f + P];
-        }
-        float hx = a.Hx[f], hy = a.Hy[f], hz = a.Hz[f];
+    for (int item = logical_block(a); item < a.nitems; item += gridDim.x) {
+        const Cell c = item_cell<BND>(a, item);
+        if (!c.live) continue;
+        const int j = c.j, k = c.k;
+        const uint32_t off = (uint32_t)(j * NZ1 + k);
+        const uint32_t to = off * 4u;
+        const uint32_t tid = off * (uint32_t)sizeof(IDT);
+        // A neighbour past the grid edge is clamped to the cell itself, which
+        // makes the difference exactly +0, the value the reference's loop
+        // bounds leave there.  Interior cells always have their neighbours.
+        const uint32_t toj = (!BND || j < ny) ? to + 4u * NZ1 : to;
+        const uint32_t tok = (!BND || k < nz) ? to + 4u : to;
+        const uint32_t jkmask = (BND && (flags & F_PML_ALL)) ? lane_slab_mask(a, j, k) & ((uint32_t)flags >> 4) : 0u;
 
-        if (flags & F_RX) {
-            for (int r = 0; r < a.nrx; ++r) {
-                if (a.rxc[3 * r] == i && a.rxc[3 * r + 1] == j && a.rxc[3 * r + 2] == k) {
-                    float *o = a.rxbuf + (long long)a.it * a.nrx + r;
-                    const long long cs = (long long)a.iterations * a.nrx;
-                    o[0] = ex; o[cs] = ey; o[2 * cs] = ez;
-                    o[3 * cs] = hx; o[4 * cs] = hy; o[5 * cs] = hz;
+        // One plane's loads.  The interior variant issues the next plane's
+        // loads before computing the current one (its register budget allows
+        // two sets in flight); the boundary variant loads as it goes.
+        auto load = [&](int ip, HLoad &L) {
+            const uint32_t so = (uint32_t)ip * P4;
+            const uint32_t sn = (!BND || ip < nx) ? so + P4 : so;  // plane ip+1, or ip itself at the edge
+            const uint32_t si = (uint32_t)ip * Pid;
+            L.ex = bld(rEx, to, so);
+            L.ey1 = bld(rEy, to, sn);
+            L.ez1 = bld(rEz, to, sn);
+            L.hx = bld(rHx, to, so);
+            L.hy = bld(rHy, to, so);
+            L.hz = bld(rHz, to, so);
+            L.ezj = bld(rEz, toj, so);
+            L.exj = bld(rEx, toj, so);
+            L.eyk = bld(rEy, tok, so);
+            L.exk = bld(rEx, tok, so);
+            L.m3 = bld_id<IDT>(rI3, tid, si);
+            L.m4 = bld_id<IDT>(rI4, tid, si);
+            L.m5 = bld_id<IDT>(rI5, tid, si);
+        };
+        float ey = bld(rEy, to, (uint32_t)c.xa * P4), ez = bld(rEz, to, (uint32_t)c.xa * P4);
+        HLoad cur, nxt;
+        if (!BND) load(c.xa, cur);
+        for (int i = c.xa; i < c.xb; ++i) {
+            const uint32_t so = (uint32_t)i * P4;
+            const uint32_t pf = a.planeflag[i];
+            if (BND) load(i, cur);
+            else load(i + 1 < c.xb ? i + 1 : i, nxt);
+            const float ex = cur.ex, ey1 = cur.ey1, ez1 = cur.ez1;
+            const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz;
+            const float ezj = cur.ezj, exj = cur.exj, eyk = cur.eyk, exk = cur.exk;
+            const uint32_t m3 = cur.m3, m4 = cur.m4, m5 = cur.m5;
+
+            // Differences of E used by the H update and the H-PML (forward).
+            const float dEz_y = ezj - ez, dEy_z = eyk - ey, dEx_z = exk - ex;
+            const float dEz_x = ez1 - ez, dEy_x = ey1 - ey, dEx_y = exj - ex;
+            float hx = hx0, hy = hy0, hz = hz0;
+
+            if (flags & F_MAIN) {
+                if (!BND || (i >= 1 && j < ny && k < nz)) {
+                    const float4 q = lc[m3];
+                    hx = q.x * hx - q.z * dEz_y + q.w * dEy_z;
+                }
+                if (!BND || (i < nx && j >= 1 && k < nz)) {
+                    const float4 q = lc[m4];
+                    hy = q.x * hy - q.w * dEx_z + q.y * dEz_x;
+                }
+                if (!BND || (i < nx && j < ny && k >= 1)) {
+                    const float4 q = lc[m5];
+                    hz = q.x * hz - q.y * dEy_x + q.z * dEx_y;
                 }
             }
-        }
 
-        // Differences of E used by the H update and the H-PML (forward).
-        const float dEz_y = (j < ny) ? a.Ez[f + NZ1] - ez : 0.f;
-        const float dEy_z = (k < nz) ? a.Ey[f + 1] - ey : 0.f;
-        const float dEx_z = (k < nz) ? a.Ex[f + 1] - ex : 0.f;
-        const float dEz_x = (i < nx) ? ez1 - ez : 0.f;
-        const float dEy_x = (i < nx) ? ey1 - ey : 0.f;
-        const float dEx_y = (j < ny) ? a.Ex[f + NZ1] - ex : 0.f;
-        const uint32_t m3 = ID[3 * N + f], m4 = ID[4 * N + f], m5 = ID[5 * N + f];
-
-        if (flags & F_MAIN) {  // gprMax/fields_updates_ext.pyx:404-412
-            if (i >= 1 && j < ny && k < nz)
-                hx = lds[m3 * 5] * hx - lds[m3 * 5 + 2] * dEz_y + lds[m3 * 5 + 3] * dEy_z;
-            if (i < nx && j >= 1 && k < nz)
-                hy = lds[m4 * 5] * hy - lds[m4 * 5 + 3] * dEx_z + lds[m4 * 5 + 1] * dEz_x;
-            if (i < nx && j < ny && k >= 1)
-                hz = lds[m5 * 5] * hz - lds[m5 * 5 + 1] * dEy_x + lds[m5 * 5 + 2] * dEx_y;
-        }
-
-        if (flags & F_PML_ALL) {
-            for (int s = 0; s < a.nslab; ++s) {
-                if (!(flags & (F_PML0 << s))) continue;
-                const SlabArgs &sl = a.slab[s];
-                if (!in_box(sl, i, j, k)) continue;
-                int p;
-                long long q1, q2, o1, o2;
-                slab_index(sl, i, j, k, p, q1, q2, o1, o2);
-                if (sl.axis == 0) {
-                    hy = pml_correct<FORM>(sl, p, hy, true, lds[m4 * 5 + 4], dEz_x / sl.d, sl.phi1, q1, o1);
-                    hz = pml_correct<FORM>(sl, p, hz, false, lds[m5 * 5 + 4], dEy_x / sl.d, sl.phi2, q2, o2);
-                } else if (sl.axis == 1) {
-                    hx = pml_correct<FORM>(sl, p, hx, false, lds[m3 * 5 + 4], dEz_y / sl.d, sl.phi1, q1, o1);
-                    hz = pml_correct<FORM>(sl, p, hz, true, lds[m5 * 5 + 4], dEx_y / sl.d, sl.phi2, q2, o2);
-                } else {
-                    hx = pml_correct<FORM>(sl, p, hx, true, lds[m3 * 5 + 4], dEy_z / sl.d, sl.phi1, q1, o1);
-                    hy = pml_correct<FORM>(sl, p, hy, false, lds[m4 * 5 + 4], dEx_z / sl.d, sl.phi2, q2, o2);
+            if (BND) {
+                const uint32_t smask = jkmask & (pf >> PF_SLAB_H);
+                if (smask) {
+                    for (int s = 0; s < a.nslab; ++s) {
+                        if (!(smask & (1u << s))) continue;
+                        const SlabArgs &sl = lslab[s];
+                        int p;
+                        uint32_t q1, q2, o1, o2;
+                        slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                        const float *R = lR + sl.roff;
+                        if (sl.axis == 0) {
+                            hy = pml_correct(a.form, sl, R, p, hy, true, ls[m4], dEz_x / sl.d, sl.phi1, q1, o1);
+                            hz = pml_correct(a.form, sl, R, p, hz, false, ls[m5], dEy_x / sl.d, sl.phi2, q2, o2);
+                        } else if (sl.axis == 1) {
+                            hx = pml_correct(a.form, sl, R, p, hx, false, ls[m3], dEz_y / sl.d, sl.phi1, q1, o1);
+                            hz = pml_correct(a.form, sl, R, p, hz, true, ls[m5], dEx_y / sl.d, sl.phi2, q2, o2);
+                        } else {
+                            hx = pml_correct(a.form, sl, R, p, hx, true, ls[m3], dEy_z / sl.d, sl.phi1, q1, o1);
+                            hy = pml_correct(a.form, sl, R, p, hy, false, ls[m4], dEx_z / sl.d, sl.phi2, q2, o2);
+                        }
+                    }
                 }
             }
-        }
 
-        if (flags & F_SRC) {  // magnetic dipoles, gprMax/sources.py:202-232
-            for (int s = 0; s < a.src0.n; ++s) {
-                const int *in = a.src0.info + 6 * s;
-                if (in[0] == i && in[1] == j && in[2] == k && a.it >= in[4] && a.it <= in[5]) {
-                    const int pol = in[3];
-                    const uint32_t m = pol == 0 ? m3 : (pol == 1 ? m4 : m5);
-                    const float w = a.src0.waves[(long long)s * a.iterations + a.it];
-                    const float v = lds[m * 5 + 4] * w * a.src0.fac[2 * s];
-                    if (pol == 0) hx = hx - v;
-                    else if (pol == 1) hy = hy - v;
-                    else hz = hz - v;
+            if ((flags & F_SRC) && (pf & PF_SRCH)) {  // magnetic dipoles, gprMax/sources.py:202-232
+                for (int s = 0; s < a.src0.n; ++s) {
+                    const int *in = a.src0.info + 6 * s;
+                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[1] == j && in[2] == k) {
+                        const int pol = in[3];
+                        const uint32_t m = pol == 0 ? m3 : (pol == 1 ? m4 : m5);
+                        const float w = a.src0.waves[(size_t)s * a.iterations + a.it];
+                        const float v = ls[m] * w * a.src0.fac[2 * s];
+                        if (pol == 0) hx = hx - v;
+                        else if (pol == 1) hy = hy - v;
+                        else hz = hz - v;
+                    }
                 }
             }
-        }
 
-        if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
-            a.Hx[f] = hx;
-            a.Hy[f] = hy;
-            a.Hz[f] = hz;
+            if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
+                bst(rHx, to, so, hx);
+                bst(rHy, to, so, hy);
+                bst(rHz, to, so, hz);
+            }
+
+            if ((flags & F_RX) && (pf & PF_RX)) {  // the state at the start of the iteration
+                for (int r = 0; r < a.nrx; ++r) {
+                    if (a.rxc[3 * r] != i) continue;
+                    if (a.rxc[3 * r + 1] == j && a.rxc[3 * r + 2] == k) {
+                        float *o = a.rxbuf + (size_t)a.it * a.nrx + r;
+                        const size_t cs = (size_t)a.iterations * a.nrx;
+                        o[0] = ex; o[cs] = ey; o[2 * cs] = ez;
+                        o[3 * cs] = hx0; o[4 * cs] = hy0; o[5 * cs] = hz0;
+                    }
+                }
+            }
+            ey = ey1;
+            ez = ez1;
+            if (!BND) cur = nxt;
         }
-        ey = ey1;
-        ez = ez1;
     }
 }
 
@@ -291,7 +511,7 @@ __device__ __forceinline__ float disp_a(const float *D, uint32_t m, float *tr, f
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (CPLX) {
-            const float *c = D + ((long long)m * 3 * P + 3 * p) * 2;
+            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
             if (P == 1) phi = c[0] * tr[p];
             else phi = phi + c[0] * tr[p];
             const float er = c[2], ei = c[3], zr = c[4], zi = c[5];
@@ -300,7 +520,7 @@ __device__ __forceinline__ float disp_a(const float *D, uint32_t m, float *tr, f
             tr[p] = nr;
             ti[p] = ni;
         } else {
-            const float *c = D + (long long)m * 3 * P + 3 * p;
+            const float *c = D + (size_t)m * 3 * P + 3 * p;
             if (P == 1) phi = c[0] * tr[p];
             else phi = phi + c[0] * tr[p];
             tr[p] = c[1] * tr[p] + c[2] * e;
@@ -316,201 +536,246 @@ __device__ __forceinline__ void disp_b(const float *D, uint32_t m, float *tr, fl
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (CPLX) {
-            const float *c = D + ((long long)m * 3 * P + 3 * p) * 2;
+            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
             tr[p] = tr[p] - c[4] * e;
             ti[p] = ti[p] - c[5] * e;
         } else {
-            tr[p] = tr[p] - D[(long long)m * 3 * P + 3 * p + 2] * e;
+            tr[p] = tr[p] - D[(size_t)m * 3 * P + 3 * p + 2] * e;
         }
     }
 }
 
 template <int P, bool CPLX>
-__device__ __forceinline__ void t_load(float *const *T, int c, long long N, long long f, float *tr,
-                                       float *ti)
+__device__ __forceinline__ void t_load(float *const *T, int c, size_t N, size_t po, uint32_t to,
+                                       float *tr, float *ti)
 {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (CPLX) {
-            const float2 v = reinterpret_cast<const float2 *>(T[c])[(long long)p * N + f];
+            const float2 v = ld(reinterpret_cast<const float2 *>(T[c]) + p * N + po, 2 * to);
             tr[p] = v.x;
             ti[p] = v.y;
         } else {
-            tr[p] = T[c][(long long)p * N + f];
+            tr[p] = ld((const float *)T[c] + p * N + po, to);
             ti[p] = 0.f;
         }
     }
 }
 
 template <int P, bool CPLX>
-__device__ __forceinline__ void t_store(float *const *T, int c, long long N, long long f,
+__device__ __forceinline__ void t_store(float *const *T, int c, size_t N, size_t po, uint32_t to,
                                         const float *tr, const float *ti)
 {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (CPLX)
-            reinterpret_cast<float2 *>(T[c])[(long long)p * N + f] = make_float2(tr[p], ti[p]);
+            st(reinterpret_cast<float2 *>(T[c]) + p * N + po, 2 * to, make_float2(tr[p], ti[p]));
         else
-            T[c][(long long)p * N + f] = tr[p];
+            st(T[c] + p * N + po, to, tr[p]);
     }
 }
 
-template <int FORM, int P, bool CPLX>
+// update_electric or dispersive part A (gprMax/fields_updates_ext.pyx:81-107,
+// 146-179), each E-PML slab in order, voltage sources, Hertzian dipoles and
+// dispersive part B, all on the cell's own thread.
+template <class IDT, int P, bool CPLX, bool BND>
 __global__ void k_e_pass(StepArgs a)
 {
-    extern __shared__ float lds[];
-    const int l = logical_block(a);
-    if (l >= a.nblocks) return;
-    const int nc = a.nmat * 5;
-    const int nd = P > 0 ? a.nmat * 3 * P * (CPLX ? 2 : 1) : 0;
-    float *ldisp = lds + nc;
-    uint8_t *lflag = reinterpret_cast<uint8_t *>(ldisp + nd);
-    for (int n = threadIdx.x; n < nc; n += blockDim.x) lds[n] = a.coef[n];
+    extern __shared__ float4 lds4[];
+    __shared__ SlabArgs lslab[MAX_SLABS];
+    const int nm = a.nmat;
+    const int nd = P > 0 ? nm * 3 * P * (CPLX ? 2 : 1) : 0;
+    const float4 *lc = lds4;
+    const float *ls = reinterpret_cast<const float *>(lds4 + nm);
+    const float *lR = ls + nm;
+    const float *ldisp = lR + a.nR;
+    uint8_t *lflag = reinterpret_cast<uint8_t *>(const_cast<float *>(ldisp + nd));
+    stage_coef(lds4, const_cast<float *>(ls), a.coef, nm);
+    if (BND) {
+        stage(const_cast<float *>(lR), a.R, a.nR);
+        stage_slabs(lslab, a);
+    }
     if (P > 0) {
-        for (int n = threadIdx.x; n < nd; n += blockDim.x) ldisp[n] = a.disp[n];
-        for (int n = threadIdx.x; n < a.nmat; n += blockDim.x) lflag[n] = a.dispflag[n];
+        stage(const_cast<float *>(ldisp), a.disp, nd);
+        for (int q = threadIdx.x; q < nm; q += blockDim.x) lflag[q] = a.dispflag[q];
     }
     __syncthreads();
 
-    const int seg = l % a.nseg, chunk = l / a.nseg;
-    const int t = seg * blockDim.x + threadIdx.x;
-    if (t >= a.plane) return;
-    const int NZ1 = a.nz + 1;
-    const int j = t / NZ1, k = t - j * NZ1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int i0 = chunk * a.xc;
-    const int i1 = min(i0 + a.xc, nx + 1);
-    const long long Pl = a.plane, N = a.ntot;
+    const int NZ1 = nz + 1;
+    const size_t Pl = a.plane, N = a.ntot;
     const int flags = a.flags;
-    const uint32_t *ID = a.ID;
+    const IDT *ID = static_cast<const IDT *>(a.ID);
+    const rsrc_t rEx = rsrc(a.Ex), rEy = rsrc(a.Ey), rEz = rsrc(a.Ez);
+    const rsrc_t rHx = rsrc(a.Hx), rHy = rsrc(a.Hy), rHz = rsrc(a.Hz);
+    const rsrc_t rI0 = rsrc(ID), rI1 = rsrc(ID + N), rI2 = rsrc(ID + 2 * N);
+    const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
 
-    long long f = (long long)i0 * Pl + t;
-    float hy_m = 0.f, hz_m = 0.f;  // Hy, Hz of plane i-1
-    if (i0 >= 1) {
-        hy_m = a.Hy[f - Pl];
-        hz_m = a.Hz[f - Pl];
-    }
-    for (int i = i0; i < i1; ++i, f += Pl) {
-        const float hx = a.Hx[f], hy = a.Hy[f], hz = a.Hz[f];
-        float ex = a.Ex[f], ey = a.Ey[f], ez = a.Ez[f];
-        // Differences of H used by the E update and the E-PML (backward).
-        const float dHz_y = (j >= 1) ? hz - a.Hz[f - NZ1] : 0.f;
-        const float dHy_z = (k >= 1) ? hy - a.Hy[f - 1] : 0.f;
-        const float dHx_z = (k >= 1) ? hx - a.Hx[f - 1] : 0.f;
-        const float dHz_x = (i >= 1) ? hz - hz_m : 0.f;
-        const float dHy_x = (i >= 1) ? hy - hy_m : 0.f;
-        const float dHx_y = (j >= 1) ? hx - a.Hx[f - NZ1] : 0.f;
-        const uint32_t m0 = ID[f], m1 = ID[N + f], m2 = ID[2 * N + f];
+    for (int item = logical_block(a); item < a.nitems; item += gridDim.x) {
+        const Cell c = item_cell<BND>(a, item);
+        if (!c.live) continue;
+        const int j = c.j, k = c.k;
+        const uint32_t off = (uint32_t)(j * NZ1 + k);
+        const uint32_t to = off * 4u;
+        const uint32_t tid = off * (uint32_t)sizeof(IDT);
+        const uint32_t toj = (!BND || j >= 1) ? to - 4u * NZ1 : to;
+        const uint32_t tok = (!BND || k >= 1) ? to - 4u : to;
+        const uint32_t jkmask = (BND && (flags & F_PML_ALL)) ? lane_slab_mask(a, j, k) & ((uint32_t)flags >> 4) : 0u;
 
-        const bool rx_ = i < nx && j >= 1 && j < ny && k >= 1 && k < nz;
-        const bool ry_ = i >= 1 && i < nx && j < ny && k >= 1 && k < nz;
-        const bool rz_ = i >= 1 && i < nx && j >= 1 && j < ny && k < nz;
-        float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
-        float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
-        float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
-        bool dx_ = false, dy_ = false, dz_ = false;
-        if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-            dx_ = rx_ && lflag[m0];
-            dy_ = ry_ && lflag[m1];
-            dz_ = rz_ && lflag[m2];
-            if (dx_) t_load<P, CPLX>(a.T, 0, N, f, txr, txi);
-            if (dy_) t_load<P, CPLX>(a.T, 1, N, f, tyr, tyi);
-            if (dz_) t_load<P, CPLX>(a.T, 2, N, f, tzr, tzi);
-        }
+        // Hy, Hz of plane i-1 (the plane itself at i = 0, giving a +0 difference)
+        const uint32_t sm = (uint32_t)(c.xa >= 1 ? c.xa - 1 : c.xa) * P4;
+        float hy_m = bld(rHy, to, sm), hz_m = bld(rHz, to, sm);
+        auto load = [&](int ip, ELoad &L) {
+            const uint32_t so = (uint32_t)ip * P4, si = (uint32_t)ip * Pid;
+            L.hx = bld(rHx, to, so);
+            L.hy = bld(rHy, to, so);
+            L.hz = bld(rHz, to, so);
+            L.ex = bld(rEx, to, so);
+            L.ey = bld(rEy, to, so);
+            L.ez = bld(rEz, to, so);
+            L.hzj = bld(rHz, toj, so);
+            L.hxj = bld(rHx, toj, so);
+            L.hyk = bld(rHy, tok, so);
+            L.hxk = bld(rHx, tok, so);
+            L.m0 = bld_id<IDT>(rI0, tid, si);
+            L.m1 = bld_id<IDT>(rI1, tid, si);
+            L.m2 = bld_id<IDT>(rI2, tid, si);
+        };
+        ELoad cur, nxt;
+        if (!BND && P == 0) load(c.xa, cur);
+        for (int i = c.xa; i < c.xb; ++i) {
+            const uint32_t so = (uint32_t)i * P4;
+            const size_t po = (size_t)i * Pl;
+            const uint32_t pf = a.planeflag[i];
+            if (BND || P > 0) load(i, cur);
+            else load(i + 1 < c.xb ? i + 1 : i, nxt);
+            const float hx = cur.hx, hy = cur.hy, hz = cur.hz;
+            float ex = cur.ex, ey = cur.ey, ez = cur.ez;
+            const float hzj = cur.hzj, hxj = cur.hxj, hyk = cur.hyk, hxk = cur.hxk;
+            const uint32_t m0 = cur.m0, m1 = cur.m1, m2 = cur.m2;
+            float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
+            float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
+            float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
+            if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
+                t_load<P, CPLX>(a.T, 0, N, po, to, txr, txi);
+                t_load<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
+                t_load<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
+            }
+            // Differences of H used by the E update and the E-PML (backward).
+            const float dHz_y = hz - hzj, dHy_z = hy - hyk, dHx_z = hx - hxk;
+            const float dHz_x = hz - hz_m, dHy_x = hy - hy_m, dHx_y = hx - hxj;
 
-        if (flags & F_MAIN) {  // gprMax/fields_updates_ext.pyx:81-107 and 146-179
-            if (rx_) {
-                float v = lds[m0 * 5] * ex + lds[m0 * 5 + 2] * dHz_y - lds[m0 * 5 + 3] * dHy_z;
-                if (P > 0) {
-                    const float phi = dx_ ? disp_a<P, CPLX>(ldisp, m0, txr, txi, ex) : 0.f;
-                    v = v - lds[m0 * 5 + 4] * phi;
-                }
-                ex = v;
-            }
-            if (ry_) {
-                float v = lds[m1 * 5] * ey + lds[m1 * 5 + 3] * dHx_z - lds[m1 * 5 + 1] * dHz_x;
-                if (P > 0) {
-                    const float phi = dy_ ? disp_a<P, CPLX>(ldisp, m1, tyr, tyi, ey) : 0.f;
-                    v = v - lds[m1 * 5 + 4] * phi;
-                }
-                ey = v;
-            }
-            if (rz_) {
-                float v = lds[m2 * 5] * ez + lds[m2 * 5 + 1] * dHy_x - lds[m2 * 5 + 2] * dHx_y;
-                if (P > 0) {
-                    const float phi = dz_ ? disp_a<P, CPLX>(ldisp, m2, tzr, tzi, ez) : 0.f;
-                    v = v - lds[m2 * 5 + 4] * phi;
-                }
-                ez = v;
-            }
-        }
+            const bool rx_ = !BND || (i < nx && j >= 1 && j < ny && k >= 1 && k < nz);
+            const bool ry_ = !BND || (i >= 1 && i < nx && j < ny && k >= 1 && k < nz);
+            const bool rz_ = !BND || (i >= 1 && i < nx && j >= 1 && j < ny && k < nz);
+            // Auxiliary values change only on edges of materials with poles.
+            const bool dx_ = P > 0 && rx_ && lflag[m0];
+            const bool dy_ = P > 0 && ry_ && lflag[m1];
+            const bool dz_ = P > 0 && rz_ && lflag[m2];
 
-        if (flags & F_PML_ALL) {
-            for (int s = 0; s < a.nslab; ++s) {
-                if (!(flags & (F_PML0 << s))) continue;
-                const SlabArgs &sl = a.slab[s];
-                if (!in_box(sl, i, j, k)) continue;
-                int p;
-                long long q1, q2, o1, o2;
-                slab_index(sl, i, j, k, p, q1, q2, o1, o2);
-                if (sl.axis == 0) {
-                    ey = pml_correct<FORM>(sl, p, ey, false, lds[m1 * 5 + 4], dHz_x / sl.d, sl.phi1, q1, o1);
-                    ez = pml_correct<FORM>(sl, p, ez, true, lds[m2 * 5 + 4], dHy_x / sl.d, sl.phi2, q2, o2);
-                } else if (sl.axis == 1) {
-                    ex = pml_correct<FORM>(sl, p, ex, true, lds[m0 * 5 + 4], dHz_y / sl.d, sl.phi1, q1, o1);
-                    ez = pml_correct<FORM>(sl, p, ez, false, lds[m2 * 5 + 4], dHx_y / sl.d, sl.phi2, q2, o2);
-                } else {
-                    ex = pml_correct<FORM>(sl, p, ex, false, lds[m0 * 5 + 4], dHy_z / sl.d, sl.phi1, q1, o1);
-                    ey = pml_correct<FORM>(sl, p, ey, true, lds[m1 * 5 + 4], dHx_z / sl.d, sl.phi2, q2, o2);
+            if (flags & F_MAIN) {
+                if (rx_) {
+                    const float4 q = lc[m0];
+                    float v = q.x * ex + q.z * dHz_y - q.w * dHy_z;
+                    if (P > 0) {
+                        const float phi = dx_ ? disp_a<P, CPLX>(ldisp, m0, txr, txi, ex) : 0.f;
+                        v = v - ls[m0] * phi;
+                    }
+                    ex = v;
                 }
-            }
-        }
-
-        if (flags & F_SRC) {
-            // Voltage sources, then Hertzian dipoles (gprMax/model_build_run.py:458-461).
-            for (int s = 0; s < a.src0.n; ++s) {  // gprMax/sources.py:81-120
-                const int *in = a.src0.info + 6 * s;
-                if (in[0] == i && in[1] == j && in[2] == k && a.it >= in[4] && a.it <= in[5]) {
-                    const int pol = in[3] & 3;
-                    const bool hard = in[3] >= 4;
-                    const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
-                    const float w = a.src0.waves[(long long)s * a.iterations + a.it];
-                    float *F = pol == 0 ? &ex : (pol == 1 ? &ey : &ez);
-                    if (hard) *F = -w / a.src0.fac[2 * s];
-                    else *F = *F - lds[m * 5 + 4] * w * a.src0.fac[2 * s];
+                if (ry_) {
+                    const float4 q = lc[m1];
+                    float v = q.x * ey + q.w * dHx_z - q.y * dHz_x;
+                    if (P > 0) {
+                        const float phi = dy_ ? disp_a<P, CPLX>(ldisp, m1, tyr, tyi, ey) : 0.f;
+                        v = v - ls[m1] * phi;
+                    }
+                    ey = v;
                 }
-            }
-            for (int s = 0; s < a.src1.n; ++s) {  // gprMax/sources.py:163-193
-                const int *in = a.src1.info + 6 * s;
-                if (in[0] == i && in[1] == j && in[2] == k && a.it >= in[4] && a.it <= in[5]) {
-                    const int pol = in[3];
-                    const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
-                    const float w = a.src1.waves[(long long)s * a.iterations + a.it];
-                    const float v = lds[m * 5 + 4] * w * a.src1.fac[2 * s] * a.src1.fac[2 * s + 1];
-                    if (pol == 0) ex = ex - v;
-                    else if (pol == 1) ey = ey - v;
-                    else ez = ez - v;
+                if (rz_) {
+                    const float4 q = lc[m2];
+                    float v = q.x * ez + q.y * dHy_x - q.z * dHx_y;
+                    if (P > 0) {
+                        const float phi = dz_ ? disp_a<P, CPLX>(ldisp, m2, tzr, tzi, ez) : 0.f;
+                        v = v - ls[m2] * phi;
+                    }
+                    ez = v;
                 }
             }
-        }
 
-        if (P > 0 && (flags & F_DISPB)) {
-            if (dx_) disp_b<P, CPLX>(ldisp, m0, txr, txi, ex);
-            if (dy_) disp_b<P, CPLX>(ldisp, m1, tyr, tyi, ey);
-            if (dz_) disp_b<P, CPLX>(ldisp, m2, tzr, tzi, ez);
+            if (BND) {
+                const uint32_t smask = jkmask & (pf >> PF_SLAB_E);
+                if (smask) {
+                    for (int s = 0; s < a.nslab; ++s) {
+                        if (!(smask & (1u << s))) continue;
+                        const SlabArgs &sl = lslab[s];
+                        int p;
+                        uint32_t q1, q2, o1, o2;
+                        slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                        const float *R = lR + sl.roff;
+                        if (sl.axis == 0) {
+                            ey = pml_correct(a.form, sl, R, p, ey, false, ls[m1], dHz_x / sl.d, sl.phi1, q1, o1);
+                            ez = pml_correct(a.form, sl, R, p, ez, true, ls[m2], dHy_x / sl.d, sl.phi2, q2, o2);
+                        } else if (sl.axis == 1) {
+                            ex = pml_correct(a.form, sl, R, p, ex, true, ls[m0], dHz_y / sl.d, sl.phi1, q1, o1);
+                            ez = pml_correct(a.form, sl, R, p, ez, false, ls[m2], dHx_y / sl.d, sl.phi2, q2, o2);
+                        } else {
+                            ex = pml_correct(a.form, sl, R, p, ex, false, ls[m0], dHy_z / sl.d, sl.phi1, q1, o1);
+                            ey = pml_correct(a.form, sl, R, p, ey, true, ls[m1], dHx_z / sl.d, sl.phi2, q2, o2);
+                        }
+                    }
+                }
+            }
+
+            if ((flags & F_SRC) && (pf & PF_SRCE)) {
+                // Voltage sources, then Hertzian dipoles (gprMax/model_build_run.py:458-461).
+                for (int s = 0; s < a.src0.n; ++s) {  // gprMax/sources.py:81-120
+                    const int *in = a.src0.info + 6 * s;
+                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[1] == j && in[2] == k) {
+                        const int pol = in[3] & 3;
+                        const bool hard = in[3] >= 4;
+                        const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
+                        const float w = a.src0.waves[(size_t)s * a.iterations + a.it];
+                        float *F = pol == 0 ? &ex : (pol == 1 ? &ey : &ez);
+                        if (hard) *F = -w / a.src0.fac[2 * s];
+                        else *F = *F - ls[m] * w * a.src0.fac[2 * s];
+                    }
+                }
+                for (int s = 0; s < a.src1.n; ++s) {  // gprMax/sources.py:163-193
+                    const int *in = a.src1.info + 6 * s;
+                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[1] == j && in[2] == k) {
+                        const int pol = in[3];
+                        const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
+                        const float w = a.src1.waves[(size_t)s * a.iterations + a.it];
+                        const float v = ls[m] * w * a.src1.fac[2 * s] * a.src1.fac[2 * s + 1];
+                        if (pol == 0) ex = ex - v;
+                        else if (pol == 1) ey = ey - v;
+                        else ez = ez - v;
+                    }
+                }
+            }
+
+            if (P > 0 && (flags & F_DISPB)) {
+                if (dx_) disp_b<P, CPLX>(ldisp, m0, txr, txi, ex);
+                if (dy_) disp_b<P, CPLX>(ldisp, m1, tyr, tyi, ey);
+                if (dz_) disp_b<P, CPLX>(ldisp, m2, tzr, tzi, ez);
+            }
+            if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
+                if (dx_) t_store<P, CPLX>(a.T, 0, N, po, to, txr, txi);
+                if (dy_) t_store<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
+                if (dz_) t_store<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
+            }
+            if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
+                bst(rEx, to, so, ex);
+                bst(rEy, to, so, ey);
+                bst(rEz, to, so, ez);
+            }
+            hy_m = hy;
+            hz_m = hz;
+            if (!BND && P == 0) cur = nxt;
         }
-        if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-            if (dx_) t_store<P, CPLX>(a.T, 0, N, f, txr, txi);
-            if (dy_) t_store<P, CPLX>(a.T, 1, N, f, tyr, tyi);
-            if (dz_) t_store<P, CPLX>(a.T, 2, N, f, tzr, tzi);
-        }
-        if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
-            a.Ex[f] = ex;
-            a.Ey[f] = ey;
-            a.Ez[f] = ez;
-        }
-        hy_m = hy;
-        hz_m = hz;
     }
 }
 
@@ -544,7 +809,7 @@ __global__ void k_snapshot(int ny, int nz, int xs, int ys, int zs, int sdx, int 
 
 // ====================================================================== ctx
 struct Slab {
-    int dir, xs, xf, ys, yf, zs, zf, order, t;
+    int dir, xs, xf, ys, yf, zs, zf, order, t, roff = 0;
     float d;
     float *phi[4] = {nullptr, nullptr, nullptr, nullptr};  // EPhi1, EPhi2, HPhi1, HPhi2
     long long dims[4][3];
@@ -558,6 +823,7 @@ struct Snap {
 
 struct SrcSet {
     int n = 0;
+    std::vector<int> host;  // info rows, for the plane flags
     int *info = nullptr;
     float *fac = nullptr, *waves = nullptr;
 };
@@ -568,8 +834,11 @@ struct gpx_ctx {
     int plane = 0;
     hipStream_t stream = nullptr;
     float *F[6] = {nullptr};  // Ex, Ey, Ez, Hx, Hy, Hz
-    uint32_t *ID = nullptr;
+    void *ID = nullptr;       // [6][ntot] as uint8, uint16 or uint32
+    int id_bytes = 4;         // smallest width that holds every index < nmat
     float *cE = nullptr, *cH = nullptr;
+    float *Rall[2] = {nullptr, nullptr};  // E, H: all slabs' R tables back to back
+    std::vector<float> Rhost[2];
     int disp_complex = 0;
     float *disp = nullptr;    // real [nmat][3P] or complex [nmat][3P][2]
     uint8_t *dispflag = nullptr;
@@ -578,15 +847,23 @@ struct gpx_ctx {
     std::vector<Slab> slabs;
     SrcSet src[3];
     int nrx = 0;
+    std::vector<int> rxhost;
     int *rxc = nullptr;
     float *rxbuf = nullptr;
     std::vector<Snap> snaps;
+    uint32_t *planeflag = nullptr;  // [nx+1]
+    int box[6] = {0, 0, 0, 0, 0, 0}; // interior box (see StepArgs)
+    int *frame = nullptr;            // plane offsets outside the interior rectangle
+    int nframe = 0;
     int xchunk = 0, block = 0;
+    int cus = 0;                     // compute units of the device
+    mutable std::map<std::pair<const void *, size_t>, int> occupancy;
     size_t bytes = 0;
     std::string err;
 };
 
 static std::string g_err;
+static int update_planeflags(gpx_ctx *ctx);
 
 #define GPX_TRY(call)                                                                          \
     do {                                                                                       \
@@ -641,6 +918,15 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
     if (e != hipSuccess) return fail(nullptr, (int)e, std::string("hipSetDevice: ") + hipGetErrorString(e));
     ctx = new gpx_ctx();
     ctx->desc = *d;
+    {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, d->device);
+        if (e != hipSuccess) {
+            delete ctx;
+            return fail(nullptr, (int)e, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+        }
+        ctx->cus = prop.multiProcessorCount;
+    }
     ctx->plane = (d->ny + 1) * (d->nz + 1);
     ctx->ntot = (long long)(d->nx + 1) * ctx->plane;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -654,12 +940,19 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
         e = dalloc(ctx, (void **)&ctx->F[c], fb);
         if (e != hipSuccess) goto fail_alloc;
     }
-    e = dalloc(ctx, (void **)&ctx->ID, 6 * fb, false);
+    ctx->id_bytes = d->nmat <= 256 ? 1 : (d->nmat <= 65536 ? 2 : 4);
+    e = dalloc(ctx, (void **)&ctx->ID, 6 * (size_t)ctx->ntot * ctx->id_bytes, false);
     if (e != hipSuccess) goto fail_alloc;
     e = dalloc(ctx, (void **)&ctx->cE, (size_t)d->nmat * 5 * 4);
     if (e != hipSuccess) goto fail_alloc;
     e = dalloc(ctx, (void **)&ctx->cH, (size_t)d->nmat * 5 * 4);
     if (e != hipSuccess) goto fail_alloc;
+    e = dalloc(ctx, (void **)&ctx->planeflag, (size_t)(d->nx + 1) * 4);
+    if (e != hipSuccess) goto fail_alloc;
+    if (int rc = update_planeflags(ctx)) {
+        gpx_destroy(ctx);
+        return fail(nullptr, rc, "gpx_create: layout set-up failed");
+    }
     *out = ctx;
     return 0;
 fail_alloc : {
@@ -679,6 +972,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->ID) hipFree(ctx->ID);
     if (ctx->cE) hipFree(ctx->cE);
     if (ctx->cH) hipFree(ctx->cH);
+    for (auto &p : ctx->Rall) if (p) hipFree(p);
     if (ctx->disp) hipFree(ctx->disp);
     if (ctx->dispflag) hipFree(ctx->dispflag);
     for (auto &s : ctx->slabs) {
@@ -693,6 +987,8 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     }
     if (ctx->rxc) hipFree(ctx->rxc);
     if (ctx->rxbuf) hipFree(ctx->rxbuf);
+    if (ctx->planeflag) hipFree(ctx->planeflag);
+    if (ctx->frame) hipFree(ctx->frame);
     for (auto &s : ctx->snaps) if (s.out) hipFree(s.out);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -725,7 +1021,16 @@ extern "C" int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID)
             return fail(ctx, GPX_EARG, "gpx_upload_id: material index " + std::to_string(ID[q]) +
                                            " >= nmat " + std::to_string(nmat));
     hipSetDevice(ctx->desc.device);
-    int rc = copy_in(ctx, ctx->ID, ID, n * 4);
+    int rc;
+    if (ctx->id_bytes == 4) {
+        rc = copy_in(ctx, ctx->ID, ID, n * 4);
+    } else if (ctx->id_bytes == 2) {
+        std::vector<uint16_t> c(ID, ID + n);
+        rc = copy_in(ctx, ctx->ID, c.data(), n * 2);
+    } else {
+        std::vector<uint8_t> c(ID, ID + n);
+        rc = copy_in(ctx, ctx->ID, c.data(), n);
+    }
     if (!rc) ctx->have_id = true;
     return rc;
 }
@@ -825,12 +1130,30 @@ extern "C" int gpx_add_pml(gpx_ctx *ctx, int dir, int xs, int xf, int ys, int yf
     const size_t rb = (size_t)4 * order * t * 4;
     GPX_TRY(dalloc(ctx, (void **)&s.eR, rb, false));
     GPX_TRY(dalloc(ctx, (void **)&s.hR, rb, false));
+    s.roff = (int)ctx->Rhost[0].size();
+    const size_t need = (size_t)g.nmat * 5 * 4 + (ctx->Rhost[0].size() + rb / 4) * 4 +
+                        (size_t)g.nmat * 3 * g.maxpoles * 2 * 4 + g.nmat;
+    if (need > (size_t)MAX_LDS_BYTES) {
+        hipFree(s.eR);
+        hipFree(s.hR);
+        for (auto &p : s.phi) hipFree(p);
+        return fail(ctx, GPX_ELIMIT, "gpx_add_pml: coefficient tables exceed the LDS budget");
+    }
     ctx->slabs.push_back(s);
     Slab &ref = ctx->slabs.back();
     int rc = copy_in(ctx, ref.eR, eR, rb);
     if (!rc) rc = copy_in(ctx, ref.hR, hR, rb);
+    if (rc) return rc;
+    ctx->Rhost[0].insert(ctx->Rhost[0].end(), eR, eR + rb / 4);
+    ctx->Rhost[1].insert(ctx->Rhost[1].end(), hR, hR + rb / 4);
+    for (int f = 0; f < 2; ++f) {
+        if (ctx->Rall[f]) hipFree(ctx->Rall[f]);
+        ctx->Rall[f] = nullptr;
+        GPX_TRY(dalloc(ctx, (void **)&ctx->Rall[f], ctx->Rhost[f].size() * 4, false));
+        if ((rc = copy_in(ctx, ctx->Rall[f], ctx->Rhost[f].data(), ctx->Rhost[f].size() * 4))) return rc;
+    }
     if (slab) *slab = (int)ctx->slabs.size() - 1;
-    return rc;
+    return update_planeflags(ctx);
 }
 
 extern "C" int gpx_set_sources(gpx_ctx *ctx, int kind, int n, const int32_t *info,
@@ -852,13 +1175,16 @@ extern "C" int gpx_set_sources(gpx_ctx *ctx, int kind, int n, const int32_t *inf
     if (S.fac) { hipFree(S.fac); S.fac = nullptr; }
     if (S.waves) { hipFree(S.waves); S.waves = nullptr; }
     S.n = n;
-    if (n == 0) return 0;
+    S.host.clear();
+    if (n == 0) return update_planeflags(ctx);
     GPX_TRY(dalloc(ctx, (void **)&S.info, (size_t)n * 6 * 4, false));
     GPX_TRY(dalloc(ctx, (void **)&S.fac, (size_t)n * 2 * 4, false));
     GPX_TRY(dalloc(ctx, (void **)&S.waves, (size_t)n * g.iterations * 4, false));
+    S.host.assign(info, info + (size_t)n * 6);
     int rc = copy_in(ctx, S.info, info, (size_t)n * 6 * 4);
     if (!rc) rc = copy_in(ctx, S.fac, factors, (size_t)n * 2 * 4);
     if (!rc) rc = copy_in(ctx, S.waves, waves, (size_t)n * g.iterations * 4);
+    if (!rc) rc = update_planeflags(ctx);
     return rc;
 }
 
@@ -874,10 +1200,12 @@ extern "C" int gpx_set_receivers(gpx_ctx *ctx, int n, const int32_t *coords)
     if (ctx->rxc) { hipFree(ctx->rxc); ctx->rxc = nullptr; }
     if (ctx->rxbuf) { hipFree(ctx->rxbuf); ctx->rxbuf = nullptr; }
     ctx->nrx = n;
-    if (n == 0) return 0;
+    ctx->rxhost.assign(coords, coords + (size_t)n * 3);
+    if (n == 0) return update_planeflags(ctx);
     GPX_TRY(dalloc(ctx, (void **)&ctx->rxc, (size_t)n * 3 * 4, false));
     GPX_TRY(dalloc(ctx, (void **)&ctx->rxbuf, (size_t)6 * g.iterations * n * 4));
-    return copy_in(ctx, ctx->rxc, coords, (size_t)n * 3 * 4);
+    int rc = copy_in(ctx, ctx->rxc, coords, (size_t)n * 3 * 4);
+    return rc ? rc : update_planeflags(ctx);
 }
 
 extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, int dy, int dz,
@@ -909,21 +1237,71 @@ extern "C" int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block)
 }
 
 // ---------------------------------------------------------------- launching
-static void launch_shape(const gpx_ctx *ctx, int &block, int &xc, int &nseg, int &nchunk)
+// Work split of one pass.  `resident` blocks run at once (occupancy x CUs);
+// the grid is that many persistent blocks.  Items are (cell segment, x chunk);
+// unless fixed by gpx_set_launch, the chunk length is picked so that the items
+// fill the resident blocks in nearly whole rounds, with a small preference for
+// longer chunks (each chunk reloads one neighbour plane).
+static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, StepArgs *a)
 {
-    block = ctx->block ? ctx->block : 256;
-    nseg = (ctx->plane + block - 1) / block;
-    const int NX1 = ctx->desc.nx + 1;
-    if (ctx->xchunk) xc = ctx->xchunk;
-    else {
-        // Aim for about 8 blocks per CU of the 256 CUs, at least 8 planes per chunk.
-        long long want = 2048;
-        long long chunks = (want + nseg - 1) / nseg;
-        xc = (int)((NX1 + chunks - 1) / chunks);
-        if (xc < 8) xc = 8;
+    const int *b = ctx->box;
+    auto ch = [&](int len) { return len > 0 ? (len + xc - 1) / xc : 0; };
+    if (!bnd) {
+        const long long cells = (long long)(b[3] - b[2]) * (b[5] - b[4]);
+        const int nseg = (int)((cells + block - 1) / block);
+        const long long items = cells > 0 ? (long long)nseg * ch(b[1] - b[0]) : 0;
+        if (a) { a->nseg = nseg; a->nitems = (int)items; }
+        return items;
     }
-    if (xc > NX1) xc = NX1;
-    nchunk = (NX1 + xc - 1) / xc;
+    const int nseg_a = (ctx->plane + block - 1) / block;
+    const int nseg_b = (ctx->nframe + block - 1) / block;
+    const long long a1 = (long long)nseg_a * ch(b[0]);
+    const long long a2 = (long long)nseg_a * ch(ctx->desc.nx + 1 - b[1]);
+    const long long bb = (long long)nseg_b * ch(b[1] - b[0]);
+    if (a) {
+        a->nseg_a = nseg_a; a->nseg_b = nseg_b;
+        a->items_a1 = (int)a1; a->items_a2 = (int)a2;
+        a->nitems = (int)(a1 + a2 + bb);
+    }
+    return a1 + a2 + bb;
+}
+
+static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs &a, int &grid)
+{
+    const int NX1 = ctx->desc.nx + 1;
+    int xc = ctx->xchunk;
+    if (!xc) {
+        double best = 1e30;
+        for (int c = 1; c <= (NX1 < 96 ? NX1 : 96); ++c) {
+            const long long items = count_items(ctx, bnd, block, c, nullptr);
+            if (items <= 0) { xc = c; break; }
+            const long long rounds = (items + resident - 1) / resident;
+            const double cost = (double)(rounds * resident) / (double)items * (1.0 + 0.15 / c);
+            if (cost < best - 1e-9) {
+                best = cost;
+                xc = c;
+            }
+        }
+    }
+    a.xc = xc;
+    const long long items = count_items(ctx, bnd, block, xc, &a);
+    long long g = resident < items ? resident : items;
+    g = (g + 7) / 8 * 8;
+    if (g < 8) g = 8;
+    a.per_xcd = (int)(g / 8);
+    grid = (int)g;
+}
+
+static int resident_blocks(const gpx_ctx *ctx, const void *kernel, int block, size_t lds)
+{
+    auto key = std::make_pair(kernel, lds * 4096 + (size_t)block);
+    auto it = ctx->occupancy.find(key);
+    if (it != ctx->occupancy.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb < 1) nb = 1;
+    const int r = nb * (ctx->cus > 0 ? ctx->cus : 256);
+    ctx->occupancy[key] = r;
+    return r;
 }
 
 static void fill_slab_args(const gpx_ctx *ctx, int field, int s, SlabArgs &a)
@@ -963,31 +1341,96 @@ static void fill_slab_args(const gpx_ctx *ctx, int field, int s, SlabArgs &a)
     a.d = S.d;
     a.phi1 = S.phi[q];
     a.phi2 = S.phi[q + 1];
-    a.R = field == 0 ? S.eR : S.hR;
+    a.roff = S.roff;
 }
 
-static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, int &grid, size_t &lds)
+static int update_planeflags(gpx_ctx *ctx)
 {
-    int xc, nseg, nchunk;
-    launch_shape(ctx, block, xc, nseg, nchunk);
+    const int NX1 = ctx->desc.nx + 1;
+    std::vector<uint32_t> pf(NX1, 0);
+    for (int r = 0; r < ctx->nrx; ++r) pf[ctx->rxhost[3 * r]] |= PF_RX;
+    for (int s = 0; s < ctx->src[GPX_SRC_MAGNETIC].n; ++s) pf[ctx->src[GPX_SRC_MAGNETIC].host[6 * s]] |= PF_SRCH;
+    for (int kind : {GPX_SRC_HERTZIAN, GPX_SRC_VOLTAGE})
+        for (int s = 0; s < ctx->src[kind].n; ++s) pf[ctx->src[kind].host[6 * s]] |= PF_SRCE;
+    for (size_t s = 0; s < ctx->slabs.size(); ++s) {
+        SlabArgs e, h;
+        fill_slab_args(ctx, 0, (int)s, e);
+        fill_slab_args(ctx, 1, (int)s, h);
+        for (int i = 0; i < NX1; ++i) {
+            if (i >= e.lo[0] && i < e.hi[0]) pf[i] |= 1u << (PF_SLAB_E + s);
+            if (i >= h.lo[0] && i < h.hi[0]) pf[i] |= 1u << (PF_SLAB_H + s);
+        }
+    }
+    int rc = copy_in(ctx, ctx->planeflag, pf.data(), pf.size() * 4);
+    if (rc) return rc;
+
+    // Interior box: cells 1..n-1 on every axis (all neighbours and all updates
+    // present), shrunk past every slab box that spans the cross-section.
+    const gpx_desc &g = ctx->desc;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int lo[3] = {1, 1, 1}, hi[3] = {g.nx, g.ny, g.nz};
+    bool empty = false;
+    for (size_t s = 0; s < ctx->slabs.size() && !empty; ++s) {
+        for (int field = 0; field < 2; ++field) {
+            SlabArgs b;
+            fill_slab_args(ctx, field, (int)s, b);
+            const int ax = b.axis;
+            bool spans = true;
+            for (int o = 0; o < 3; ++o)
+                if (o != ax && (b.lo[o] > 1 || b.hi[o] < n[o])) spans = false;
+            if (!spans) { empty = true; break; }
+            if (b.minus) lo[ax] = lo[ax] > b.hi[ax] ? lo[ax] : b.hi[ax];
+            else hi[ax] = hi[ax] < b.lo[ax] ? hi[ax] : b.lo[ax];
+        }
+    }
+    for (int o = 0; o < 3; ++o)
+        if (lo[o] >= hi[o]) empty = true;
+    if (empty) {
+        const int b[6] = {g.nx + 1, g.nx + 1, 0, 0, 0, 0};
+        memcpy(ctx->box, b, sizeof(b));
+    } else {
+        const int b[6] = {lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]};
+        memcpy(ctx->box, b, sizeof(b));
+    }
+    std::vector<int> fr;
+    if (!empty) {
+        for (int j = 0; j <= g.ny; ++j)
+            for (int k = 0; k <= g.nz; ++k)
+                if (!(j >= lo[1] && j < hi[1] && k >= lo[2] && k < hi[2])) fr.push_back(j * (g.nz + 1) + k);
+    }
+    if (ctx->frame) { hipFree(ctx->frame); ctx->frame = nullptr; }
+    ctx->nframe = (int)fr.size();
+    if (!fr.empty()) {
+        GPX_TRY(dalloc(ctx, (void **)&ctx->frame, fr.size() * 4, false));
+        rc = copy_in(ctx, ctx->frame, fr.data(), fr.size() * 4);
+    }
+    return rc;
+}
+
+static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, size_t &lds)
+{
+    block = ctx->block ? ctx->block : 256;
     memset(&a, 0, sizeof(a));
     const gpx_desc &g = ctx->desc;
     a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
     a.plane = ctx->plane;
-    a.nseg = nseg;
-    a.nblocks = nseg * nchunk;
-    a.per_xcd = (a.nblocks + 7) / 8;
-    a.xc = xc;
     a.Ex = ctx->F[0]; a.Ey = ctx->F[1]; a.Ez = ctx->F[2];
     a.Hx = ctx->F[3]; a.Hy = ctx->F[4]; a.Hz = ctx->F[5];
     a.ID = ctx->ID;
     a.ntot = ctx->ntot;
     a.nmat = g.nmat;
     a.coef = field == 0 ? ctx->cE : ctx->cH;
+    a.R = ctx->Rall[field == 0 ? 0 : 1];
+    a.nR = (int)ctx->Rhost[0].size();
+    a.form = g.pml_formulation;
+    a.planeflag = ctx->planeflag;
+    memcpy(a.box, ctx->box, sizeof(a.box));
+    a.frame = ctx->frame;
+    a.nframe = ctx->nframe;
     a.iterations = g.iterations;
     a.nslab = (int)ctx->slabs.size();
     for (int s = 0; s < a.nslab; ++s) fill_slab_args(ctx, field, s, a.slab[s]);
-    lds = (size_t)g.nmat * 5 * 4;
+    lds = (size_t)g.nmat * 20 + (size_t)a.nR * 4;
     if (field == 0) {
         a.src0 = {ctx->src[GPX_SRC_VOLTAGE].n, ctx->src[GPX_SRC_VOLTAGE].info, ctx->src[GPX_SRC_VOLTAGE].fac, ctx->src[GPX_SRC_VOLTAGE].waves};
         a.src1 = {ctx->src[GPX_SRC_HERTZIAN].n, ctx->src[GPX_SRC_HERTZIAN].info, ctx->src[GPX_SRC_HERTZIAN].fac, ctx->src[GPX_SRC_HERTZIAN].waves};
@@ -1004,45 +1447,57 @@ static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, in
         a.rxc = ctx->rxc;
         a.rxbuf = ctx->rxbuf;
     }
-    grid = a.per_xcd * 8;
 }
 
-template <int FORM>
-static hipError_t launch_e_form(const gpx_ctx *ctx, const StepArgs &a, int grid, int block, size_t lds)
-{
-    const int P = ctx->desc.maxpoles;
-    const bool C = ctx->disp_complex != 0;
-    hipStream_t st = ctx->stream;
-#define GPX_E(PP, CC) hipLaunchKernelGGL((k_e_pass<FORM, PP, CC>), dim3(grid), dim3(block), lds, st, a)
-    switch (P) {
-    case 0: GPX_E(0, false); break;
-    case 1: if (C) GPX_E(1, true); else GPX_E(1, false); break;
-    case 2: if (C) GPX_E(2, true); else GPX_E(2, false); break;
-    case 3: if (C) GPX_E(3, true); else GPX_E(3, false); break;
-    case 4: if (C) GPX_E(4, true); else GPX_E(4, false); break;
-    default: return hipErrorInvalidValue;
-    }
-#undef GPX_E
-    return hipGetLastError();
-}
+typedef void (*pass_kernel)(StepArgs);
 
-static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it)
+static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd, int field, int flags, int it)
 {
     StepArgs a;
     int block, grid;
     size_t lds;
-    base_args(ctx, field, a, block, grid, lds);
+    base_args(ctx, field, a, block, lds);
+    if (count_items(ctx, bnd, block, 1, nullptr) <= 0) return hipSuccess;
+    plan(ctx, bnd, resident_blocks(ctx, (const void *)kernel, block, lds), block, a, grid);
     a.flags = flags;
     a.it = it;
-    if (field == 1) {
-        if (ctx->desc.pml_formulation == 0)
-            hipLaunchKernelGGL((k_h_pass<0>), dim3(grid), dim3(block), lds, ctx->stream, a);
-        else
-            hipLaunchKernelGGL((k_h_pass<1>), dim3(grid), dim3(block), lds, ctx->stream, a);
-        return hipGetLastError();
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, ctx->stream, a);
+    return hipGetLastError();
+}
+
+template <class IDT, bool BND>
+static pass_kernel e_kernel(int P, bool C)
+{
+    switch (P) {
+    case 0: return k_e_pass<IDT, 0, false, BND>;
+    case 1: return C ? k_e_pass<IDT, 1, true, BND> : k_e_pass<IDT, 1, false, BND>;
+    case 2: return C ? k_e_pass<IDT, 2, true, BND> : k_e_pass<IDT, 2, false, BND>;
+    case 3: return C ? k_e_pass<IDT, 3, true, BND> : k_e_pass<IDT, 3, false, BND>;
+    default: return C ? k_e_pass<IDT, 4, true, BND> : k_e_pass<IDT, 4, false, BND>;
     }
-    return ctx->desc.pml_formulation == 0 ? launch_e_form<0>(ctx, a, grid, block, lds)
-                                          : launch_e_form<1>(ctx, a, grid, block, lds);
+}
+
+template <bool BND>
+static pass_kernel pick_kernel(const gpx_ctx *ctx, int field)
+{
+    if (field == 1)
+        return ctx->id_bytes == 1 ? k_h_pass<uint8_t, BND>
+                                  : (ctx->id_bytes == 2 ? k_h_pass<uint16_t, BND> : k_h_pass<uint32_t, BND>);
+    const int P = ctx->desc.maxpoles;
+    const bool C = ctx->disp_complex != 0;
+    return ctx->id_bytes == 1 ? e_kernel<uint8_t, BND>(P, C)
+                              : (ctx->id_bytes == 2 ? e_kernel<uint16_t, BND>(P, C) : e_kernel<uint32_t, BND>(P, C));
+}
+
+// One pass = interior kernel (skipped when only PML work is asked for) then
+// boundary kernel; they touch disjoint cells.
+static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it)
+{
+    if (flags & ~F_PML_ALL) {
+        hipError_t e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it);
+        if (e != hipSuccess) return e;
+    }
+    return launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it);
 }
 
 static hipError_t launch_snapshot(const gpx_ctx *ctx, int s)
