@@ -9,8 +9,10 @@ The default workload is the 300x300x200 heterogeneous half-space (256 soil
 materials, PEC cylinder, magnetic dipole, nine receivers).
 
 Prints one JSON line: Mcells/s (NX*NY*NZ*K / T), the roofline of the dominant
-pass (algorithmic bytes / mean pass duration against 8 TB/s HBM), and the CPU
-oracle timed on the host cores on a bounded sample of the same workload.
+kernel (its algorithmic bytes per launch / its mean launch duration from HIP
+events, against 8 TB/s HBM; HBM traffic per launch from the committed counter
+summary), and the CPU oracle timed on the host cores on a bounded sample of the
+same workload.
 """
 
 import argparse
@@ -54,6 +56,19 @@ def cpu_baseline(workload, seconds=12.0):
                 n, n, G.title, threads, el)}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same
+    counters read for a copy of known size), or None when there is none."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic.json')
+    try:
+        with open(path) as f:
+            rec = json.load(f)[workload][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return int(rec['fetch_kib'] * 1024 * rec['fetch_scale'] + rec['write_kib'] * 1024 * rec['write_scale'])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -71,7 +86,7 @@ def main():
         return 2
 
     from gprmax_amd.hip import HipSolver, device_info
-    from gprmax_amd.workloads import WORKLOADS, pass_bytes
+    from gprmax_amd.workloads import WORKLOADS, kernel_bytes
 
     t_build = time.perf_counter()
     G = WORKLOADS[args.workload]()
@@ -86,15 +101,22 @@ def main():
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
         s.synchronize()
-        # per-pass durations, separate from the timed region
+        # Per-kernel durations (HIP events around every launch on the solver's
+        # stream), in a separate run after the timed region.
         n_prof = min(50, G.iterations - W - K)
-        _, ms_h, ms_e = s.run_timed(W + K, n_prof, per_kernel=True) if n_prof > 0 else (0, 0, 0)
+        ms4 = s.run_profiled(W + K, n_prof) if n_prof > 0 else (0.0,) * 4
+        interior = s.interior_cells()
         mem = s.memory_bytes()
     cells = G.nx * G.ny * G.nz
     mcells = cells * K / (ms * 1e-3) / 1e6
-    bh, be = pass_bytes(G, 1), pass_bytes(G, 0)
-    dom = ('e_pass', be, ms_e) if ms_e >= ms_h else ('h_pass', bh, ms_h)
-    achieved = dom[1] / (dom[2] * 1e-3) / 1e9 if dom[2] > 0 else 0.0
+    bh, be = kernel_bytes(G, 1, interior), kernel_bytes(G, 0, interior)
+    kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
+               'e_interior': (be[0], ms4[2]), 'e_boundary': (be[1], ms4[3])}
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    dbytes, dms = kernels[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
+    step_gbs = (sum(b for b, _ in kernels.values())) / (ms / K * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, dom)
     line = {
         'metric': METRIC,
         'value': round(mcells, 2),
@@ -109,11 +131,14 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic',
         'config': {'workload': G.title, 'cells': cells, 'nmat': len(G.materials),
-                   'pml_cells': 10, 'iterations_timed': K, 'device': info['name']},
-        'roofline': {'bound': 'hbm', 'kernel': dom[0], 'achieved': round(achieved, 1),
+                   'pml_cells': G.pmls[0].thickness if G.pmls else 0, 'poles': G.maxpoles,
+                   'iterations_timed': K, 'device': info['name']},
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': None, 'bytes_per_launch': dom[1], 'ms_per_launch': round(dom[2], 5),
-                     'ms_h_pass': round(ms_h, 5), 'ms_e_pass': round(ms_e, 5)},
+                     'traffic': traffic, 'bytes_per_launch': dbytes, 'ms_per_launch': round(dms, 5),
+                     'kernels_ms': {k: round(v[1], 5) for k, v in kernels.items()},
+                     'kernels_bytes': {k: v[0] for k, v in kernels.items()},
+                     'step_achieved': round(step_gbs, 1), 'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),
     }

@@ -59,7 +59,8 @@ struct SlabArgs {
     int t, order;
     int roff;        // offset of this slab's [4][order][t] table in the LDS R area
     float d;
-    float *phi1, *phi2;
+    float *phi1, *phi2;   // Phi read
+    float *phw1, *phw2;   // Phi written (the same arrays except in the fused step)
 };
 
 struct SrcArgs {
@@ -165,9 +166,12 @@ __device__ __forceinline__ void st(T *base, uint32_t boff, T v)
 // ---------------------------------------------------------------- PML
 // One corrected component (gprMax/pml_updates/*_ext.pyx).  R = [RA, RB, RE, RF]
 // each [order][t] in LDS; i1/off index Phi order 0 and the order-1 offset.
-__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
-                                             float F, bool plus_sign, float c, float dd,
-                                             float *phi, uint32_t i1, uint32_t off)
+// Phi is read from `pin` and, when `wr`, the advanced values are written to
+// `pout` (the same array for an in-place update).
+__device__ __forceinline__ float pml_apply(int form, const SlabArgs &s, const float *R, int p,
+                                           float F, bool plus_sign, float c, float dd,
+                                           const float *pin, float *pout, bool wr, uint32_t i1,
+                                           uint32_t off)
 {
     const int ot = s.order * s.t;
     float corr;
@@ -175,19 +179,21 @@ __device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const 
         if (s.order == 1) {
             const float RA01 = R[p] - 1.0f;
             const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            const float ph = phi[i1];
+            const float ph = pin[i1];
             corr = c * (RA01 * dd + RB0 * ph);
-            phi[i1] = RE0 * ph - RF0 * dd;
+            if (wr) pout[i1] = RE0 * ph - RF0 * dd;
         } else {
             const float RA0 = R[p], RA1 = R[s.t + p];
             const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
             const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
             const float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
             const float RA01 = RA0 * RA1 - 1.0f;
-            const float p0 = phi[i1], p1 = phi[i1 + off];
+            const float p0 = pin[i1], p1 = pin[i1 + off];
             corr = c * (RA01 * dd + RA1 * RB0 * p0 + RB1 * p1);
-            phi[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
-            phi[i1] = RE0 * p0 - RF0 * dd;
+            if (wr) {
+                pout[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
+                pout[i1] = RE0 * p0 - RF0 * dd;
+            }
         }
     } else {  // MRIPML
         if (s.order == 1) {
@@ -195,23 +201,32 @@ __device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const 
             const float IRA1 = IRA - 1.0f;
             const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
             const float RC0 = IRA * RB0 * RF0;
-            const float ph = phi[i1];
+            const float ph = pin[i1];
             corr = c * (IRA1 * dd - IRA * ph);
-            phi[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
+            if (wr) pout[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
         } else {
             const float IRA = 1.0f / (R[p] + R[s.t + p]);
             const float IRA1 = IRA - 1.0f;
             const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
             const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
             const float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
-            const float p0 = phi[i1], p1 = phi[i1 + off];
+            const float p0 = pin[i1], p1 = pin[i1 + off];
             const float psi = RB0 * p0 + RB1 * p1;
             corr = c * (IRA1 * dd - IRA * psi);
-            phi[i1 + off] = RE1 * p1 + RC1 * (dd - psi);
-            phi[i1] = RE0 * p0 + RC0 * (dd - psi);
+            if (wr) {
+                pout[i1 + off] = RE1 * p1 + RC1 * (dd - psi);
+                pout[i1] = RE0 * p0 + RC0 * (dd - psi);
+            }
         }
     }
     return plus_sign ? F + corr : F - corr;
+}
+
+__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
+                                             float F, bool plus_sign, float c, float dd,
+                                             float *phi, uint32_t i1, uint32_t off)
+{
+    return pml_apply(form, s, R, p, F, plus_sign, c, dd, phi, phi, true, i1, off);
 }
 
 __device__ __forceinline__ bool in_box(const SlabArgs &s, int i, int j, int k)
@@ -1339,8 +1354,8 @@ static void fill_slab_args(const gpx_ctx *ctx, int field, int s, SlabArgs &a)
     a.t = S.t;
     a.order = S.order;
     a.d = S.d;
-    a.phi1 = S.phi[q];
-    a.phi2 = S.phi[q + 1];
+    a.phi1 = a.phw1 = S.phi[q];
+    a.phi2 = a.phw2 = S.phi[q + 1];
     a.roff = S.roff;
 }
 
@@ -1490,14 +1505,21 @@ static pass_kernel pick_kernel(const gpx_ctx *ctx, int field)
 }
 
 // One pass = interior kernel (skipped when only PML work is asked for) then
-// boundary kernel; they touch disjoint cells.
-static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it)
+// boundary kernel; they touch disjoint cells.  With `ev`, an event is recorded
+// before, between and after the two launches.
+static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it, hipEvent_t *ev = nullptr)
 {
+    hipError_t e;
+    if (ev && (e = hipEventRecord(ev[0], ctx->stream)) != hipSuccess) return e;
     if (flags & ~F_PML_ALL) {
-        hipError_t e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it);
+        e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it);
         if (e != hipSuccess) return e;
     }
-    return launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it);
+    if (ev && (e = hipEventRecord(ev[1], ctx->stream)) != hipSuccess) return e;
+    e = launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it);
+    if (e != hipSuccess) return e;
+    if (ev) e = hipEventRecord(ev[2], ctx->stream);
+    return e;
 }
 
 static hipError_t launch_snapshot(const gpx_ctx *ctx, int s)
@@ -1535,34 +1557,34 @@ static int e_flags(const gpx_ctx *ctx)
     return f;
 }
 
-static int run_steps(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_h, float *ms_e)
+// Runs iterations [it0, it0+n).  With `ms4`, events around every kernel give
+// the mean duration of the H interior, H boundary, E interior and E boundary
+// kernels over the run (the host waits after each iteration to read them).
+static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
     const int hf = h_flags(ctx), ef = e_flags(ctx);
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    double th = 0, te = 0;
-    if (per_kernel)
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double acc[4] = {0, 0, 0, 0};
+    if (ms4)
         for (auto &e : ev) GPX_TRY(hipEventCreate(&e));
     for (int it = it0; it < it0 + n; ++it) {
         for (size_t s = 0; s < ctx->snaps.size(); ++s)
             if (ctx->snaps[s].time == it + 1) GPX_TRY(launch_snapshot(ctx, (int)s));
-        if (per_kernel) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
-        GPX_TRY(launch_pass(ctx, 1, hf, it));
-        if (per_kernel) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
-        GPX_TRY(launch_pass(ctx, 0, ef, it));
-        if (per_kernel) {
-            GPX_TRY(hipEventRecord(ev[2], ctx->stream));
-            GPX_TRY(hipEventSynchronize(ev[2]));
-            float a = 0, b = 0;
-            GPX_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
-            GPX_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
-            th += a;
-            te += b;
+        GPX_TRY(launch_pass(ctx, 1, hf, it, ms4 ? ev : nullptr));
+        GPX_TRY(launch_pass(ctx, 0, ef, it, ms4 ? ev + 3 : nullptr));
+        if (ms4) {
+            GPX_TRY(hipEventSynchronize(ev[5]));
+            float t[4];
+            GPX_TRY(hipEventElapsedTime(&t[0], ev[0], ev[1]));
+            GPX_TRY(hipEventElapsedTime(&t[1], ev[1], ev[2]));
+            GPX_TRY(hipEventElapsedTime(&t[2], ev[3], ev[4]));
+            GPX_TRY(hipEventElapsedTime(&t[3], ev[4], ev[5]));
+            for (int q = 0; q < 4; ++q) acc[q] += t[q];
         }
     }
-    if (per_kernel) {
+    if (ms4) {
         for (auto &e : ev) hipEventDestroy(e);
-        if (ms_h) *ms_h = (float)(th / (n > 0 ? n : 1));
-        if (ms_e) *ms_e = (float)(te / (n > 0 ? n : 1));
+        for (int q = 0; q < 4; ++q) ms4[q] = (float)(acc[q] / (n > 0 ? n : 1));
     }
     return 0;
 }
@@ -1573,7 +1595,7 @@ extern "C" int gpx_run(gpx_ctx *ctx, int it0, int n)
         return fail(ctx, GPX_EARG, "gpx_run: iteration range outside [0, iterations)");
     if (int rc = check_ready(ctx)) return rc;
     hipSetDevice(ctx->desc.device);
-    return run_steps(ctx, it0, n, 0, nullptr, nullptr);
+    return run_steps(ctx, it0, n, nullptr);
 }
 
 extern "C" int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
@@ -1587,8 +1609,13 @@ extern "C" int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float
     GPX_TRY(hipEventCreate(&a));
     GPX_TRY(hipEventCreate(&b));
     GPX_TRY(hipEventRecord(a, ctx->stream));
-    int rc = run_steps(ctx, it0, n, per_kernel, ms_h, ms_e);
+    float ms4[4] = {0, 0, 0, 0};
+    int rc = run_steps(ctx, it0, n, per_kernel ? ms4 : nullptr);
     if (rc) return rc;
+    if (per_kernel) {
+        if (ms_h) *ms_h = ms4[0] + ms4[1];
+        if (ms_e) *ms_e = ms4[2] + ms4[3];
+    }
     GPX_TRY(hipEventRecord(b, ctx->stream));
     GPX_TRY(hipEventSynchronize(b));
     float ms = 0;
@@ -1597,6 +1624,15 @@ extern "C" int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float
     hipEventDestroy(b);
     if (ms_total) *ms_total = ms;
     return 0;
+}
+
+extern "C" int gpx_run_profiled(gpx_ctx *ctx, int it0, int n, float *ms4)
+{
+    if (!ctx || !ms4 || it0 < 0 || n < 1 || it0 + n > ctx->desc.iterations)
+        return fail(ctx, GPX_EARG, "gpx_run_profiled: bad argument");
+    if (int rc = check_ready(ctx)) return rc;
+    hipSetDevice(ctx->desc.device);
+    return run_steps(ctx, it0, n, ms4);
 }
 
 extern "C" int gpx_part(gpx_ctx *ctx, int part, int arg, int it)
@@ -1762,6 +1798,14 @@ extern "C" int gpx_device_info(int device, char *name, int namelen, size_t *tota
         *freemem = fr;
     }
     return 0;
+}
+
+extern "C" long long gpx_interior_cells(const gpx_ctx *ctx)
+{
+    if (!ctx) return 0;
+    const int *b = ctx->box;
+    if (b[1] <= b[0]) return 0;
+    return (long long)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
 }
 
 extern "C" size_t gpx_memory_bytes(const gpx_ctx *ctx)

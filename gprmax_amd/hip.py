@@ -43,6 +43,8 @@ SIGNATURES = {
     'gpx_add_snapshot': ([_P] + [_I] * 10 + [ctypes.POINTER(_I)], _I),
     'gpx_run': ([_P, _I, _I], _I),
     'gpx_run_timed': ([_P, _I, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
+    'gpx_run_profiled': ([_P, _I, _I, ctypes.POINTER(_F)], _I),
+    'gpx_interior_cells': ([_P], ctypes.c_longlong),
     'gpx_part': ([_P, _I, _I, _I], _I),
     'gpx_set_launch': ([_P, _I, _I], _I),
     'gpx_synchronize': ([_P], _I),
@@ -171,6 +173,15 @@ class HipSolver:
         self._check(self.lib.gpx_run_timed(self.ctx, it0, n, int(per_kernel), ctypes.byref(t),
                                            ctypes.byref(h), ctypes.byref(e)))
         return t.value, h.value, e.value
+
+    def run_profiled(self, it0, n):
+        """Mean ms of (H interior, H boundary, E interior, E boundary) kernels."""
+        ms = (ctypes.c_float * 4)()
+        self._check(self.lib.gpx_run_profiled(self.ctx, it0, n, ms))
+        return tuple(ms)
+
+    def interior_cells(self):
+        return int(self.lib.gpx_interior_cells(self.ctx))
 
     def part(self, name, arg=0, it=0):
         self._check(self.lib.gpx_part(self.ctx, PART[name], arg, it))

@@ -131,3 +131,16 @@ def pass_bytes(G, field):
     if field == 0:
         b += 24 * G.maxpoles * nc
     return b
+
+
+def kernel_bytes(G, field, interior_cells):
+    """Algorithmic bytes of the (interior, boundary) kernels of one pass.
+
+    The interior kernel updates `interior_cells` cells away from the faces and
+    the PML slabs at 48 B each (plus 24 B x poles for E); the boundary kernel
+    updates the remaining cells and carries all the PML Phi traffic, so the
+    two add up to ``pass_bytes``.
+    """
+    per_cell = 48 + (24 * G.maxpoles if field == 0 else 0)
+    inner = per_cell * interior_cells
+    return inner, pass_bytes(G, field) - inner

@@ -107,6 +107,15 @@ int gpx_run(gpx_ctx *ctx, int it0, int n);
 int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
                   float *ms_h, float *ms_e);
 
+/* As gpx_run, with HIP events around every kernel; ms4 receives the mean
+ * duration in ms of the four kernels of an iteration: H interior, H boundary,
+ * E interior, E boundary (interior = cells away from the faces and PML
+ * slabs, boundary = the rest).  Waits after each iteration. */
+int gpx_run_profiled(gpx_ctx *ctx, int it0, int n, float *ms4);
+
+/* Cells of the interior box (the rest are boundary cells). */
+long long gpx_interior_cells(const gpx_ctx *ctx);
+
 /* One unfused sub-step (enum gpx_part_id) at iteration it. */
 int gpx_part(gpx_ctx *ctx, int part, int arg, int it);
 

@@ -12,17 +12,24 @@ wrong index fails something):
 
 * update_electric / update_magnetic: golden receiver traces of the reference
   (tests/models_basic/hertzian_dipole_fs, magnetic_dipole_fs,
-  hertzian_dipole_hs), the closed-form Hertzian dipole field, and an
-  independent NumPy slicing twin (``oracle.numpy_twin``) bit for bit.
+  hertzian_dipole_hs), the closed-form Hertzian dipole field, an independent
+  NumPy slicing twin (``oracle.numpy_twin``) bit for bit, and -- where
+  /root/reference is present -- the reference's own Cython kernels compiled
+  into oracle/_ref (``oracle.build_ref``), bit for bit.
 * dispersive A/B: golden traces of hertzian_dipole_dispersive (1 Debye pole);
-  multipole with extra zero-weight poles equals one pole bit for bit; real
-  Debye coefficients keep the imaginary part of T at exactly zero.
-* PML (HORIPML order 1): the golden traces above all run through the default
-  10-cell PML; every slab and order is also checked against the NumPy twin,
-  and reflection from the PML is checked against a large-domain run.
-  HORIPML order 2 and MRIPML orders 1/2: NumPy twin and reflection only --
-  no reference vector exists for them (parity pinned by invariants only).
-* snapshots: NumPy twin of the slicing definition (no reference vector).
+  the compiled reference kernels bit for bit with random complex coefficients
+  (1 and 3 poles, covering Lorentz/Drude); multipole with extra zero-weight
+  poles equals one pole bit for bit; real Debye coefficients keep the
+  imaginary part of T at exactly zero.
+* PML: HORIPML order 1 is exercised by every golden trace (default 10-cell
+  PML).  All 24 slab kernels (HORIPML/MRIPML x order 1/2 x 6 directions,
+  electric and magnetic) match the compiled reference kernels bit for bit and
+  the NumPy twin; reflection from the PML is checked against a large-domain
+  run.  No reference trace exists for HORIPML order 2 or MRIPML, so outside
+  this container (no oracle/_ref) those rest on the twin and the reflection
+  bound.
+* snapshots: the compiled reference kernel bit for bit and the NumPy twin of
+  the slicing definition (no reference vector).
 """
 
 import ctypes
