@@ -78,6 +78,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--xchunk', type=int, default=0)
     ap.add_argument('--block', type=int, default=0)
+    ap.add_argument('--tile', type=int, default=0, help='cells per fused block (0 = auto)')
+    ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this solver runs on one GPU; --gpus {} is not supported '
@@ -86,7 +88,7 @@ def main():
         return 2
 
     from gprmax_amd.hip import HipSolver, device_info
-    from gprmax_amd.workloads import WORKLOADS, kernel_bytes
+    from gprmax_amd.workloads import WORKLOADS, algorithmic_bytes_per_step, kernel_bytes
 
     t_build = time.perf_counter()
     G = WORKLOADS[args.workload]()
@@ -97,6 +99,8 @@ def main():
     info = device_info(0)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
+        s.set_step(1 if args.fused else 0, args.tile)
+        fused = s.step_kind()
         s.run(0, W)
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
@@ -109,9 +113,12 @@ def main():
         mem = s.memory_bytes()
     cells = G.nx * G.ny * G.nz
     mcells = cells * K / (ms * 1e-3) / 1e6
-    bh, be = kernel_bytes(G, 1, interior), kernel_bytes(G, 0, interior)
-    kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
-               'e_interior': (be[0], ms4[2]), 'e_boundary': (be[1], ms4[3])}
+    if fused:
+        kernels = {'step': (algorithmic_bytes_per_step(G), ms4[0])}
+    else:
+        bh, be = kernel_bytes(G, 1, interior), kernel_bytes(G, 0, interior)
+        kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
+                   'e_interior': (be[0], ms4[2]), 'e_boundary': (be[1], ms4[3])}
     dom = max(kernels, key=lambda k: kernels[k][1])
     dbytes, dms = kernels[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
@@ -138,7 +145,9 @@ def main():
                      'traffic': traffic, 'bytes_per_launch': dbytes, 'ms_per_launch': round(dms, 5),
                      'kernels_ms': {k: round(v[1], 5) for k, v in kernels.items()},
                      'kernels_bytes': {k: v[0] for k, v in kernels.items()},
-                     'step_achieved': round(step_gbs, 1), 'interior_cells': interior},
+                     'step_achieved': round(step_gbs, 1),
+                     'step_form': 'fused, {} cells per block'.format(fused) if fused else 'two-pass',
+                     'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),
     }

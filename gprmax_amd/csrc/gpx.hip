@@ -30,45 +30,13 @@
 
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gpx.h"
+#include "gpx_step.h"
 
-namespace {
-
-constexpr int MAX_SLABS = 6;
-constexpr int MAX_POLES = 4;
-constexpr int MAX_LDS_BYTES = 64 * 1024;
-
-enum : int {
-    F_MAIN = 1,
-    F_SRC = 2,
-    F_RX = 4,
-    F_DISPB = 8,
-    F_PML0 = 16,  // bits 4..9: one per slab
-};
-constexpr int F_PML_ALL = F_PML0 * ((1 << MAX_SLABS) - 1);
-
-struct SlabArgs {
-    int axis;        // 0 x, 1 y, 2 z
-    int minus;       // 1 for xminus/yminus/zminus
-    int lo[3], hi[3];// box of corrected cells for this field, [lo, hi)
-    int pend;        // profile index p = pend - c (minus) or c - pend (plus)
-    int org[3];      // slab origin (xs, ys, zs) for the Phi index
-    int d1[3], d2[3];// Phi1/Phi2 dims without the order axis
-    int t, order;
-    int roff;        // offset of this slab's [4][order][t] table in the LDS R area
-    float d;
-    float *phi1, *phi2;   // Phi read
-    float *phw1, *phw2;   // Phi written (the same arrays except in the fused step)
-};
-
-struct SrcArgs {
-    int n;
-    const int *info;     // [n][6]
-    const float *fac;    // [n][2]
-    const float *waves;  // [n][iterations]
-};
+namespace gpx {
 
 struct StepArgs {
     int nx, ny, nz;
@@ -109,8 +77,6 @@ struct StepArgs {
     int nframe;
 };
 
-constexpr uint32_t PF_RX = 1, PF_SRCH = 2, PF_SRCE = 4;
-constexpr int PF_SLAB_E = 8, PF_SLAB_H = 16;
 
 // Slabs whose box, for one field, contains lane (j, k) in y and z; the x part
 // comes per plane from the plane flags.
@@ -124,164 +90,6 @@ __device__ __forceinline__ uint32_t lane_slab_mask(const StepArgs &a, int j, int
     return m;
 }
 
-// Streamed arrays are read and written with buffer instructions: a scalar
-// resource per array (base address in SGPRs), the lane's byte offset within
-// the plane in a VGPR and the plane's byte offset in an SGPR, so no
-// per-access vector address arithmetic is needed.  Offsets are 32-bit: an
-// array holds fewer than 2^30 floats.
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-__device__ __forceinline__ rsrc_t rsrc(const void *p)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7fffffff, 0x00020000);
-}
-// The b32 buffer builtins move raw 32-bit words; floats pass through as bits.
-__device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
-}
-template <class IDT>
-__device__ __forceinline__ uint32_t bld_id(rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    if (sizeof(IDT) == 1) return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
-    if (sizeof(IDT) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
-    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-}
-
-// Loads and stores at a uniform base pointer plus a 32-bit lane byte offset.
-template <class T>
-__device__ __forceinline__ T ld(const T *base, uint32_t boff)
-{
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
-}
-template <class T>
-__device__ __forceinline__ void st(T *base, uint32_t boff, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
-}
-
-// ---------------------------------------------------------------- PML
-// One corrected component (gprMax/pml_updates/*_ext.pyx).  R = [RA, RB, RE, RF]
-// each [order][t] in LDS; i1/off index Phi order 0 and the order-1 offset.
-// Phi is read from `pin` and, when `wr`, the advanced values are written to
-// `pout` (the same array for an in-place update).
-__device__ __forceinline__ float pml_apply(int form, const SlabArgs &s, const float *R, int p,
-                                           float F, bool plus_sign, float c, float dd,
-                                           const float *pin, float *pout, bool wr, uint32_t i1,
-                                           uint32_t off)
-{
-    const int ot = s.order * s.t;
-    float corr;
-    if (form == 0) {  // HORIPML
-        if (s.order == 1) {
-            const float RA01 = R[p] - 1.0f;
-            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            const float ph = pin[i1];
-            corr = c * (RA01 * dd + RB0 * ph);
-            if (wr) pout[i1] = RE0 * ph - RF0 * dd;
-        } else {
-            const float RA0 = R[p], RA1 = R[s.t + p];
-            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
-            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
-            const float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
-            const float RA01 = RA0 * RA1 - 1.0f;
-            const float p0 = pin[i1], p1 = pin[i1 + off];
-            corr = c * (RA01 * dd + RA1 * RB0 * p0 + RB1 * p1);
-            if (wr) {
-                pout[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
-                pout[i1] = RE0 * p0 - RF0 * dd;
-            }
-        }
-    } else {  // MRIPML
-        if (s.order == 1) {
-            const float IRA = 1.0f / R[p];
-            const float IRA1 = IRA - 1.0f;
-            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            const float RC0 = IRA * RB0 * RF0;
-            const float ph = pin[i1];
-            corr = c * (IRA1 * dd - IRA * ph);
-            if (wr) pout[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
-        } else {
-            const float IRA = 1.0f / (R[p] + R[s.t + p]);
-            const float IRA1 = IRA - 1.0f;
-            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
-            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
-            const float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
-            const float p0 = pin[i1], p1 = pin[i1 + off];
-            const float psi = RB0 * p0 + RB1 * p1;
-            corr = c * (IRA1 * dd - IRA * psi);
-            if (wr) {
-                pout[i1 + off] = RE1 * p1 + RC1 * (dd - psi);
-                pout[i1] = RE0 * p0 + RC0 * (dd - psi);
-            }
-        }
-    }
-    return plus_sign ? F + corr : F - corr;
-}
-
-__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
-                                             float F, bool plus_sign, float c, float dd,
-                                             float *phi, uint32_t i1, uint32_t off)
-{
-    return pml_apply(form, s, R, p, F, plus_sign, c, dd, phi, phi, true, i1, off);
-}
-
-__device__ __forceinline__ bool in_box(const SlabArgs &s, int i, int j, int k)
-{
-    return i >= s.lo[0] && i < s.hi[0] && j >= s.lo[1] && j < s.hi[1] && k >= s.lo[2] &&
-           k < s.hi[2];
-}
-
-// Profile index and the two Phi element indices of cell (i, j, k).
-__device__ __forceinline__ void slab_index(const SlabArgs &s, int i, int j, int k, int &p,
-                                           uint32_t &i1, uint32_t &i2, uint32_t &o1, uint32_t &o2)
-{
-    int c[3] = {i - s.org[0], j - s.org[1], k - s.org[2]};
-    const int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
-    p = s.minus ? s.pend - along : along - s.pend;
-    c[s.axis] = p;
-    i1 = ((uint32_t)c[0] * s.d1[1] + c[1]) * s.d1[2] + c[2];
-    i2 = ((uint32_t)c[0] * s.d2[1] + c[1]) * s.d2[2] + c[2];
-    o1 = (uint32_t)s.d1[0] * s.d1[1] * s.d1[2];
-    o2 = (uint32_t)s.d2[0] * s.d2[1] * s.d2[2];
-}
-
-// Persistent grid: blocks are dealt to the 8 XCDs round robin, so block b is
-// renumbered (b mod 8) * per_xcd + b / 8; in each round an XCD then works on a
-// contiguous run of items, i.e. neighbouring plane segments that share rows
-// through its L2.
-__device__ __forceinline__ int logical_block(const StepArgs &a)
-{
-    const int b = blockIdx.x;
-    return (b & 7) * a.per_xcd + (b >> 3);
-}
-
-// Copy a small table to LDS with four loads in flight per thread.
-// Slab descriptions go to LDS so the PML branch reads them into VGPRs when a
-// lane needs them instead of holding them in SGPRs for the whole loop.
-__device__ __forceinline__ void stage_slabs(SlabArgs *dst, const StepArgs &a)
-{
-    const int words = (int)(sizeof(SlabArgs) / 4) * a.nslab;
-    const int *src = reinterpret_cast<const int *>(a.slab);
-    int *d = reinterpret_cast<int *>(dst);
-    for (int q = threadIdx.x; q < words; q += blockDim.x) d[q] = src[q];
-}
-
-__device__ __forceinline__ void stage(float *dst, const float *src, int n)
-{
-    const int B = blockDim.x;
-    for (int q = threadIdx.x; q < n; q += 4 * B) {
-        float v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = q + u * B < n ? src[q + u * B] : 0.f;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (q + u * B < n) dst[q + u * B] = v[u];
-    }
-}
 
 // ---------------------------------------------------------------- work items
 struct Cell {
@@ -334,16 +142,6 @@ __device__ __forceinline__ Cell item_cell(const StepArgs &a, int item)
     return c;
 }
 
-// Coefficients in LDS: (CA, CBx, CBy, CBz) or (DA, DBx, DBy, DBz) as one
-// float4 per material, then srce/srcm as one float per material.
-__device__ __forceinline__ void stage_coef(float4 *c4, float *c1, const float *src, int nmat)
-{
-    for (int m = threadIdx.x; m < nmat; m += blockDim.x) {
-        const float *r = src + 5 * m;
-        c4[m] = make_float4(r[0], r[1], r[2], r[3]);
-        c1[m] = r[4];
-    }
-}
 
 // ---------------------------------------------------------------- H pass
 struct HLoad {
@@ -370,7 +168,7 @@ __global__ void k_h_pass(StepArgs a)
     stage_coef(lds4, const_cast<float *>(ls), a.coef, nm);
     if (BND) {
         stage(const_cast<float *>(lR), a.R, a.nR);
-        stage_slabs(lslab, a);
+        stage_slabs(lslab, a.slab, a.nslab);
     }
     __syncthreads();
 
@@ -384,7 +182,7 @@ __global__ void k_h_pass(StepArgs a)
     const rsrc_t rI3 = rsrc(ID + 3 * N), rI4 = rsrc(ID + 4 * N), rI5 = rsrc(ID + 5 * N);
     const uint32_t P4 = (uint32_t)a.plane * 4u, Pid = (uint32_t)a.plane * (uint32_t)sizeof(IDT);
 
-    for (int item = logical_block(a); item < a.nitems; item += gridDim.x) {
+    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
         const Cell c = item_cell<BND>(a, item);
         if (!c.live) continue;
         const int j = c.j, k = c.k;
@@ -517,78 +315,6 @@ __global__ void k_h_pass(StepArgs a)
 }
 
 // ---------------------------------------------------------------- E pass
-// Dispersive part A for one component (gprMax/fields_updates_ext.pyx:113-296):
-// returns phi and advances the P auxiliary values from the old field e.
-template <int P, bool CPLX>
-__device__ __forceinline__ float disp_a(const float *D, uint32_t m, float *tr, float *ti, float e)
-{
-    float phi = 0.f;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if (CPLX) {
-            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
-            if (P == 1) phi = c[0] * tr[p];
-            else phi = phi + c[0] * tr[p];
-            const float er = c[2], ei = c[3], zr = c[4], zi = c[5];
-            const float nr = (er * tr[p] - ei * ti[p]) + zr * e;
-            const float ni = (er * ti[p] + ei * tr[p]) + zi * e;
-            tr[p] = nr;
-            ti[p] = ni;
-        } else {
-            const float *c = D + (size_t)m * 3 * P + 3 * p;
-            if (P == 1) phi = c[0] * tr[p];
-            else phi = phi + c[0] * tr[p];
-            tr[p] = c[1] * tr[p] + c[2] * e;
-        }
-    }
-    return phi;
-}
-
-// Dispersive part B (gprMax/fields_updates_ext.pyx:183-235, 299-346).
-template <int P, bool CPLX>
-__device__ __forceinline__ void disp_b(const float *D, uint32_t m, float *tr, float *ti, float e)
-{
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if (CPLX) {
-            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
-            tr[p] = tr[p] - c[4] * e;
-            ti[p] = ti[p] - c[5] * e;
-        } else {
-            tr[p] = tr[p] - D[(size_t)m * 3 * P + 3 * p + 2] * e;
-        }
-    }
-}
-
-template <int P, bool CPLX>
-__device__ __forceinline__ void t_load(float *const *T, int c, size_t N, size_t po, uint32_t to,
-                                       float *tr, float *ti)
-{
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if (CPLX) {
-            const float2 v = ld(reinterpret_cast<const float2 *>(T[c]) + p * N + po, 2 * to);
-            tr[p] = v.x;
-            ti[p] = v.y;
-        } else {
-            tr[p] = ld((const float *)T[c] + p * N + po, to);
-            ti[p] = 0.f;
-        }
-    }
-}
-
-template <int P, bool CPLX>
-__device__ __forceinline__ void t_store(float *const *T, int c, size_t N, size_t po, uint32_t to,
-                                        const float *tr, const float *ti)
-{
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if (CPLX)
-            st(reinterpret_cast<float2 *>(T[c]) + p * N + po, 2 * to, make_float2(tr[p], ti[p]));
-        else
-            st(T[c] + p * N + po, to, tr[p]);
-    }
-}
 
 // update_electric or dispersive part A (gprMax/fields_updates_ext.pyx:81-107,
 // 146-179), each E-PML slab in order, voltage sources, Hertzian dipoles and
@@ -608,7 +334,7 @@ __global__ void k_e_pass(StepArgs a)
     stage_coef(lds4, const_cast<float *>(ls), a.coef, nm);
     if (BND) {
         stage(const_cast<float *>(lR), a.R, a.nR);
-        stage_slabs(lslab, a);
+        stage_slabs(lslab, a.slab, a.nslab);
     }
     if (P > 0) {
         stage(const_cast<float *>(ldisp), a.disp, nd);
@@ -626,7 +352,7 @@ __global__ void k_e_pass(StepArgs a)
     const rsrc_t rI0 = rsrc(ID), rI1 = rsrc(ID + N), rI2 = rsrc(ID + 2 * N);
     const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
 
-    for (int item = logical_block(a); item < a.nitems; item += gridDim.x) {
+    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
         const Cell c = item_cell<BND>(a, item);
         if (!c.live) continue;
         const int j = c.j, k = c.k;
@@ -794,6 +520,7 @@ __global__ void k_e_pass(StepArgs a)
     }
 }
 
+
 // ---------------------------------------------------------------- snapshots
 // Cell averages of gprMax/snapshots_ext.pyx:58-71 over the slices of
 // gprMax/snapshots.py:78-80; one thread per output cell.
@@ -820,13 +547,16 @@ __global__ void k_snapshot(int ny, int nz, int xs, int ys, int zs, int sdx, int 
     out[5 * n + o] = (Hz[at(i, j, k)] + Hz[at(i, j, k + 1)]) / 2.0f;
 }
 
-}  // namespace
+}  // namespace gpx
+
+using namespace gpx;
 
 // ====================================================================== ctx
 struct Slab {
     int dir, xs, xf, ys, yf, zs, zf, order, t, roff = 0;
     float d;
-    float *phi[4] = {nullptr, nullptr, nullptr, nullptr};  // EPhi1, EPhi2, HPhi1, HPhi2
+    float *phi[4] = {nullptr, nullptr, nullptr, nullptr};  // EPhi1, EPhi2, HPhi1, HPhi2 (current)
+    float *phin[2] = {nullptr, nullptr};                    // HPhi1, HPhi2 of the next state
     long long dims[4][3];
     float *eR = nullptr, *hR = nullptr;
 };
@@ -848,9 +578,12 @@ struct gpx_ctx {
     long long ntot = 0;
     int plane = 0;
     hipStream_t stream = nullptr;
-    float *F[6] = {nullptr};  // Ex, Ey, Ez, Hx, Hy, Hz
+    float *F[6] = {nullptr};  // Ex, Ey, Ez, Hx, Hy, Hz (current state)
+    float *Fn[6] = {nullptr}; // the next state written by the fused step
+    float *Fmem[2] = {nullptr, nullptr};  // the two states, 6 x fstride floats each
+    long long fstride = 0;    // floats between components: ntot rounded up to 64
     void *ID = nullptr;       // [6][ntot] as uint8, uint16 or uint32
-    int id_bytes = 4;         // smallest width that holds every index < nmat
+    int id_bytes = 4;         // 2 when every index < nmat fits 16 bits, else 4
     float *cE = nullptr, *cH = nullptr;
     float *Rall[2] = {nullptr, nullptr};  // E, H: all slabs' R tables back to back
     std::vector<float> Rhost[2];
@@ -867,10 +600,14 @@ struct gpx_ctx {
     float *rxbuf = nullptr;
     std::vector<Snap> snaps;
     uint32_t *planeflag = nullptr;  // [nx+1]
+    StepRare *rare = nullptr;        // device copy of the source/receiver tables for the fused step
+    bool rare_dirty = true;
     int box[6] = {0, 0, 0, 0, 0, 0}; // interior box (see StepArgs)
     int *frame = nullptr;            // plane offsets outside the interior rectangle
     int nframe = 0;
     int xchunk = 0, block = 0;
+    int fused = 0;                   // 1: one fused kernel per step when eligible; 0: two passes
+    int tile_cells = 0;              // cells per fused block (0 = auto)
     int cus = 0;                     // compute units of the device
     mutable std::map<std::pair<const void *, size_t>, int> occupancy;
     size_t bytes = 0;
@@ -950,12 +687,16 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
         delete ctx;
         return rc;
     }
-    const size_t fb = (size_t)ctx->ntot * 4;
-    for (int c = 0; c < 6; ++c) {
-        e = dalloc(ctx, (void **)&ctx->F[c], fb);
+    ctx->fstride = (ctx->ntot + 63) / 64 * 64;
+    for (int s = 0; s < 2; ++s) {
+        e = dalloc(ctx, (void **)&ctx->Fmem[s], (size_t)6 * ctx->fstride * 4);
         if (e != hipSuccess) goto fail_alloc;
     }
-    ctx->id_bytes = d->nmat <= 256 ? 1 : (d->nmat <= 65536 ? 2 : 4);
+    for (int c = 0; c < 6; ++c) {
+        ctx->F[c] = ctx->Fmem[0] + c * ctx->fstride;
+        ctx->Fn[c] = ctx->Fmem[1] + c * ctx->fstride;
+    }
+    ctx->id_bytes = d->nmat <= 65536 ? 2 : 4;
     e = dalloc(ctx, (void **)&ctx->ID, 6 * (size_t)ctx->ntot * ctx->id_bytes, false);
     if (e != hipSuccess) goto fail_alloc;
     e = dalloc(ctx, (void **)&ctx->cE, (size_t)d->nmat * 5 * 4);
@@ -982,7 +723,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (!ctx) return 0;
     hipSetDevice(ctx->desc.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (auto &p : ctx->F) if (p) hipFree(p);
+    for (auto &p : ctx->Fmem) if (p) hipFree(p);
     for (auto &p : ctx->T) if (p) hipFree(p);
     if (ctx->ID) hipFree(ctx->ID);
     if (ctx->cE) hipFree(ctx->cE);
@@ -992,6 +733,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->dispflag) hipFree(ctx->dispflag);
     for (auto &s : ctx->slabs) {
         for (auto &p : s.phi) if (p) hipFree(p);
+        for (auto &p : s.phin) if (p) hipFree(p);
         if (s.eR) hipFree(s.eR);
         if (s.hR) hipFree(s.hR);
     }
@@ -1003,6 +745,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->rxc) hipFree(ctx->rxc);
     if (ctx->rxbuf) hipFree(ctx->rxbuf);
     if (ctx->planeflag) hipFree(ctx->planeflag);
+    if (ctx->rare) hipFree(ctx->rare);
     if (ctx->frame) hipFree(ctx->frame);
     for (auto &s : ctx->snaps) if (s.out) hipFree(s.out);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1039,12 +782,9 @@ extern "C" int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID)
     int rc;
     if (ctx->id_bytes == 4) {
         rc = copy_in(ctx, ctx->ID, ID, n * 4);
-    } else if (ctx->id_bytes == 2) {
+    } else {
         std::vector<uint16_t> c(ID, ID + n);
         rc = copy_in(ctx, ctx->ID, c.data(), n * 2);
-    } else {
-        std::vector<uint8_t> c(ID, ID + n);
-        rc = copy_in(ctx, ctx->ID, c.data(), n);
     }
     if (!rc) ctx->have_id = true;
     return rc;
@@ -1141,6 +881,7 @@ extern "C" int gpx_add_pml(gpx_ctx *ctx, int dir, int xs, int xf, int ys, int yf
     for (int q = 0; q < 4; ++q) {
         size_t b = (size_t)order * s.dims[q][0] * s.dims[q][1] * s.dims[q][2] * 4;
         GPX_TRY(dalloc(ctx, (void **)&s.phi[q], b));
+        if (q >= 2) GPX_TRY(dalloc(ctx, (void **)&s.phin[q - 2], b));
     }
     const size_t rb = (size_t)4 * order * t * 4;
     GPX_TRY(dalloc(ctx, (void **)&s.eR, rb, false));
@@ -1152,6 +893,7 @@ extern "C" int gpx_add_pml(gpx_ctx *ctx, int dir, int xs, int xf, int ys, int yf
         hipFree(s.eR);
         hipFree(s.hR);
         for (auto &p : s.phi) hipFree(p);
+        for (auto &p : s.phin) hipFree(p);
         return fail(ctx, GPX_ELIMIT, "gpx_add_pml: coefficient tables exceed the LDS budget");
     }
     ctx->slabs.push_back(s);
@@ -1239,6 +981,16 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
     GPX_TRY(dalloc(ctx, (void **)&s.out, (size_t)6 * snx * sny * snz * 4));
     ctx->snaps.push_back(s);
     if (index) *index = (int)ctx->snaps.size() - 1;
+    return 0;
+}
+
+extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
+{
+    if (!ctx || (fused != 0 && fused != 1) ||
+        !(tile_cells == 0 || tile_cells == 512 || tile_cells == 1024 || tile_cells == 2048))
+        return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
+    ctx->fused = fused;
+    ctx->tile_cells = tile_cells;
     return 0;
 }
 
@@ -1361,6 +1113,7 @@ static void fill_slab_args(const gpx_ctx *ctx, int field, int s, SlabArgs &a)
 
 static int update_planeflags(gpx_ctx *ctx)
 {
+    ctx->rare_dirty = true;  // sources or receivers may have changed
     const int NX1 = ctx->desc.nx + 1;
     std::vector<uint32_t> pf(NX1, 0);
     for (int r = 0; r < ctx->nrx; ++r) pf[ctx->rxhost[3 * r]] |= PF_RX;
@@ -1496,12 +1249,10 @@ template <bool BND>
 static pass_kernel pick_kernel(const gpx_ctx *ctx, int field)
 {
     if (field == 1)
-        return ctx->id_bytes == 1 ? k_h_pass<uint8_t, BND>
-                                  : (ctx->id_bytes == 2 ? k_h_pass<uint16_t, BND> : k_h_pass<uint32_t, BND>);
+        return ctx->id_bytes == 2 ? k_h_pass<uint16_t, BND> : k_h_pass<uint32_t, BND>;
     const int P = ctx->desc.maxpoles;
     const bool C = ctx->disp_complex != 0;
-    return ctx->id_bytes == 1 ? e_kernel<uint8_t, BND>(P, C)
-                              : (ctx->id_bytes == 2 ? e_kernel<uint16_t, BND>(P, C) : e_kernel<uint32_t, BND>(P, C));
+    return ctx->id_bytes == 2 ? e_kernel<uint16_t, BND>(P, C) : e_kernel<uint32_t, BND>(P, C);
 }
 
 // One pass = interior kernel (skipped when only PML work is asked for) then
@@ -1520,6 +1271,189 @@ static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it, 
     if (e != hipSuccess) return e;
     if (ev) e = hipEventRecord(ev[2], ctx->stream);
     return e;
+}
+
+// ---------------------------------------------------------------- fused step
+constexpr int STEP_LDS_MAX = 64 * 1024;
+
+// The fused kernels are compiled in step_*.hip, one file per PML form.
+namespace gpx {
+step_kernel step_kernel_plain(int id_bytes, int P, bool C);
+step_kernel step_kernel_hori1(int id_bytes, int P, bool C);
+step_kernel step_kernel_hori2(int id_bytes, int P, bool C);
+step_kernel step_kernel_mri1(int id_bytes, int P, bool C);
+step_kernel step_kernel_mri2(int id_bytes, int P, bool C);
+}  // namespace gpx
+
+// PML order shared by every slab, 0 without slabs, -1 when they differ.
+static int slab_order(const gpx_ctx *ctx)
+{
+    int o = 0;
+    for (const Slab &S : ctx->slabs) {
+        if (o && S.order != o) return -1;
+        o = S.order;
+    }
+    return o;
+}
+
+static step_kernel pick_step(const gpx_ctx *ctx)
+{
+    const int P = ctx->desc.maxpoles, idb = ctx->id_bytes;
+    const bool C = ctx->disp_complex != 0;
+    const int o = slab_order(ctx);
+    if (o == 0) return step_kernel_plain(idb, P, C);
+    if (ctx->desc.pml_formulation == 0)
+        return o == 1 ? step_kernel_hori1(idb, P, C) : step_kernel_hori2(idb, P, C);
+    return o == 1 ? step_kernel_mri1(idb, P, C) : step_kernel_mri2(idb, P, C);
+}
+
+// LDS of the fused step besides the H exchange area: both coefficient
+// tables, both R tables, dispersive coefficients and flags.
+static size_t step_table_lds(const gpx_ctx *ctx)
+{
+    const gpx_desc &g = ctx->desc;
+    size_t b = (size_t)g.nmat * 40 + 2 * ctx->Rhost[0].size() * 4;
+    if (g.maxpoles > 0) b += (size_t)g.nmat * 3 * g.maxpoles * 4 * (ctx->disp_complex ? 2 : 1) + g.nmat;
+    return b;
+}
+
+static size_t step_lds(const gpx_ctx *ctx, int tc) { return step_table_lds(ctx) + (size_t)12 * tc + 16; }
+
+// Cells per fused block (threads x STEP_CELLS_PER_THREAD): the smallest of
+// 512, 1024 and 2048 whose halo of nz+1 cells is at most an eighth of it, else
+// the largest whose halo is at most half; 0 when none fits the LDS budget, and
+// then the two-pass kernels run instead.
+static int step_tile(const gpx_ctx *ctx)
+{
+    if (!ctx->fused || slab_order(ctx) < 0) return 0;
+    const int NZ1 = ctx->desc.nz + 1;
+    // Streamed offsets are 32-bit: a state of six components and the six
+    // index arrays must each stay below 4 GiB.
+    if ((double)ctx->fstride * 24.0 >= 4294967295.0 || (double)ctx->ntot * 6 * ctx->id_bytes >= 4294967295.0)
+        return 0;
+    auto ok = [&](int tc) { return step_lds(ctx, tc) <= (size_t)STEP_LDS_MAX && 2 * NZ1 <= tc; };
+    if (ctx->tile_cells) return ok(ctx->tile_cells) ? ctx->tile_cells : 0;
+    int best = 0;
+    for (int tc : {512, 1024, 2048}) {
+        if (!ok(tc)) continue;
+        best = tc;
+        if (8 * NZ1 <= tc) break;
+    }
+    return best;
+}
+
+static void fill_step_args(const gpx_ctx *ctx, int tc, int it, FusedArgs &a)
+{
+    memset(&a, 0, sizeof(a));
+    const gpx_desc &g = ctx->desc;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+    a.plane = ctx->plane;
+    a.L = tc - (g.nz + 1);
+    a.ntiles = (ctx->plane + a.L - 1) / a.L;
+    a.Fc = ctx->F[0];
+    a.Fx = ctx->Fn[0];
+    a.cs = (uint32_t)(ctx->fstride * 4);
+    a.ID = ctx->ID;
+    a.ntot = ctx->ntot;
+    a.coefE = ctx->cE;
+    a.coefH = ctx->cH;
+    a.RE = ctx->Rall[0];
+    a.RH = ctx->Rall[1];
+    a.nR = (int)ctx->Rhost[0].size();
+    if (g.maxpoles > 0) {
+        a.disp = ctx->disp;
+        a.dispflag = ctx->dispflag;
+        for (int c = 0; c < 3; ++c) a.T[c] = ctx->T[c];
+    }
+    a.nmat = g.nmat;
+    a.form = g.pml_formulation;
+    a.nslab = (int)ctx->slabs.size();
+    for (int s = 0; s < a.nslab; ++s) {
+        fill_slab_args(ctx, 0, s, a.slabE[s]);
+        fill_slab_args(ctx, 1, s, a.slabH[s]);
+        a.slabH[s].phw1 = ctx->slabs[s].phin[0];
+        a.slabH[s].phw2 = ctx->slabs[s].phin[1];
+    }
+    a.it = it;
+    a.iterations = g.iterations;
+    a.rare = ctx->rare;
+    a.planeflag = ctx->planeflag;
+}
+
+// Items are (tile, x chunk); each chunk also recomputes H on the plane before
+// it.  Unless fixed by gpx_set_launch, the chunk length minimises
+// rounds x (planes per item), preferring longer chunks on ties.
+static void plan_step(const gpx_ctx *ctx, int resident, FusedArgs &a, int &grid)
+{
+    const int NX1 = ctx->desc.nx + 1;
+    int xc = ctx->xchunk;
+    if (!xc) {
+        long long best = -1;
+        for (int c = 2; c <= NX1; ++c) {
+            const long long items = (long long)a.ntiles * ((NX1 + c - 1) / c);
+            const long long cost = (items + resident - 1) / resident * (long long)(c + 1);
+            if (best < 0 || cost <= best) {
+                best = cost;
+                xc = c;
+            }
+        }
+    }
+    a.xc = xc;
+    a.nitems = a.ntiles * ((NX1 + xc - 1) / xc);
+    long long g = resident < a.nitems ? resident : a.nitems;
+    g = (g + 7) / 8 * 8;
+    a.per_xcd = (int)(g / 8);
+    grid = (int)g;
+}
+
+// One fused iteration, then the next state becomes the current one.
+// Copies the source and receiver tables of the fused step to the device.
+static hipError_t upload_rare(gpx_ctx *ctx)
+{
+    if (!ctx->rare) {
+        hipError_t e = dalloc(ctx, (void **)&ctx->rare, sizeof(StepRare), false);
+        if (e != hipSuccess) return e;
+    }
+    StepRare r;
+    memset(&r, 0, sizeof(r));
+    auto src = [&](int kind) {
+        const SrcSet &S = ctx->src[kind];
+        return SrcArgs{S.n, S.info, S.fac, S.waves};
+    };
+    r.srcM = src(GPX_SRC_MAGNETIC);
+    r.srcV = src(GPX_SRC_VOLTAGE);
+    r.srcD = src(GPX_SRC_HERTZIAN);
+    r.nrx = ctx->nrx;
+    r.rxc = ctx->rxc;
+    r.rxbuf = ctx->rxbuf;
+    hipError_t e = hipMemcpyAsync(ctx->rare, &r, sizeof(r), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) ctx->rare_dirty = false;
+    return e;
+}
+
+static hipError_t launch_step(gpx_ctx *ctx, int tc, int it)
+{
+    if (ctx->rare_dirty || !ctx->rare) {
+        hipError_t e = upload_rare(ctx);
+        if (e != hipSuccess) return e;
+    }
+    FusedArgs a;
+    fill_step_args(ctx, tc, it, a);
+    const step_kernel k = pick_step(ctx);
+    const int B = tc / STEP_CELLS_PER_THREAD;
+    const size_t lds = step_lds(ctx, tc);
+    int grid;
+    plan_step(ctx, resident_blocks(ctx, (const void *)k, B, lds), a, grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, ctx->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
+    for (auto &S : ctx->slabs) {
+        std::swap(S.phi[2], S.phin[0]);
+        std::swap(S.phi[3], S.phin[1]);
+    }
+    return hipSuccess;
 }
 
 static hipError_t launch_snapshot(const gpx_ctx *ctx, int s)
@@ -1557,12 +1491,15 @@ static int e_flags(const gpx_ctx *ctx)
     return f;
 }
 
-// Runs iterations [it0, it0+n).  With `ms4`, events around every kernel give
-// the mean duration of the H interior, H boundary, E interior and E boundary
-// kernels over the run (the host waits after each iteration to read them).
+// Runs iterations [it0, it0+n): one fused kernel per iteration when the grid
+// allows it, else the H and E passes.  With `ms4`, events around every kernel
+// give the mean duration of each kernel of an iteration over the run: the
+// fused kernel in ms4[0] (the rest 0), or H interior, H boundary, E interior
+// and E boundary (the host waits after each iteration to read them).
 static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
     const int hf = h_flags(ctx), ef = e_flags(ctx);
+    const int tc = step_tile(ctx);
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double acc[4] = {0, 0, 0, 0};
     if (ms4)
@@ -1570,6 +1507,18 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
     for (int it = it0; it < it0 + n; ++it) {
         for (size_t s = 0; s < ctx->snaps.size(); ++s)
             if (ctx->snaps[s].time == it + 1) GPX_TRY(launch_snapshot(ctx, (int)s));
+        if (tc) {
+            if (ms4) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            GPX_TRY(launch_step(ctx, tc, it));
+            if (ms4) {
+                GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+                GPX_TRY(hipEventSynchronize(ev[1]));
+                float t;
+                GPX_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+                acc[0] += t;
+            }
+            continue;
+        }
         GPX_TRY(launch_pass(ctx, 1, hf, it, ms4 ? ev : nullptr));
         GPX_TRY(launch_pass(ctx, 0, ef, it, ms4 ? ev + 3 : nullptr));
         if (ms4) {
@@ -1587,6 +1536,11 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
         for (int q = 0; q < 4; ++q) ms4[q] = (float)(acc[q] / (n > 0 ? n : 1));
     }
     return 0;
+}
+
+extern "C" int gpx_step_kind(const gpx_ctx *ctx)
+{
+    return ctx ? step_tile(ctx) : 0;
 }
 
 extern "C" int gpx_run(gpx_ctx *ctx, int it0, int n)
@@ -1691,8 +1645,11 @@ extern "C" int gpx_download_fields(gpx_ctx *ctx, float *out6)
 extern "C" int gpx_upload_fields(gpx_ctx *ctx, const float *in6)
 {
     if (!ctx || !in6) return fail(ctx, GPX_EARG, "gpx_upload_fields: null argument");
-    for (int c = 0; c < 6; ++c)
+    // Both states: cells that no update touches keep their value in either.
+    for (int c = 0; c < 6; ++c) {
         if (int rc = copy_in(ctx, ctx->F[c], in6 + (size_t)c * ctx->ntot, (size_t)ctx->ntot * 4)) return rc;
+        if (int rc = copy_in(ctx, ctx->Fn[c], in6 + (size_t)c * ctx->ntot, (size_t)ctx->ntot * 4)) return rc;
+    }
     return 0;
 }
 
@@ -1721,6 +1678,8 @@ extern "C" int gpx_upload_pml_phi(gpx_ctx *ctx, int slab, int which, const float
     float *p;
     size_t b;
     if (int rc = phi_ref(ctx, slab, which, &p, &b)) return rc;
+    if (which >= 2)  // H Phi is double-buffered like the fields
+        if (int rc = copy_in(ctx, ctx->slabs[slab].phin[which - 2], in, b)) return rc;
     return copy_in(ctx, p, in, b);
 }
 

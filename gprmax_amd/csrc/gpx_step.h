@@ -1,0 +1,730 @@
+// Device code shared by the translation units of libgprmax_hip.so: launch
+// argument types, buffer-resource helpers, the PML, dispersive and table
+// helpers, and the fused time-step kernel template k_step (instantiated per
+// PML formulation and order in step_*.hip).
+#ifndef GPX_STEP_H
+#define GPX_STEP_H
+
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gpx {
+
+constexpr int MAX_SLABS = 6;
+constexpr int MAX_POLES = 4;
+constexpr int MAX_LDS_BYTES = 64 * 1024;
+
+enum : int {
+    F_MAIN = 1,
+    F_SRC = 2,
+    F_RX = 4,
+    F_DISPB = 8,
+    F_PML0 = 16,  // bits 4..9: one per slab
+};
+constexpr int F_PML_ALL = F_PML0 * ((1 << MAX_SLABS) - 1);
+
+struct SlabArgs {
+    int axis;        // 0 x, 1 y, 2 z
+    int minus;       // 1 for xminus/yminus/zminus
+    int lo[3], hi[3];// box of corrected cells for this field, [lo, hi)
+    int pend;        // profile index p = pend - c (minus) or c - pend (plus)
+    int org[3];      // slab origin (xs, ys, zs) for the Phi index
+    int d1[3], d2[3];// Phi1/Phi2 dims without the order axis
+    int t, order;
+    int roff;        // offset of this slab's [4][order][t] table in the LDS R area
+    float d;
+    float *phi1, *phi2;   // Phi read
+    float *phw1, *phw2;   // Phi written (the same arrays except in the fused step)
+};
+
+struct SrcArgs {
+    int n;
+    const int *info;     // [n][6]
+    const float *fac;    // [n][2]
+    const float *waves;  // [n][iterations]
+};
+
+constexpr uint32_t PF_RX = 1, PF_SRCH = 2, PF_SRCE = 4;
+constexpr int PF_SLAB_E = 8, PF_SLAB_H = 16;
+
+// Streamed arrays are read and written with buffer instructions: a scalar
+// resource per array (base address in SGPRs), the lane's byte offset within
+// the plane in a VGPR and the plane's byte offset in an SGPR, so no
+// per-access vector address arithmetic is needed.  Offsets are 32-bit: an
+// array holds fewer than 2^30 floats.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
+}
+// The b32 buffer builtins move raw 32-bit words; floats pass through as bits.
+__device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+template <class IDT>
+__device__ __forceinline__ uint32_t bld_id(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    if (sizeof(IDT) == 1) return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
+    if (sizeof(IDT) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
+// Loads and stores at a uniform base pointer plus a 32-bit lane byte offset.
+template <class T>
+__device__ __forceinline__ T ld(const T *base, uint32_t boff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void st(T *base, uint32_t boff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+// ---------------------------------------------------------------- PML
+// One corrected component (gprMax/pml_updates/*_ext.pyx) for formulation
+// FORM (0 HORIPML, 1 MRIPML) and ORDER CFS terms.  R = [RA, RB, RE, RF] each
+// [order][t] in LDS; i1/off index Phi order 0 and the order-1 offset.  Phi is
+// read from `pin` and, when `wr`, the advanced values are written to `pout`
+// (the same array for an in-place update).
+template <int FORM, int ORDER>
+__device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p, float F,
+                                       bool plus_sign, float c, float dd, const float *pin,
+                                       float *pout, bool wr, uint32_t i1, uint32_t off)
+{
+    const int ot = ORDER * s.t;
+    float corr;
+    if (FORM == 0) {
+        if (ORDER == 1) {  // pml_updates_*_HORIPML_ext.pyx, order1_*
+            const float RA01 = R[p] - 1.0f;
+            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
+            const float ph = pin[i1];
+            corr = c * (RA01 * dd + RB0 * ph);
+            if (wr) pout[i1] = RE0 * ph - RF0 * dd;
+        } else {           // order2_*
+            const float RA0 = R[p], RA1 = R[s.t + p];
+            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
+            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
+            const float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
+            const float RA01 = RA0 * RA1 - 1.0f;
+            const float p0 = pin[i1], p1 = pin[i1 + off];
+            corr = c * (RA01 * dd + RA1 * RB0 * p0 + RB1 * p1);
+            if (wr) {
+                pout[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
+                pout[i1] = RE0 * p0 - RF0 * dd;
+            }
+        }
+    } else {
+        if (ORDER == 1) {  // pml_updates_*_MRIPML_ext.pyx, order1_*
+            const float IRA = 1.0f / R[p];
+            const float IRA1 = IRA - 1.0f;
+            const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
+            const float RC0 = IRA * RB0 * RF0;
+            const float ph = pin[i1];
+            corr = c * (IRA1 * dd - IRA * ph);
+            if (wr) pout[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
+        } else {           // order2_*
+            const float IRA = 1.0f / (R[p] + R[s.t + p]);
+            const float IRA1 = IRA - 1.0f;
+            const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
+            const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
+            const float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
+            const float p0 = pin[i1], p1 = pin[i1 + off];
+            const float psi = RB0 * p0 + RB1 * p1;
+            corr = c * (IRA1 * dd - IRA * psi);
+            if (wr) {
+                pout[i1 + off] = RE1 * p1 + RC1 * (dd - psi);
+                pout[i1] = RE0 * p0 + RC0 * (dd - psi);
+            }
+        }
+    }
+    return plus_sign ? F + corr : F - corr;
+}
+
+// The same with formulation and order chosen at run time.
+__device__ __forceinline__ float pml_apply(int form, const SlabArgs &s, const float *R, int p,
+                                           float F, bool plus_sign, float c, float dd,
+                                           const float *pin, float *pout, bool wr, uint32_t i1,
+                                           uint32_t off)
+{
+    if (form == 0)
+        return s.order == 1 ? pml_t<0, 1>(s, R, p, F, plus_sign, c, dd, pin, pout, wr, i1, off)
+                            : pml_t<0, 2>(s, R, p, F, plus_sign, c, dd, pin, pout, wr, i1, off);
+    return s.order == 1 ? pml_t<1, 1>(s, R, p, F, plus_sign, c, dd, pin, pout, wr, i1, off)
+                        : pml_t<1, 2>(s, R, p, F, plus_sign, c, dd, pin, pout, wr, i1, off);
+}
+
+__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
+                                             float F, bool plus_sign, float c, float dd,
+                                             float *phi, uint32_t i1, uint32_t off)
+{
+    return pml_apply(form, s, R, p, F, plus_sign, c, dd, phi, phi, true, i1, off);
+}
+
+__device__ __forceinline__ bool in_box(const SlabArgs &s, int i, int j, int k)
+{
+    return i >= s.lo[0] && i < s.hi[0] && j >= s.lo[1] && j < s.hi[1] && k >= s.lo[2] &&
+           k < s.hi[2];
+}
+
+// Profile index and the two Phi element indices of cell (i, j, k).
+__device__ __forceinline__ void slab_index(const SlabArgs &s, int i, int j, int k, int &p,
+                                           uint32_t &i1, uint32_t &i2, uint32_t &o1, uint32_t &o2)
+{
+    int c[3] = {i - s.org[0], j - s.org[1], k - s.org[2]};
+    const int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
+    p = s.minus ? s.pend - along : along - s.pend;
+    c[s.axis] = p;
+    i1 = ((uint32_t)c[0] * s.d1[1] + c[1]) * s.d1[2] + c[2];
+    i2 = ((uint32_t)c[0] * s.d2[1] + c[1]) * s.d2[2] + c[2];
+    o1 = (uint32_t)s.d1[0] * s.d1[1] * s.d1[2];
+    o2 = (uint32_t)s.d2[0] * s.d2[1] * s.d2[2];
+}
+
+// Persistent grid: blocks are dealt to the 8 XCDs round robin, so block b is
+// renumbered (b mod 8) * per_xcd + b / 8; in each round an XCD then works on a
+// contiguous run of items, i.e. neighbouring plane segments that share rows
+// through its L2.
+__device__ __forceinline__ int logical_block(int per_xcd)
+{
+    const int b = blockIdx.x;
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+// Copy a small table to LDS with four loads in flight per thread.
+// Slab descriptions go to LDS so the PML branch reads them into VGPRs when a
+// lane needs them instead of holding them in SGPRs for the whole loop.
+__device__ __forceinline__ void stage_slabs(SlabArgs *dst, const SlabArgs *slab, int nslab)
+{
+    const int words = (int)(sizeof(SlabArgs) / 4) * nslab;
+    const int *src = reinterpret_cast<const int *>(slab);
+    int *d = reinterpret_cast<int *>(dst);
+    for (int q = threadIdx.x; q < words; q += blockDim.x) d[q] = src[q];
+}
+
+__device__ __forceinline__ void stage(float *dst, const float *src, int n)
+{
+    const int B = blockDim.x;
+    for (int q = threadIdx.x; q < n; q += 4 * B) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = q + u * B < n ? src[q + u * B] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q + u * B < n) dst[q + u * B] = v[u];
+    }
+}
+
+// Coefficients in LDS: (CA, CBx, CBy, CBz) or (DA, DBx, DBy, DBz) as one
+// float4 per material, then srce/srcm as one float per material.
+__device__ __forceinline__ void stage_coef(float4 *c4, float *c1, const float *src, int nmat)
+{
+    for (int m = threadIdx.x; m < nmat; m += blockDim.x) {
+        const float *r = src + 5 * m;
+        c4[m] = make_float4(r[0], r[1], r[2], r[3]);
+        c1[m] = r[4];
+    }
+}
+
+// Dispersive part A for one component (gprMax/fields_updates_ext.pyx:113-296):
+// returns phi and advances the P auxiliary values from the old field e.
+template <int P, bool CPLX>
+__device__ __forceinline__ float disp_a(const float *D, uint32_t m, float *tr, float *ti, float e)
+{
+    float phi = 0.f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (CPLX) {
+            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
+            if (P == 1) phi = c[0] * tr[p];
+            else phi = phi + c[0] * tr[p];
+            const float er = c[2], ei = c[3], zr = c[4], zi = c[5];
+            const float nr = (er * tr[p] - ei * ti[p]) + zr * e;
+            const float ni = (er * ti[p] + ei * tr[p]) + zi * e;
+            tr[p] = nr;
+            ti[p] = ni;
+        } else {
+            const float *c = D + (size_t)m * 3 * P + 3 * p;
+            if (P == 1) phi = c[0] * tr[p];
+            else phi = phi + c[0] * tr[p];
+            tr[p] = c[1] * tr[p] + c[2] * e;
+        }
+    }
+    return phi;
+}
+
+// Dispersive part B (gprMax/fields_updates_ext.pyx:183-235, 299-346).
+template <int P, bool CPLX>
+__device__ __forceinline__ void disp_b(const float *D, uint32_t m, float *tr, float *ti, float e)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (CPLX) {
+            const float *c = D + ((size_t)m * 3 * P + 3 * p) * 2;
+            tr[p] = tr[p] - c[4] * e;
+            ti[p] = ti[p] - c[5] * e;
+        } else {
+            tr[p] = tr[p] - D[(size_t)m * 3 * P + 3 * p + 2] * e;
+        }
+    }
+}
+
+template <int P, bool CPLX>
+__device__ __forceinline__ void t_load(float *const *T, int c, size_t N, size_t po, uint32_t to,
+                                       float *tr, float *ti)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (CPLX) {
+            const float2 v = ld(reinterpret_cast<const float2 *>(T[c]) + p * N + po, 2 * to);
+            tr[p] = v.x;
+            ti[p] = v.y;
+        } else {
+            tr[p] = ld((const float *)T[c] + p * N + po, to);
+            ti[p] = 0.f;
+        }
+    }
+}
+
+template <int P, bool CPLX>
+__device__ __forceinline__ void t_store(float *const *T, int c, size_t N, size_t po, uint32_t to,
+                                        const float *tr, const float *ti)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (CPLX)
+            st(reinterpret_cast<float2 *>(T[c]) + p * N + po, 2 * to, make_float2(tr[p], ti[p]));
+        else
+            st(T[c] + p * N + po, to, tr[p]);
+    }
+}
+
+// ---------------------------------------------------------------- fused step
+// One whole iteration -- the H pass and then the E pass -- in one kernel.  The
+// state is double-buffered: fields and H-PML Phi are read from the current
+// arrays and written to the next ones, so no block ever reads what another
+// block writes in the same launch and blocks need no ordering.  (Dispersive T
+// and E-PML Phi are read and written only by the owner of the cell, in place.)
+//
+// Work item = (tile, x chunk).  A tile is a run of L consecutive cells of the
+// flattened (j, k) plane, preceded by the nz+1 cells before it: that halo holds
+// the (j-1) and (k-1) neighbours of every tile cell.  A block of B threads
+// holds B*CPT = nz+1+L cells, thread t the cells t, t+B, ...  For each plane i
+// of the chunk, and first for the plane before it, the block computes new H
+// for all its cells (the halo cells and the halo plane are recomputed, never
+// written), shares them through LDS, and computes new E for the tile cells.
+// The new Hy, Hz of plane i-1 that E at plane i needs stay in registers.
+// Sources and receivers of the fused step.  They are read only on planes that
+// hold one, from device memory, so their addresses do not occupy registers
+// for the whole kernel.
+struct StepRare {
+    SrcArgs srcM, srcV, srcD;    // magnetic dipoles, voltage sources, Hertzian dipoles
+    int nrx;
+    const int *rxc;              // [nrx][3]
+    float *rxbuf;                // [6][iterations][nrx]
+};
+
+struct FusedArgs {
+    int nx, ny, nz, plane;
+    int L, ntiles, nitems, per_xcd, xc;
+    const float *Fc;             // current state: Ex, Ey, Ez, Hx, Hy, Hz, `cs` bytes apart
+    float *Fx;                   // next state, same layout
+    uint32_t cs;                 // bytes between the components of a state
+    const void *ID;              // [6][ntot] compact material indices
+    long long ntot;
+    const float *coefE, *coefH;
+    const float *RE, *RH;        // all slabs' R tables, E and H
+    int nR;
+    const float *disp;
+    const uint8_t *dispflag;
+    float *T[3];
+    int nmat, form, nslab;
+    SlabArgs slabE[MAX_SLABS], slabH[MAX_SLABS];  // slabH: Phi read current, written next
+    int it, iterations;
+    const StepRare *rare;
+    const uint32_t *planeflag;
+};
+
+template <class IDT, int P, bool CPLX, int FORM, int ORDER, int CPT>
+__global__ void __launch_bounds__(1024) k_step(FusedArgs a)
+{
+    extern __shared__ float4 lds4[];
+    __shared__ SlabArgs lsE[MAX_SLABS], lsH[MAX_SLABS];
+    const int nm = a.nmat, B = blockDim.x, TC = B * CPT;
+    const int nd = P > 0 ? nm * 3 * P * (CPLX ? 2 : 1) : 0;
+    float4 *cE = lds4, *cH = lds4 + nm;
+    float *sE = reinterpret_cast<float *>(lds4 + 2 * nm), *sH = sE + nm;
+    float *lRE = sH + nm, *lRH = lRE + a.nR;
+    float *ldisp = lRH + a.nR;
+    float *lHx = ldisp + nd, *lHy = lHx + TC, *lHz = lHy + TC;
+    uint8_t *lflag = reinterpret_cast<uint8_t *>(lHz + TC);
+    stage_coef(cE, sE, a.coefE, nm);
+    stage_coef(cH, sH, a.coefH, nm);
+    stage(lRE, a.RE, a.nR);
+    stage(lRH, a.RH, a.nR);
+    stage_slabs(lsE, a.slabE, a.nslab);
+    stage_slabs(lsH, a.slabH, a.nslab);
+    if (P > 0) {
+        stage(ldisp, a.disp, nd);
+        for (int q = threadIdx.x; q < nm; q += B) lflag[q] = a.dispflag[q];
+    }
+    __syncthreads();
+
+    const int nx = a.nx, ny = a.ny, nz = a.nz, NZ1 = nz + 1;
+    const size_t N = a.ntot, Pl = a.plane;
+    // Three buffer resources cover everything streamed: the current state,
+    // the next state and the material indices; the component is part of the
+    // scalar offset.
+    const rsrc_t rF = rsrc(a.Fc), wF = rsrc(a.Fx), rI = rsrc(a.ID);
+    const uint32_t cs = a.cs, ci = (uint32_t)N * (uint32_t)sizeof(IDT);
+    const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
+
+    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
+        const int tile = item % a.ntiles, chunk = item / a.ntiles;
+        const int q0 = tile * a.L - NZ1;
+        const int qe = min(tile * a.L + a.L, a.plane);
+        const int xa = chunk * a.xc, xb = min(xa + a.xc, nx + 1);
+        const int xs = xa > 0 ? xa - 1 : xa;
+
+        uint32_t to[CPT];      // byte offset of the cell within a plane (0 for dead lanes)
+        uint32_t jk[CPT];      // j << 16 | k
+        // bit 0: live cell; bit 1: tile cell (not halo); bits 2..7: slabs whose
+        // (j, k) range holds the cell for E, bits 8..13 the same for H
+        uint32_t st[CPT];
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int t = threadIdx.x + u * B;
+            const int q = q0 + t;
+            const bool live = q >= 0 && q < qe;
+            const int qq = live ? q : 0;
+            const int j = qq / NZ1, k = qq - j * NZ1;
+            to[u] = (uint32_t)qq * 4u;
+            jk[u] = ((uint32_t)j << 16) | (uint32_t)k;
+            uint32_t me = 0, mh = 0;
+            for (int s = 0; ORDER > 0 && s < a.nslab; ++s) {
+                if (j >= lsE[s].lo[1] && j < lsE[s].hi[1] && k >= lsE[s].lo[2] && k < lsE[s].hi[2]) me |= 1u << s;
+                if (j >= lsH[s].lo[1] && j < lsH[s].hi[1] && k >= lsH[s].lo[2] && k < lsH[s].hi[2]) mh |= 1u << s;
+            }
+            st[u] = live ? (1u | (t >= NZ1 ? 2u : 0u) | (me << 2) | (mh << 8)) : 0u;
+        }
+
+        // The loads of one plane for one cell: old E of the next plane, old H
+        // and material indices of this plane, E neighbours in y and z.
+        struct PlaneLoad {
+            float ex1, ey1, ez1, hx, hy, hz, ezj, exj, eyk, exk;
+            uint32_t m0, m1, m2, m3, m4, m5;
+        };
+        auto load = [&](int ip, PlaneLoad *L) {
+            const uint32_t so = (uint32_t)ip * P4;
+            const uint32_t sn = (uint32_t)(ip < nx ? ip + 1 : ip) * P4;
+            const uint32_t si = (uint32_t)ip * Pid;
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                const int j = (int)(jk[u] >> 16), k = (int)(jk[u] & 0xffffu);
+                // Neighbours past the grid edge are clamped to the cell itself
+                // (a +0 difference, which no update applied there uses).
+                const uint32_t toj = j < ny ? to[u] + 4u * NZ1 : to[u];
+                const uint32_t tok = k < nz ? to[u] + 4u : to[u];
+                const uint32_t ti = (to[u] >> 2) * (uint32_t)sizeof(IDT);
+                L[u].ex1 = bld(rF, to[u], sn);
+                L[u].ey1 = bld(rF, to[u], sn + cs);
+                L[u].ez1 = bld(rF, to[u], sn + 2 * cs);
+                L[u].hx = bld(rF, to[u], so + 3 * cs);
+                L[u].hy = bld(rF, to[u], so + 4 * cs);
+                L[u].hz = bld(rF, to[u], so + 5 * cs);
+                L[u].ezj = bld(rF, toj, so + 2 * cs);
+                L[u].exj = bld(rF, toj, so);
+                L[u].eyk = bld(rF, tok, so + cs);
+                L[u].exk = bld(rF, tok, so);
+                L[u].m0 = bld_id<IDT>(rI, ti, si);
+                L[u].m1 = bld_id<IDT>(rI, ti, si + ci);
+                L[u].m2 = bld_id<IDT>(rI, ti, si + 2 * ci);
+                L[u].m3 = bld_id<IDT>(rI, ti, si + 3 * ci);
+                L[u].m4 = bld_id<IDT>(rI, ti, si + 4 * ci);
+                L[u].m5 = bld_id<IDT>(rI, ti, si + 5 * ci);
+            }
+        };
+
+        float ex[CPT], ey[CPT], ez[CPT], hym[CPT], hzm[CPT];
+        {
+            const uint32_t so = (uint32_t)xs * P4;
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                ex[u] = bld(rF, to[u], so);
+                ey[u] = bld(rF, to[u], so + cs);
+                ez[u] = bld(rF, to[u], so + 2 * cs);
+                hym[u] = 0.f;
+                hzm[u] = 0.f;
+            }
+        }
+        PlaneLoad cur[CPT];
+        load(xs, cur);
+
+        for (int i = xs; i < xb; ++i) {
+            const bool hp = i < xa;  // halo plane: H only, nothing written
+            const uint32_t pf = a.planeflag[i];
+            const uint32_t so = (uint32_t)i * P4;
+
+            float ex1[CPT], ey1[CPT], ez1[CPT], hx[CPT], hy[CPT], hz[CPT];
+            float ezj[CPT], exj[CPT], eyk[CPT], exk[CPT];
+            uint32_t m0[CPT], m1[CPT], m2[CPT], m3[CPT], m4[CPT], m5[CPT];
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                ex1[u] = cur[u].ex1; ey1[u] = cur[u].ey1; ez1[u] = cur[u].ez1;
+                hx[u] = cur[u].hx; hy[u] = cur[u].hy; hz[u] = cur[u].hz;
+                ezj[u] = cur[u].ezj; exj[u] = cur[u].exj; eyk[u] = cur[u].eyk; exk[u] = cur[u].exk;
+                m0[u] = cur[u].m0; m1[u] = cur[u].m1; m2[u] = cur[u].m2;
+                m3[u] = cur[u].m3; m4[u] = cur[u].m4; m5[u] = cur[u].m5;
+            }
+
+            // ---- H: update_magnetic, H-PML slabs in order, magnetic dipoles
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                const int j = (int)(jk[u] >> 16), k = (int)(jk[u] & 0xffffu);
+                const bool own = (st[u] & 2u) && !hp;
+                const float dEz_y = ezj[u] - ez[u], dEy_z = eyk[u] - ey[u], dEx_z = exk[u] - ex[u];
+                const float dEz_x = ez1[u] - ez[u], dEy_x = ey1[u] - ey[u], dEx_y = exj[u] - ex[u];
+                float nhx = hx[u], nhy = hy[u], nhz = hz[u];
+                if (i >= 1 && j < ny && k < nz) {
+                    const float4 q = cH[m3[u]];
+                    nhx = q.x * nhx - q.z * dEz_y + q.w * dEy_z;
+                }
+                if (i < nx && j >= 1 && k < nz) {
+                    const float4 q = cH[m4[u]];
+                    nhy = q.x * nhy - q.w * dEx_z + q.y * dEz_x;
+                }
+                if (i < nx && j < ny && k >= 1) {
+                    const float4 q = cH[m5[u]];
+                    nhz = q.x * nhz - q.y * dEy_x + q.z * dEx_y;
+                }
+                const uint32_t smask = (st[u] >> 8) & (pf >> PF_SLAB_H) & 63u;
+                if (ORDER > 0 && smask) {
+                    for (int s = 0; s < a.nslab; ++s) {
+                        if (!(smask & (1u << s))) continue;
+                        const SlabArgs &sl = lsH[s];
+                        int p;
+                        uint32_t q1, q2, o1, o2;
+                        slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                        const float *R = lRH + sl.roff;
+                        if (sl.axis == 0) {
+                            nhy = pml_t<FORM, ORDER>(sl, R, p, nhy, true, sH[m4[u]], dEz_x / sl.d, sl.phi1, sl.phw1, own, q1, o1);
+                            nhz = pml_t<FORM, ORDER>(sl, R, p, nhz, false, sH[m5[u]], dEy_x / sl.d, sl.phi2, sl.phw2, own, q2, o2);
+                        } else if (sl.axis == 1) {
+                            nhx = pml_t<FORM, ORDER>(sl, R, p, nhx, false, sH[m3[u]], dEz_y / sl.d, sl.phi1, sl.phw1, own, q1, o1);
+                            nhz = pml_t<FORM, ORDER>(sl, R, p, nhz, true, sH[m5[u]], dEx_y / sl.d, sl.phi2, sl.phw2, own, q2, o2);
+                        } else {
+                            nhx = pml_t<FORM, ORDER>(sl, R, p, nhx, true, sH[m3[u]], dEy_z / sl.d, sl.phi1, sl.phw1, own, q1, o1);
+                            nhy = pml_t<FORM, ORDER>(sl, R, p, nhy, false, sH[m4[u]], dEx_z / sl.d, sl.phi2, sl.phw2, own, q2, o2);
+                        }
+                    }
+                }
+                if ((pf & PF_SRCH) && (st[u] & 1u)) {  // magnetic dipoles, gprMax/sources.py:202-232
+                    const SrcArgs &M = a.rare->srcM;
+                    for (int s = 0; s < M.n; ++s) {
+                        const int *in = M.info + 6 * s;
+                        if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                        if (in[1] == j && in[2] == k) {
+                            const int pol = in[3];
+                            const uint32_t m = pol == 0 ? m3[u] : (pol == 1 ? m4[u] : m5[u]);
+                            const float w = M.waves[(size_t)s * a.iterations + a.it];
+                            const float v = sH[m] * w * M.fac[2 * s];
+                            if (pol == 0) nhx = nhx - v;
+                            else if (pol == 1) nhy = nhy - v;
+                            else nhz = nhz - v;
+                        }
+                    }
+                }
+                if (own && (pf & PF_RX)) {  // the state at the start of the iteration
+                    const StepRare &Q = *a.rare;
+                    for (int r = 0; r < Q.nrx; ++r) {
+                        if (Q.rxc[3 * r] != i || Q.rxc[3 * r + 1] != j || Q.rxc[3 * r + 2] != k) continue;
+                        float *o = Q.rxbuf + (size_t)a.it * Q.nrx + r;
+                        const size_t cs = (size_t)a.iterations * Q.nrx;
+                        o[0] = ex[u]; o[cs] = ey[u]; o[2 * cs] = ez[u];
+                        o[3 * cs] = hx[u]; o[4 * cs] = hy[u]; o[5 * cs] = hz[u];
+                    }
+                }
+                if (own) {
+                    bst(wF, to[u], so + 3 * cs, nhx);
+                    bst(wF, to[u], so + 4 * cs, nhy);
+                    bst(wF, to[u], so + 5 * cs, nhz);
+                }
+                const int t = threadIdx.x + u * B;
+                lHx[t] = nhx;
+                lHy[t] = nhy;
+                lHz[t] = nhz;
+                hx[u] = nhx;
+                hy[u] = nhy;
+                hz[u] = nhz;
+            }
+            // The next plane's loads fly while the block waits and updates E.
+            if (i + 1 < xb) load(i + 1, cur);
+            __syncthreads();
+
+            // ---- E: update_electric or dispersive A, E-PML slabs in order,
+            // voltage sources, Hertzian dipoles, dispersive B
+            if (!hp) {
+#pragma unroll
+                for (int u = 0; u < CPT; ++u) {
+                    if (!(st[u] & 2u)) continue;
+                    const int j = (int)(jk[u] >> 16), k = (int)(jk[u] & 0xffffu);
+                    const int t = threadIdx.x + u * B;
+                    const float hzj = lHz[t - NZ1], hxj = lHx[t - NZ1];
+                    const float hyk = lHy[t - 1], hxk = lHx[t - 1];
+                    const float dHz_y = hz[u] - hzj, dHy_z = hy[u] - hyk, dHx_z = hx[u] - hxk;
+                    const float dHz_x = hz[u] - hzm[u], dHy_x = hy[u] - hym[u], dHx_y = hx[u] - hxj;
+                    float nex = ex[u], ney = ey[u], nez = ez[u];
+                    const size_t po = (size_t)i * Pl;
+                    float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
+                    float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
+                    float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
+                    const bool rx_ = i < nx && j >= 1 && j < ny && k >= 1 && k < nz;
+                    const bool ry_ = i >= 1 && i < nx && j < ny && k >= 1 && k < nz;
+                    const bool rz_ = i >= 1 && i < nx && j >= 1 && j < ny && k < nz;
+                    const bool dx_ = P > 0 && rx_ && lflag[m0[u]];
+                    const bool dy_ = P > 0 && ry_ && lflag[m1[u]];
+                    const bool dz_ = P > 0 && rz_ && lflag[m2[u]];
+                    if (P > 0) {
+                        if (dx_) t_load<P, CPLX>(a.T, 0, N, po, to[u], txr, txi);
+                        if (dy_) t_load<P, CPLX>(a.T, 1, N, po, to[u], tyr, tyi);
+                        if (dz_) t_load<P, CPLX>(a.T, 2, N, po, to[u], tzr, tzi);
+                    }
+                    if (rx_) {
+                        const float4 q = cE[m0[u]];
+                        float v = q.x * nex + q.z * dHz_y - q.w * dHy_z;
+                        if (P > 0) {
+                            const float phi = dx_ ? disp_a<P, CPLX>(ldisp, m0[u], txr, txi, nex) : 0.f;
+                            v = v - sE[m0[u]] * phi;
+                        }
+                        nex = v;
+                    }
+                    if (ry_) {
+                        const float4 q = cE[m1[u]];
+                        float v = q.x * ney + q.w * dHx_z - q.y * dHz_x;
+                        if (P > 0) {
+                            const float phi = dy_ ? disp_a<P, CPLX>(ldisp, m1[u], tyr, tyi, ney) : 0.f;
+                            v = v - sE[m1[u]] * phi;
+                        }
+                        ney = v;
+                    }
+                    if (rz_) {
+                        const float4 q = cE[m2[u]];
+                        float v = q.x * nez + q.y * dHy_x - q.z * dHx_y;
+                        if (P > 0) {
+                            const float phi = dz_ ? disp_a<P, CPLX>(ldisp, m2[u], tzr, tzi, nez) : 0.f;
+                            v = v - sE[m2[u]] * phi;
+                        }
+                        nez = v;
+                    }
+                    const uint32_t smask = (st[u] >> 2) & (pf >> PF_SLAB_E) & 63u;
+                    if (ORDER > 0 && smask) {
+                        for (int s = 0; s < a.nslab; ++s) {
+                            if (!(smask & (1u << s))) continue;
+                            const SlabArgs &sl = lsE[s];
+                            int p;
+                            uint32_t q1, q2, o1, o2;
+                            slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                            const float *R = lRE + sl.roff;
+                            if (sl.axis == 0) {
+                                ney = pml_t<FORM, ORDER>(sl, R, p, ney, false, sE[m1[u]], dHz_x / sl.d, sl.phi1, sl.phi1, true, q1, o1);
+                                nez = pml_t<FORM, ORDER>(sl, R, p, nez, true, sE[m2[u]], dHy_x / sl.d, sl.phi2, sl.phi2, true, q2, o2);
+                            } else if (sl.axis == 1) {
+                                nex = pml_t<FORM, ORDER>(sl, R, p, nex, true, sE[m0[u]], dHz_y / sl.d, sl.phi1, sl.phi1, true, q1, o1);
+                                nez = pml_t<FORM, ORDER>(sl, R, p, nez, false, sE[m2[u]], dHx_y / sl.d, sl.phi2, sl.phi2, true, q2, o2);
+                            } else {
+                                nex = pml_t<FORM, ORDER>(sl, R, p, nex, false, sE[m0[u]], dHy_z / sl.d, sl.phi1, sl.phi1, true, q1, o1);
+                                ney = pml_t<FORM, ORDER>(sl, R, p, ney, true, sE[m1[u]], dHx_z / sl.d, sl.phi2, sl.phi2, true, q2, o2);
+                            }
+                        }
+                    }
+                    if (pf & PF_SRCE) {
+                        // Voltage sources, then Hertzian dipoles (gprMax/model_build_run.py:458-461).
+                        const SrcArgs &V = a.rare->srcV, &D = a.rare->srcD;
+                        for (int s = 0; s < V.n; ++s) {  // gprMax/sources.py:81-120
+                            const int *in = V.info + 6 * s;
+                            if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                            if (in[1] == j && in[2] == k) {
+                                const int pol = in[3] & 3;
+                                const bool hard = in[3] >= 4;
+                                const uint32_t m = pol == 0 ? m0[u] : (pol == 1 ? m1[u] : m2[u]);
+                                const float w = V.waves[(size_t)s * a.iterations + a.it];
+                                float *F = pol == 0 ? &nex : (pol == 1 ? &ney : &nez);
+                                if (hard) *F = -w / V.fac[2 * s];
+                                else *F = *F - sE[m] * w * V.fac[2 * s];
+                            }
+                        }
+                        for (int s = 0; s < D.n; ++s) {  // gprMax/sources.py:163-193
+                            const int *in = D.info + 6 * s;
+                            if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                            if (in[1] == j && in[2] == k) {
+                                const int pol = in[3];
+                                const uint32_t m = pol == 0 ? m0[u] : (pol == 1 ? m1[u] : m2[u]);
+                                const float w = D.waves[(size_t)s * a.iterations + a.it];
+                                const float v = sE[m] * w * D.fac[2 * s] * D.fac[2 * s + 1];
+                                if (pol == 0) nex = nex - v;
+                                else if (pol == 1) ney = ney - v;
+                                else nez = nez - v;
+                            }
+                        }
+                    }
+                    if (P > 0) {
+                        if (dx_) { disp_b<P, CPLX>(ldisp, m0[u], txr, txi, nex); t_store<P, CPLX>(a.T, 0, N, po, to[u], txr, txi); }
+                        if (dy_) { disp_b<P, CPLX>(ldisp, m1[u], tyr, tyi, ney); t_store<P, CPLX>(a.T, 1, N, po, to[u], tyr, tyi); }
+                        if (dz_) { disp_b<P, CPLX>(ldisp, m2[u], tzr, tzi, nez); t_store<P, CPLX>(a.T, 2, N, po, to[u], tzr, tzi); }
+                    }
+                    bst(wF, to[u], so, nex);
+                    bst(wF, to[u], so + cs, ney);
+                    bst(wF, to[u], so + 2 * cs, nez);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                hym[u] = hy[u];
+                hzm[u] = hz[u];
+                ex[u] = ex1[u];
+                ey[u] = ey1[u];
+                ez[u] = ez1[u];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+typedef void (*step_kernel)(FusedArgs);
+
+// Cells per thread of the fused step: a block of B threads holds B x this.
+constexpr int STEP_CELLS_PER_THREAD = 2;
+
+// Fused kernel for the given material-index width, poles and coefficient kind.
+template <int FORM, int ORDER>
+step_kernel step_kernel_table(int id_bytes, int P, bool C)
+{
+    constexpr int K = STEP_CELLS_PER_THREAD;
+    if (id_bytes == 2) {
+        switch (P) {
+        case 0: return k_step<uint16_t, 0, false, FORM, ORDER, K>;
+        case 1: return C ? k_step<uint16_t, 1, true, FORM, ORDER, K> : k_step<uint16_t, 1, false, FORM, ORDER, K>;
+        case 2: return C ? k_step<uint16_t, 2, true, FORM, ORDER, K> : k_step<uint16_t, 2, false, FORM, ORDER, K>;
+        case 3: return C ? k_step<uint16_t, 3, true, FORM, ORDER, K> : k_step<uint16_t, 3, false, FORM, ORDER, K>;
+        default: return C ? k_step<uint16_t, 4, true, FORM, ORDER, K> : k_step<uint16_t, 4, false, FORM, ORDER, K>;
+        }
+    }
+    switch (P) {
+    case 0: return k_step<uint32_t, 0, false, FORM, ORDER, K>;
+    case 1: return C ? k_step<uint32_t, 1, true, FORM, ORDER, K> : k_step<uint32_t, 1, false, FORM, ORDER, K>;
+    case 2: return C ? k_step<uint32_t, 2, true, FORM, ORDER, K> : k_step<uint32_t, 2, false, FORM, ORDER, K>;
+    case 3: return C ? k_step<uint32_t, 3, true, FORM, ORDER, K> : k_step<uint32_t, 3, false, FORM, ORDER, K>;
+    default: return C ? k_step<uint32_t, 4, true, FORM, ORDER, K> : k_step<uint32_t, 4, false, FORM, ORDER, K>;
+    }
+}
+
+}  // namespace gpx
+
+#endif  // GPX_STEP_H

@@ -1,0 +1,10 @@
+// Fused time-step kernels for models with HORIPML, two CFS terms (one translation unit
+// per PML form, compiled in parallel).
+#include "gpx_step.h"
+
+namespace gpx {
+step_kernel step_kernel_hori2(int id_bytes, int P, bool C)
+{
+    return step_kernel_table<0, 2>(id_bytes, P, C);
+}
+}  // namespace gpx

@@ -47,6 +47,8 @@ SIGNATURES = {
     'gpx_interior_cells': ([_P], ctypes.c_longlong),
     'gpx_part': ([_P, _I, _I, _I], _I),
     'gpx_set_launch': ([_P, _I, _I], _I),
+    'gpx_set_step': ([_P, _I, _I], _I),
+    'gpx_step_kind': ([_P], _I),
     'gpx_synchronize': ([_P], _I),
     'gpx_download_rx': ([_P, _P], _I),
     'gpx_download_fields': ([_P, _P], _I),
@@ -161,6 +163,13 @@ class HipSolver:
     # -- stepping ----------------------------------------------------------
     def set_launch(self, xchunk=0, block=0):
         self._check(self.lib.gpx_set_launch(self.ctx, xchunk, block))
+
+    def set_step(self, fused=1, tile_cells=0):
+        self._check(self.lib.gpx_set_step(self.ctx, int(fused), int(tile_cells)))
+
+    def step_kind(self):
+        """Cells per fused block, or 0 when the H and E passes run."""
+        return int(self.lib.gpx_step_kind(self.ctx))
 
     def run(self, it0=0, n=None):
         n = self.G.iterations - it0 if n is None else n

@@ -108,9 +108,10 @@ int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
                   float *ms_h, float *ms_e);
 
 /* As gpx_run, with HIP events around every kernel; ms4 receives the mean
- * duration in ms of the four kernels of an iteration: H interior, H boundary,
- * E interior, E boundary (interior = cells away from the faces and PML
- * slabs, boundary = the rest).  Waits after each iteration. */
+ * duration in ms of each kernel of an iteration: the fused step kernel in
+ * ms4[0] (ms4[1..3] = 0), or with the two-pass form H interior, H boundary,
+ * E interior, E boundary (interior = cells away from the faces and PML slabs,
+ * boundary = the rest).  Waits after each iteration. */
 int gpx_run_profiled(gpx_ctx *ctx, int it0, int n, float *ms4);
 
 /* Cells of the interior box (the rest are boundary cells). */
@@ -119,9 +120,21 @@ long long gpx_interior_cells(const gpx_ctx *ctx);
 /* One unfused sub-step (enum gpx_part_id) at iteration it. */
 int gpx_part(gpx_ctx *ctx, int part, int arg, int it);
 
-/* Launch shape of the H and E passes: planes per chunk along x (0 = auto)
- * and threads per block (64, 128, 256, 512 or 1024; 0 = auto). */
+/* Launch shape: planes per chunk along x (0 = auto, both step forms) and
+ * threads per block of the H and E passes (64, 128, 256, 512 or 1024; 0 = auto). */
 int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
+
+/* Step form of gpx_run.  fused = 1: one kernel per iteration that does the
+ * H and the E update with the state double-buffered on the device; fused = 0
+ * (default): an H pass and an E pass per iteration.  tile_cells = cells per
+ * fused block (512, 1024 or 2048; 0 = auto).  Both forms give identical
+ * results; the fused form needs a tile of at least twice nz+1 cells within
+ * the LDS budget, and the passes run when it does not fit. */
+int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells);
+
+/* Cells per block of the fused step that gpx_run will use, or 0 when it will
+ * run the H and E passes. */
+int gpx_step_kind(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);
 

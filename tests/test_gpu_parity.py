@@ -141,13 +141,16 @@ def test_each_substep_matches_oracle(case):
                 assert np.array_equal(out[c], snap.fields[c]), ('snapshot', n, c)
 
 
+@pytest.mark.parametrize('fused', [1, 0], ids=['fused-step', 'two-pass'])
 @pytest.mark.parametrize('case', list(CASES))
-def test_fused_run_matches_oracle(case):
-    """All iterations with the fused passes: final state, receiver traces and
-    snapshots equal to the CPU solve."""
+def test_fused_run_matches_oracle(case, fused):
+    """All iterations, as one fused kernel per step or as an H and an E pass:
+    final state, receiver traces and snapshots equal to the CPU solve."""
     G = rich_model(seed=3, **CASES[case])
     Gc = rich_model(seed=3, **CASES[case])
     with _hip()(G) as s:
+        s.set_step(fused)
+        assert (s.step_kind() > 0) == bool(fused)
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -176,6 +179,7 @@ def test_launch_shapes_do_not_change_results(xchunk, block):
     G = rich_model(seed=5, poles=2, order=2)
     ref = None
     with _hip()(G) as s:
+        s.set_step(0)
         s.set_launch(xchunk, block)
         s.run(0, G.iterations)
         out = s.download_fields()
@@ -183,6 +187,48 @@ def test_launch_shapes_do_not_change_results(xchunk, block):
         s.run(0, G.iterations)
         ref = s.download_fields()
     assert np.array_equal(out, ref)
+
+
+# A cross-section of 46 x 38 = 1748 cells spans two to three tiles of the fused
+# step, so tile halos and x-chunk halo planes are crossed everywhere.
+WIDE = (23, 45, 37)
+
+
+@pytest.mark.parametrize('tile,xchunk', [(1024, 3), (2048, 2), (1024, 0), (512, 5)])
+@pytest.mark.parametrize('case', ['horipml1', 'mripml2', 'debye3', 'lorentz2'])
+def test_fused_step_from_random_state(case, tile, xchunk):
+    """Fused iterations from a random state (fields, Phi, T) with several tiles
+    per plane and short x chunks: state and receiver samples equal to the
+    oracle stepped the same number of iterations."""
+    G = rich_model(seed=21, shape=WIDE, iterations=12, snapshots=False, **CASES[case])
+    _random_state(G, 5)
+    lib = oracle.load(np.float32)
+    with _hip()(G) as s:
+        s.set_step(1, tile)
+        s.set_launch(xchunk, 0)
+        assert s.step_kind() == tile
+        _push_state(s, G)
+        s.run(2, 6)
+        for it in range(2, 8):
+            osolver.step(lib, G, it, 8)
+        _assert_state_equal(s, G, 'fused tile {} xchunk {}'.format(tile, xchunk))
+        buf = s.download_rx()
+    for r, rx in enumerate(G.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(buf[c, 2:8, r], rx.outputs[name][2:8]), (r, name)
+
+
+def test_fused_and_two_pass_agree_on_a_wide_grid():
+    """The two step forms give the same bits on a grid with several tiles."""
+    out = []
+    for fused in (1, 0):
+        G = rich_model(seed=8, shape=WIDE, iterations=30, poles=1, order=2)
+        with _hip()(G) as s:
+            s.set_step(fused)
+            s.run(0, G.iterations)
+            out.append((s.download_fields(), s.download_rx()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 @pytest.mark.parametrize('name', ['hertzian_dipole_fs', 'hertzian_dipole_dispersive', 'magnetic_dipole_fs'])

@@ -281,3 +281,28 @@ def test_pml_reflection_is_small():
     assert err < 2e-2, err            # about -40 dB or better
     wrong = run(40, 0)                # no PML: the PEC box reflects fully
     assert _maxerr(wrong[2], big[2]) > 5 * err
+
+
+def test_float32_error_against_float64_on_the_100_cube():
+    """The 100^3 free-space Ricker workload (1.5 GHz, 10-cell PML, Ez receiver
+    20 cells along x, 500 iterations) run in float32 and in float64.  The
+    float32 error of the co-polar components is below 1e-4 of their peak
+    (7.4e-5 for Ez and Hy at the time of writing); the much weaker
+    cross-polar components carry more relative rounding."""
+    from gprmax_amd.grid import ModelBuilder
+
+    def run(precision):
+        n = 100
+        b = ModelBuilder(n, n, n, 1e-3, 1e-3, 1e-3, 500, pml_cells=10, precision=precision)
+        b.add_waveform('ricker', 1, 1.5e9, 'ricker15')
+        b.add_hertzian_dipole('z', (n // 2,) * 3, 'ricker15')
+        b.add_rx((n // 2 + 20, n // 2, n // 2))
+        G = b.finalise()
+        solve_cpu(1, 1, G, nthreads=8)
+        return _trace(G).astype(np.float64)
+
+    single, double = run('single'), run('double')
+    for c in (2, 4):  # Ez, Hy
+        peak = np.abs(double[c]).max()
+        assert np.abs(single[c] - double[c]).max() < 1e-4 * peak, c
+    assert np.abs(single[0] - double[0]).max() < 1e-3 * np.abs(double[0]).max()  # Ex
