@@ -69,6 +69,135 @@ def measured_traffic(workload, kernel):
     return int(rec['fetch_kib'] * 1024 * rec['fetch_scale'] + rec['write_kib'] * 1024 * rec['write_scale'])
 
 
+def bench_bscan(args):
+    """Uneven B-scan: `--traces` models (sides 128-256 cells, 1 mm, 10-cell
+    PML, random soil), each `--bscan-its` iterations.  Sequential plain solves
+    (per trace: create, upload, run, download, destroy) against the same
+    traces resident together and advanced by one batched launch per
+    iteration, captured in one HIP graph, with the launch shape picked by the
+    autotuner.  Host model building is excluded from both."""
+    import numpy as np
+    from gprmax_amd.hip import HipBatch, HipSolver
+    from gprmax_amd.workloads import bscan_trace
+
+    plain_wall = plain_dev = 0.0
+    plain_rx, solvers, cells = [], [], 0
+    for t in range(args.traces):
+        G = bscan_trace(t, iterations=args.bscan_its)
+        cells += G.nx * G.ny * G.nz
+        t0 = time.perf_counter()
+        with HipSolver(G) as s:
+            s.set_step(1 if args.fused else 0)
+            ms, _, _ = s.run_timed(0, G.iterations)
+            plain_rx.append(s.download_rx())
+        plain_wall += time.perf_counter() - t0
+        plain_dev += ms * 1e-3
+        solvers.append(HipSolver(G))          # the batch's copy, uploaded before the timed batch
+        for name in ('ID', 'solid', 'rigidcellE', 'rigidcellH'):
+            if hasattr(G, name):
+                delattr(G, name)              # host copies are no longer needed
+        print('trace {} {} plain {:.3f} s'.format(t, G.title, ms * 1e-3), file=sys.stderr, flush=True)
+    with HipBatch(solvers) as b:
+        tile, xchunk, ms_tune = b.tune(steps=3)
+        t0 = time.perf_counter()
+        ms = b.run(0, args.bscan_its, graph=True)
+        rx = [s.download_rx() for s in solvers]
+        batch_wall = time.perf_counter() - t0
+    same = all(np.array_equal(a, b_) for a, b_ in zip(rx, plain_rx))
+    for s in solvers:
+        s.close()
+    line = {
+        'metric': 'batched B-scan time vs sequential plain solves (1 GPU)',
+        'value': round(plain_wall / batch_wall, 3),
+        'unit': 'x (sequential wall / batched wall)',
+        'n_gpus': 1, 'steps': args.bscan_its, 'warmup': 0,
+        'higher_is_better': True, 'scaling': 'strong', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'uneven B-scan, {} traces, sides 128-256 cells'.format(args.traces),
+                   'cells': cells, 'iterations': args.bscan_its},
+        'sequential_wall_s': round(plain_wall, 3), 'sequential_device_s': round(plain_dev, 3),
+        'batched_wall_s': round(batch_wall, 3), 'batched_device_s': round(ms * 1e-3, 3),
+        'kernel_ratio': round(plain_dev / (ms * 1e-3), 3),
+        'batched_mcells_per_s': round(cells * args.bscan_its / (ms * 1e-3) / 1e6, 1),
+        'tuned_shape': {'tile_cells': tile, 'xchunk': xchunk, 'ms_per_iteration': round(ms_tune, 4)},
+        'plain_step_form': 'fused' if args.fused else 'two-pass',
+        'receivers_identical': same,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
+def bench_adjoint(args):
+    """Config of the differentiable solve: 160x160x120 cells at 2 mm, 10-cell
+    PML, smooth true model (er in [4, 10], sigma in [1e-3, 2e-2]),
+    homogeneous start model, 8 z-directed Hertzian sources x 16 Ez receivers,
+    L2 trace misfit, checkpoint interval K = 64 over 2000 iterations.
+    Reports the cost of the gradient (forward + recompute + adjoint) against
+    one forward, and, with --fd-cells, central differences of the float32
+    forward at that many random interior cells (first source only)."""
+    import numpy as np
+    from gprmax_amd.inversion import InversionGrid, misfit_and_gradient
+
+    shape, d, its = (160, 160, 120), 2e-3, args.adj_its
+    rng = np.random.default_rng(1005)
+
+    def smooth(lo, hi):
+        f = rng.standard_normal(shape)
+        for ax in range(3):               # separable box blur, 9 cells wide, 3 passes
+            for _ in range(3):
+                f = sum(np.roll(f, s, axis=ax) for s in range(-4, 5)) / 9.0
+        f = (f - f.min()) / (f.max() - f.min())
+        return lo + (hi - lo) * f
+
+    er_true, se_true = smooth(4.0, 10.0), smooth(1e-3, 2e-2)
+    er0, se0 = np.full(shape, 7.0), np.full(shape, 1.05e-2)
+    grid = InversionGrid(shape, d, its, pml=10, er_pml=7.0, K=64)
+    z = 120 - 10 - 5                      # 5 cells inside the top PML
+    sources = [(40 + 26 * (n % 4), 55 + 50 * (n // 4), z) for n in range(8)]
+    receivers = [(30 + 33 * (n % 4), 30 + 33 * (n // 4), z) for n in range(16)]
+    waves = grid.waveform(freq=900e6)
+    grid.set_model(er_true, se_true)
+    observed = [grid.forward(s, receivers, waves) for s in sources]
+    t0 = time.perf_counter()
+    loss, ger, gse = misfit_and_gradient(grid, er0, se0, sources, receivers, observed, waves)
+    wall = time.perf_counter() - t0
+    ms_f, ms_b = grid.timings()          # of the last source
+    line = {
+        'metric': 'adjoint-gradient time vs one forward solve (1 GPU)',
+        'value': round((ms_f + ms_b) / ms_f, 3),
+        'unit': 'x (forward + recompute + adjoint) / forward',
+        'n_gpus': 1, 'steps': its, 'warmup': 0, 'higher_is_better': False, 'scaling': 'strong',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'adjoint 160x160x120, 8 sources x 16 receivers, K=64', 'iterations': its},
+        'forward_ms': round(ms_f, 2), 'backward_ms': round(ms_b, 2),
+        'forward_mcells_per_s': round(float(np.prod(shape)) * its / (ms_f * 1e-3) / 1e6, 1),
+        'gradient_wall_s_all_sources': round(wall, 2), 'loss': loss,
+        'grad_norm_er': float(np.linalg.norm(ger)), 'grad_norm_se': float(np.linalg.norm(gse)),
+    }
+    if args.fd_cells:
+        idx = np.argwhere(grid.interior)
+        picks = idx[rng.choice(len(idx), args.fd_cells, replace=False)]
+        src, obs = sources[:1], observed[:1]
+        _, g1, _ = misfit_and_gradient(grid, er0, se0, src, receivers, obs, waves)
+        errs = []
+        for c in picks:
+            c = tuple(c)
+            h = 1e-2 * er0[c]
+            Ls = []
+            for sgn in (1, -1):
+                e = er0.copy()
+                e[c] += sgn * h
+                Ls.append(misfit_and_gradient(grid, e, se0, src, receivers, obs, waves)[0])
+            fd = (Ls[0] - Ls[1]) / (2 * h)
+            errs.append(abs(fd - g1[c]) / max(abs(fd), 1e-30))
+        line['fd_cells'] = args.fd_cells
+        line['fd_rel_err_median'] = float(np.median(errs))
+        line['fd_rel_err_max'] = float(np.max(errs))
+    grid.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -80,6 +209,10 @@ def main():
     ap.add_argument('--block', type=int, default=0)
     ap.add_argument('--tile', type=int, default=0, help='cells per fused block (0 = auto)')
     ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
+    ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
+    ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
+    ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
+    ap.add_argument('--fd-cells', type=int, default=0, help='finite-difference check cells (adjoint workload)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this solver runs on one GPU; --gpus {} is not supported '
@@ -87,6 +220,10 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.workload == 'bscan':
+        return bench_bscan(args)
+    if args.workload == 'adjoint':
+        return bench_adjoint(args)
     from gprmax_amd.hip import HipSolver, device_info
     from gprmax_amd.workloads import WORKLOADS, algorithmic_bytes_per_step, kernel_bytes
 

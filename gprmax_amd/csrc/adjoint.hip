@@ -1,0 +1,821 @@
+// Differentiable FDTD for per-cell permittivity and conductivity: a forward
+// solve that keeps checkpoints, and the discrete adjoint (reverse mode) of it,
+// which accumulates dL/dCA, dL/dCB and dL/dIEA per cell for a trace-misfit
+// loss.  C ABI in include/gpx.h (gpx_inv_*).
+//
+// The forward scheme is the reference's solver loop (gprMax/model_build_run.py:
+// 424-470) restricted to what a per-cell inversion needs: Yee updates
+// (gprMax/fields_updates_ext.pyx:81-107, 404-412) with every cell its own
+// material (E edge (c, i, j, k) takes cell (i, j, k), the reference's
+// build_voxel without averaging), mu_r = 1 and no magnetic loss (DA = 1,
+// DB = dt / (mu0 d)), HORIPML with one CFS term on any set of slabs
+// (gprMax/pml_updates/pml_updates_*_HORIPML_ext.pyx, order1_*), one
+// z-directed Hertzian dipole per solve (gprMax/sources.py:163-193), and Ez
+// receivers sampled at the start of each iteration
+// (gprMax/fields_outputs.py:40-67).
+//
+// Iteration n maps the state s^n = (E^n, H^n, PhiE^n, PhiH^n) to s^{n+1}
+// linearly.  Its adjoint, applied from n = N-1 down to 0 to the adjoint state
+// (lE, lH, mE, mH) = dL/ds^{n+1}, is four gather kernels:
+//   k_adj_e_gather : lH += (dE^{n+1}/dH^{n+1})^T lE      (curl and E-PML terms)
+//   k_adj_e_update : gradients from lE and the stored E^n, H^{n+1};
+//                    mE = (dE/dPhiE)^T lE + RE mE;  lE = CA lE
+//   k_adj_h_gather : lE += (dH^{n+1}/dE^n)^T lH         (curl and H-PML terms)
+//   k_adj_h_update : mH = (dH/dPhiH)^T lH + RE mH       (lH = DA lH = lH)
+// then the receiver residual of iteration n is added to lE.  Forward states
+// come from checkpoints every K iterations; each segment is recomputed
+// forward, storing E^n and H^{n+1} for every iteration of it, before the
+// adjoint sweeps back over it.
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "gpx.h"
+
+namespace {
+
+constexpr int INV_MAX_SLABS = 6;
+
+// One PML slab of the inversion grid (order 1).  Boxes are the corrected
+// cells of each field; the profile index along the normal is p = pend - c
+// (minus slabs) or c - pend (plus slabs).
+struct InvSlab {
+    int axis, minus;
+    int loE[3], hiE[3], loH[3], hiH[3];
+    int pendE, pendH;
+    int org[3];
+    int dE1[3], dE2[3], dH1[3], dH2[3];  // Phi dims (without the order axis)
+    const float *RE;                     // E: RA, RB, RE, RF, each [t]
+    const float *RH;                     // H: the same
+    int t;
+    float *PE1, *PE2, *PH1, *PH2;        // Phi (forward) or their adjoints
+};
+
+struct InvArgs {
+    int nx, ny, nz;
+    long long ntot;
+    float d, DB, cH;                     // 1/d? no: d; DB = dt/(mu0 d); cH = dt/mu0
+    float *E[3], *H[3];                  // fields (forward) or adjoint fields
+    const float *Es[3], *Hs[3];          // stored forward E^n and H^{n+1} (adjoint)
+    const float *CA, *CB, *IEA;          // per cell [nx][ny][nz]
+    float *gCA, *gCB, *gIEA;             // gradient accumulators (adjoint)
+    int nslab;
+    InvSlab slab[INV_MAX_SLABS];
+    int si, sj, sk;                      // source edge (Ez), -1 when none
+    float sfac;                          // dl / (dx dy dz)
+    const float *waves;                  // [iterations]
+    int it, iterations;
+    int nrx;
+    const int *rxc;                      // [nrx][3]
+    float *traces;                       // [nrx][iterations]
+    const float *resid;                  // [nrx][iterations], dL/d(trace)
+};
+
+__device__ __forceinline__ long long fidx(const InvArgs &a, int i, int j, int k)
+{
+    return ((long long)i * (a.ny + 1) + j) * (a.nz + 1) + k;
+}
+__device__ __forceinline__ long long cidx(const InvArgs &a, int i, int j, int k)
+{
+    return ((long long)i * a.ny + j) * a.nz + k;
+}
+
+// Reference update ranges (gprMax/fields_updates_ext.pyx): component c of E or H
+// is updated at (i, j, k).
+__device__ __forceinline__ bool e_upd(const InvArgs &a, int c, int i, int j, int k)
+{
+    if (i < 0 || j < 0 || k < 0) return false;
+    if (c == 0) return i < a.nx && j >= 1 && j < a.ny && k >= 1 && k < a.nz;
+    if (c == 1) return i >= 1 && i < a.nx && j < a.ny && k >= 1 && k < a.nz;
+    return i >= 1 && i < a.nx && j >= 1 && j < a.ny && k < a.nz;
+}
+__device__ __forceinline__ bool h_upd(const InvArgs &a, int c, int i, int j, int k)
+{
+    if (i < 0 || j < 0 || k < 0) return false;
+    if (c == 0) return i >= 1 && i <= a.nx && j < a.ny && k < a.nz;
+    if (c == 1) return i < a.nx && j >= 1 && j <= a.ny && k < a.nz;
+    return i < a.nx && j < a.ny && k >= 1 && k <= a.nz;
+}
+
+__device__ __forceinline__ bool in_e(const InvSlab &s, int i, int j, int k)
+{
+    return i >= s.loE[0] && i < s.hiE[0] && j >= s.loE[1] && j < s.hiE[1] && k >= s.loE[2] && k < s.hiE[2];
+}
+__device__ __forceinline__ bool in_h(const InvSlab &s, int i, int j, int k)
+{
+    return i >= s.loH[0] && i < s.hiH[0] && j >= s.loH[1] && j < s.hiH[1] && k >= s.loH[2] && k < s.hiH[2];
+}
+
+// Profile index and Phi index of cell (i, j, k) for the given field's box.
+__device__ __forceinline__ void pidx(const InvSlab &s, bool efield, int i, int j, int k, int &p, long long &q1,
+                                     long long &q2)
+{
+    int c[3] = {i - s.org[0], j - s.org[1], k - s.org[2]};
+    const int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
+    const int pend = efield ? s.pendE : s.pendH;
+    p = s.minus ? pend - along : along - pend;
+    c[s.axis] = p;
+    const int *d1 = efield ? s.dE1 : s.dH1, *d2 = efield ? s.dE2 : s.dH2;
+    q1 = ((long long)c[0] * d1[1] + c[1]) * d1[2] + c[2];
+    q2 = ((long long)c[0] * d2[1] + c[1]) * d2[2] + c[2];
+}
+
+// E-PML pairs per axis: (corrected E component, source H component, sign) for
+// Phi1 and Phi2; H-PML likewise (gprMax/pml_updates/*_HORIPML_ext.pyx).
+__device__ __forceinline__ void e_pair(int axis, int w, int &comp, int &src, int &sign)
+{
+    const int tab[3][2][3] = {{{1, 2, -1}, {2, 1, +1}}, {{0, 2, +1}, {2, 0, -1}}, {{0, 1, -1}, {1, 0, +1}}};
+    comp = tab[axis][w][0];
+    src = tab[axis][w][1];
+    sign = tab[axis][w][2];
+}
+__device__ __forceinline__ void h_pair(int axis, int w, int &comp, int &src, int &sign)
+{
+    const int tab[3][2][3] = {{{1, 2, +1}, {2, 1, -1}}, {{0, 2, -1}, {2, 0, +1}}, {{0, 1, +1}, {1, 0, -1}}};
+    comp = tab[axis][w][0];
+    src = tab[axis][w][1];
+    sign = tab[axis][w][2];
+}
+
+__device__ __forceinline__ int axis_of(const InvSlab &s, int i, int j, int k)
+{
+    return s.axis == 0 ? i : (s.axis == 1 ? j : k);
+}
+
+// ------------------------------------------------------------------ forward
+// H update and H-PML of one iteration; one thread per grid point.
+__global__ void k_inv_h(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        const long long o = g, oi = fidx(a, i + 1, j, k), oj = fidx(a, i, j + 1, k), ok = o + 1;
+        float h[3] = {a.H[0][o], a.H[1][o], a.H[2][o]};
+        if (h_upd(a, 0, i, j, k))
+            h[0] = 1.0f * h[0] - a.DB * (a.E[2][oj] - a.E[2][o]) + a.DB * (a.E[1][ok] - a.E[1][o]);
+        if (h_upd(a, 1, i, j, k))
+            h[1] = 1.0f * h[1] - a.DB * (a.E[0][ok] - a.E[0][o]) + a.DB * (a.E[2][oi] - a.E[2][o]);
+        if (h_upd(a, 2, i, j, k))
+            h[2] = 1.0f * h[2] - a.DB * (a.E[1][oi] - a.E[1][o]) + a.DB * (a.E[0][oj] - a.E[0][o]);
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            if (!in_h(S, i, j, k)) continue;
+            int p;
+            long long q[2];
+            pidx(S, false, i, j, k, p, q[0], q[1]);
+            const float RA01 = S.RH[p] - 1.0f, RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p],
+                        RF0 = S.RH[3 * S.t + p];
+            const long long on = S.axis == 0 ? oi : (S.axis == 1 ? oj : ok);
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                h_pair(S.axis, w, comp, src, sign);
+                float *Phi = w == 0 ? S.PH1 : S.PH2;
+                const float dd = (a.E[src][on] - a.E[src][o]) / a.d;
+                const float ph = Phi[q[w]];
+                const float corr = a.cH * (RA01 * dd + RB0 * ph);
+                Phi[q[w]] = RE0 * ph - RF0 * dd;
+                h[comp] = sign > 0 ? h[comp] + corr : h[comp] - corr;
+            }
+        }
+        a.H[0][o] = h[0];
+        a.H[1][o] = h[1];
+        a.H[2][o] = h[2];
+    }
+}
+
+// Receiver samples of iteration it (the state before its H update).
+__global__ void k_inv_rx(InvArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrx) return;
+    const int *c = a.rxc + 3 * r;
+    a.traces[(long long)r * a.iterations + a.it] = a.E[2][fidx(a, c[0], c[1], c[2])];
+}
+
+// E update, E-PML and the Hertzian dipole of one iteration.
+__global__ void k_inv_e(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        const long long o = g;
+        const long long om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
+        float e[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+        const bool cell = i < a.nx && j < a.ny && k < a.nz;
+        const long long oc = cell ? cidx(a, i, j, k) : 0;
+        const float CA = cell ? a.CA[oc] : 0.f, CB = cell ? a.CB[oc] : 0.f, IEA = cell ? a.IEA[oc] : 0.f;
+        if (e_upd(a, 0, i, j, k))
+            e[0] = CA * e[0] + CB * (a.H[2][o] - a.H[2][om[1]]) - CB * (a.H[1][o] - a.H[1][om[2]]);
+        if (e_upd(a, 1, i, j, k))
+            e[1] = CA * e[1] + CB * (a.H[0][o] - a.H[0][om[2]]) - CB * (a.H[2][o] - a.H[2][om[0]]);
+        if (e_upd(a, 2, i, j, k))
+            e[2] = CA * e[2] + CB * (a.H[1][o] - a.H[1][om[0]]) - CB * (a.H[0][o] - a.H[0][om[1]]);
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            if (!in_e(S, i, j, k)) continue;
+            int p;
+            long long q[2];
+            pidx(S, true, i, j, k, p, q[0], q[1]);
+            const float RA01 = S.RE[p] - 1.0f, RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p],
+                        RF0 = S.RE[3 * S.t + p];
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                e_pair(S.axis, w, comp, src, sign);
+                float *Phi = w == 0 ? S.PE1 : S.PE2;
+                const float dd = (a.H[src][o] - a.H[src][om[S.axis]]) / a.d;
+                const float ph = Phi[q[w]];
+                const float corr = IEA * (RA01 * dd + RB0 * ph);
+                Phi[q[w]] = RE0 * ph - RF0 * dd;
+                e[comp] = sign > 0 ? e[comp] + corr : e[comp] - corr;
+            }
+        }
+        if (i == a.si && j == a.sj && k == a.sk)
+            e[2] = e[2] - IEA * a.waves[a.it] * a.sfac;
+        a.E[0][o] = e[0];
+        a.E[1][o] = e[1];
+        a.E[2][o] = e[2];
+    }
+}
+
+// ------------------------------------------------------------------ adjoint
+// lE, lH, mE, mH live in a.E, a.H and the slab Phi pointers.
+
+// dL/d(dd) of one E-PML term at E edge (comp of the pair w) of slab S; zero
+// outside the slab's E box.
+__device__ __forceinline__ float e_pml_ldd(const InvArgs &a, const InvSlab &S, int w, int i, int j, int k)
+{
+    if (i < 0 || j < 0 || k < 0 || i > a.nx || j > a.ny || k > a.nz || !in_e(S, i, j, k)) return 0.f;
+    int comp, src, sign;
+    e_pair(S.axis, w, comp, src, sign);
+    int p;
+    long long q[2];
+    pidx(S, true, i, j, k, p, q[0], q[1]);
+    const float RA01 = S.RE[p] - 1.0f, RF0 = S.RE[3 * S.t + p];
+    const float IEA = (i < a.nx && j < a.ny && k < a.nz) ? a.IEA[cidx(a, i, j, k)] : 0.f;
+    const float lF = a.E[comp][fidx(a, i, j, k)];
+    const float mu = (w == 0 ? S.PE1 : S.PE2)[q[w]];
+    return (float)sign * IEA * RA01 * lF - RF0 * mu;
+}
+
+__device__ __forceinline__ float h_pml_ldd(const InvArgs &a, const InvSlab &S, int w, int i, int j, int k)
+{
+    if (i < 0 || j < 0 || k < 0 || i > a.nx || j > a.ny || k > a.nz || !in_h(S, i, j, k)) return 0.f;
+    int comp, src, sign;
+    h_pair(S.axis, w, comp, src, sign);
+    int p;
+    long long q[2];
+    pidx(S, false, i, j, k, p, q[0], q[1]);
+    const float RA01 = S.RH[p] - 1.0f, RF0 = S.RH[3 * S.t + p];
+    const float lF = a.H[comp][fidx(a, i, j, k)];
+    const float mu = (w == 0 ? S.PH1 : S.PH2)[q[w]];
+    return (float)sign * a.cH * RA01 * lF - RF0 * mu;
+}
+
+// CB of the cell of E edge (i, j, k) times lE_c there, when that edge is updated.
+__device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k)
+{
+    if (!e_upd(a, c, i, j, k)) return 0.f;
+    return a.CB[cidx(a, i, j, k)] * a.E[c][fidx(a, i, j, k)];
+}
+
+// lH += (dE^{n+1}/dH^{n+1})^T lE.
+__global__ void k_adj_e_gather(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        // Ex(i,j,k) = ... + CB (Hz(i,j,k) - Hz(i,j-1,k)) - CB (Hy(i,j,k) - Hy(i,j,k-1))
+        // Ey(i,j,k) = ... + CB (Hx(i,j,k) - Hx(i,j,k-1)) - CB (Hz(i,j,k) - Hz(i-1,j,k))
+        // Ez(i,j,k) = ... + CB (Hy(i,j,k) - Hy(i-1,j,k)) - CB (Hx(i,j,k) - Hx(i,j-1,k))
+        float dx = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
+        float dy = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
+        float dz = xi(a, 0, i, j, k) - xi(a, 0, i, j + 1, k) - xi(a, 1, i, j, k) + xi(a, 1, i + 1, j, k);
+        float dh[3] = {dx, dy, dz};
+        // E-PML: dd = (S(e) - S(e - axis)) / d uses H at the edge and at edge - axis,
+        // so H here feeds the edge here (+) and the edge one step up the axis (-).
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            const int ip = i + (S.axis == 0), jp = j + (S.axis == 1), kp = k + (S.axis == 2);
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                e_pair(S.axis, w, comp, src, sign);
+                dh[src] += (e_pml_ldd(a, S, w, i, j, k) - e_pml_ldd(a, S, w, ip, jp, kp)) / a.d;
+            }
+        }
+        a.H[0][g] += dh[0];
+        a.H[1][g] += dh[1];
+        a.H[2][g] += dh[2];
+    }
+}
+
+// Gradients, mE and lE <- CA lE, from lE = dL/dE^{n+1}.
+__global__ void k_adj_e_update(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        const bool cell = i < a.nx && j < a.ny && k < a.nz;
+        if (!cell) continue;  // edges outside the cells are never updated
+        const long long o = g, oc = cidx(a, i, j, k);
+        const long long om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
+        const float CA = a.CA[oc], IEA = a.IEA[oc];
+        float gca = 0.f, gcb = 0.f, giea = 0.f;
+        float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+        // E-PML adjoint Phi (before lE changes) and the gradient terms.
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            if (!in_e(S, i, j, k)) continue;
+            int p;
+            long long q[2];
+            pidx(S, true, i, j, k, p, q[0], q[1]);
+            const float RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p];
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                e_pair(S.axis, w, comp, src, sign);
+                float *mu = w == 0 ? S.PE1 : S.PE2;
+                mu[q[w]] = RE0 * mu[q[w]] + (float)sign * IEA * RB0 * lE[comp];
+            }
+        }
+        const float curl[3] = {
+            (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
+            (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
+            (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
+        for (int c = 0; c < 3; ++c) {
+            if (!e_upd(a, c, i, j, k)) continue;
+            gca += lE[c] * a.Es[c][o];
+            gcb += lE[c] * curl[c];
+            lE[c] = CA * lE[c];
+        }
+        if (i == a.si && j == a.sj && k == a.sk) giea += -a.E[2][o] * a.waves[a.it] * a.sfac;
+        a.gCA[oc] += gca;
+        a.gCB[oc] += gcb;
+        a.gIEA[oc] += giea;
+        a.E[0][o] = lE[0];
+        a.E[1][o] = lE[1];
+        a.E[2][o] = lE[2];
+    }
+}
+
+// DB times lH_c at H cell (i, j, k), when that cell is updated.
+__device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int k)
+{
+    if (!h_upd(a, c, i, j, k)) return 0.f;
+    return a.DB * a.H[c][fidx(a, i, j, k)];
+}
+
+// lE += (dH^{n+1}/dE^n)^T lH.
+__global__ void k_adj_h_gather(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        // Hx(i,j,k) = Hx - DB (Ez(i,j+1,k) - Ez(i,j,k)) + DB (Ey(i,j,k+1) - Ey(i,j,k))
+        // Hy(i,j,k) = Hy - DB (Ex(i,j,k+1) - Ex(i,j,k)) + DB (Ez(i+1,j,k) - Ez(i,j,k))
+        // Hz(i,j,k) = Hz - DB (Ey(i+1,j,k) - Ey(i,j,k)) + DB (Ex(i,j+1,k) - Ex(i,j,k))
+        float dx = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
+        float dy = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
+        float dz = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
+        float de[3] = {dx, dy, dz};
+        // H-PML: dd = (S(h + axis) - S(h)) / d, so E here feeds the cell here (-)
+        // and the cell one step down the axis (+).
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            const int im = i - (S.axis == 0), jm = j - (S.axis == 1), km = k - (S.axis == 2);
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                h_pair(S.axis, w, comp, src, sign);
+                de[src] += (h_pml_ldd(a, S, w, im, jm, km) - h_pml_ldd(a, S, w, i, j, k)) / a.d;
+            }
+        }
+        a.E[0][g] += de[0];
+        a.E[1][g] += de[1];
+        a.E[2][g] += de[2];
+    }
+}
+
+// mH from lH = dL/dH^{n+1}; lH itself is unchanged (DA = 1).
+__global__ void k_adj_h_update(InvArgs a)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(g % (a.nz + 1));
+        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
+        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            if (!in_h(S, i, j, k)) continue;
+            int p;
+            long long q[2];
+            pidx(S, false, i, j, k, p, q[0], q[1]);
+            const float RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p];
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                h_pair(S.axis, w, comp, src, sign);
+                float *mu = w == 0 ? S.PH1 : S.PH2;
+                mu[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * a.H[comp][g];
+            }
+        }
+    }
+}
+
+// lEz at the receivers += dL/d(trace sample of iteration it).
+__global__ void k_adj_rx(InvArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrx) return;
+    const int *c = a.rxc + 3 * r;
+    a.E[2][fidx(a, c[0], c[1], c[2])] += a.resid[(long long)r * a.iterations + a.it];
+}
+
+}  // namespace
+
+// ====================================================================== host
+struct gpx_inv {
+    gpx_inv_desc desc;
+    long long ntot = 0, ncell = 0;
+    hipStream_t stream = nullptr;
+    float *F = nullptr;            // E, H: 6 x ntot
+    float *L = nullptr;            // adjoint lE, lH: 6 x ntot
+    float *ck = nullptr;           // checkpoints: nck x (6 ntot + phi)
+    float *seg = nullptr;          // segment store: K x 6 x ntot
+    float *par = nullptr;          // CA, CB, IEA: 3 x ncell
+    float *grad = nullptr;         // gCA, gCB, gIEA: 3 x ncell
+    float *waves = nullptr;
+    int *rxc = nullptr;
+    float *traces = nullptr, *resid = nullptr;
+    int nrx = 0, si = -1, sj = -1, sk = -1;
+    float sfac = 0;
+    bool have_model = false, have_forward = false;
+    struct Slab {
+        InvSlab s;
+        long long n[4] = {0, 0, 0, 0};   // Phi sizes
+        float *phi[4] = {nullptr, nullptr, nullptr, nullptr};
+        float *mu[4] = {nullptr, nullptr, nullptr, nullptr};
+        float *R = nullptr;              // E then H tables, 8 t floats
+    };
+    std::vector<Slab> slabs;
+    long long phi_total = 0;
+    float ms_fwd = 0, ms_bwd = 0;
+    std::string err;
+};
+
+static std::string g_inv_err;
+
+static int ifail(gpx_inv *v, int code, const std::string &m)
+{
+    if (v) v->err = m;
+    else g_inv_err = m;
+    return code;
+}
+
+#define INV_TRY(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return ifail(v, (int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *gpx_inv_last_error(const gpx_inv *v)
+{
+    return v ? v->err.c_str() : g_inv_err.c_str();
+}
+
+extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
+{
+    if (!d || !out) return ifail(nullptr, GPX_EARG, "gpx_inv_create: null argument");
+    *out = nullptr;
+    if (d->nx < 2 || d->ny < 2 || d->nz < 2 || d->iterations < 1 || d->K < 1 || !(d->d > 0) || !(d->dt > 0))
+        return ifail(nullptr, GPX_EARG, "gpx_inv_create: bad descriptor");
+    if (hipSetDevice(d->device) != hipSuccess) return ifail(nullptr, GPX_EARG, "gpx_inv_create: bad device");
+    gpx_inv *v = new gpx_inv();
+    v->desc = *d;
+    v->ntot = (long long)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+    v->ncell = (long long)d->nx * d->ny * d->nz;
+    hipError_t e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->F, 6 * v->ntot * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->L, 6 * v->ntot * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->seg, (size_t)d->K * 6 * v->ntot * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->par, 3 * v->ncell * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->grad, 3 * v->ncell * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->waves, (size_t)d->iterations * 4);
+    if (e != hipSuccess) {
+        gpx_inv_destroy(v);
+        return ifail(nullptr, (int)e, std::string("gpx_inv_create: ") + hipGetErrorString(e));
+    }
+    *out = v;
+    return 0;
+}
+
+extern "C" int gpx_inv_destroy(gpx_inv *v)
+{
+    if (!v) return 0;
+    hipSetDevice(v->desc.device);
+    if (v->stream) hipStreamSynchronize(v->stream);
+    for (float *p : {v->F, v->L, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid})
+        if (p) hipFree(p);
+    if (v->rxc) hipFree(v->rxc);
+    for (auto &S : v->slabs) {
+        for (float *p : S.phi) if (p) hipFree(p);
+        for (float *p : S.mu) if (p) hipFree(p);
+        if (S.R) hipFree(S.R);
+    }
+    if (v->stream) hipStreamDestroy(v->stream);
+    delete v;
+    return 0;
+}
+
+extern "C" int gpx_inv_add_pml(gpx_inv *v, int dir, int xs, int xf, int ys, int yf, int zs, int zf,
+                               const float *eR, const float *hR)
+{
+    if (!v || !eR || !hR || dir < 0 || dir > 5) return ifail(v, GPX_EARG, "gpx_inv_add_pml: bad argument");
+    if ((int)v->slabs.size() >= INV_MAX_SLABS) return ifail(v, GPX_ELIMIT, "gpx_inv_add_pml: at most 6 slabs");
+    const gpx_inv_desc &g = v->desc;
+    if (xs < 0 || ys < 0 || zs < 0 || xf > g.nx || yf > g.ny || zf > g.nz || xs >= xf || ys >= yf || zs >= zf)
+        return ifail(v, GPX_EARG, "gpx_inv_add_pml: slab box outside the grid");
+    hipSetDevice(g.device);
+    gpx_inv::Slab S;
+    InvSlab &s = S.s;
+    memset(&s, 0, sizeof(s));
+    const int axis = dir % 3, minus = dir < 3;
+    const int st[3] = {xs, ys, zs}, en[3] = {xf, yf, zf};
+    const int t = en[axis] - st[axis];
+    s.axis = axis;
+    s.minus = minus;
+    s.t = t;
+    for (int n = 0; n < 3; ++n) {
+        s.loE[n] = s.loH[n] = st[n];
+        s.hiE[n] = s.hiH[n] = en[n];
+        s.org[n] = st[n];
+    }
+    if (minus) {  // E cells f - p, H cells f - (p + 1)
+        s.loE[axis] = en[axis] - t + 1;
+        s.hiE[axis] = en[axis] + 1;
+        s.pendE = en[axis];
+        s.loH[axis] = en[axis] - t;
+        s.hiH[axis] = en[axis];
+        s.pendH = en[axis] - 1;
+    } else {
+        s.loE[axis] = s.loH[axis] = st[axis];
+        s.hiE[axis] = s.hiH[axis] = st[axis] + t;
+        s.pendE = s.pendH = st[axis];
+    }
+    // Phi shapes of gprMax/pml.py:202-219 (order 1)
+    const int nx = xf - xs, ny = yf - ys, nz = zf - zs;
+    int dims[4][3];
+    if (axis == 0) {
+        const int d4[4][3] = {{nx + 1, ny, nz + 1}, {nx + 1, ny + 1, nz}, {nx, ny + 1, nz}, {nx, ny, nz + 1}};
+        memcpy(dims, d4, sizeof(dims));
+    } else if (axis == 1) {
+        const int d4[4][3] = {{nx, ny + 1, nz + 1}, {nx + 1, ny + 1, nz}, {nx + 1, ny, nz}, {nx, ny, nz + 1}};
+        memcpy(dims, d4, sizeof(dims));
+    } else {
+        const int d4[4][3] = {{nx, ny + 1, nz + 1}, {nx + 1, ny, nz + 1}, {nx + 1, ny, nz}, {nx, ny + 1, nz}};
+        memcpy(dims, d4, sizeof(dims));
+    }
+    for (int n = 0; n < 3; ++n) {
+        s.dE1[n] = dims[0][n];
+        s.dE2[n] = dims[1][n];
+        s.dH1[n] = dims[2][n];
+        s.dH2[n] = dims[3][n];
+    }
+    for (int q = 0; q < 4; ++q) {
+        S.n[q] = (long long)dims[q][0] * dims[q][1] * dims[q][2];
+        INV_TRY(hipMalloc((void **)&S.phi[q], S.n[q] * 4));
+        INV_TRY(hipMalloc((void **)&S.mu[q], S.n[q] * 4));
+        v->phi_total += S.n[q];
+    }
+    INV_TRY(hipMalloc((void **)&S.R, (size_t)8 * t * 4));
+    INV_TRY(hipMemcpy(S.R, eR, (size_t)4 * t * 4, hipMemcpyHostToDevice));
+    INV_TRY(hipMemcpy(S.R + 4 * t, hR, (size_t)4 * t * 4, hipMemcpyHostToDevice));
+    s.RE = S.R;
+    s.RH = S.R + 4 * t;
+    v->slabs.push_back(S);
+    if (v->ck) {  // checkpoint layout depends on the Phi sizes
+        hipFree(v->ck);
+        v->ck = nullptr;
+    }
+    v->have_forward = false;
+    return 0;
+}
+
+extern "C" int gpx_inv_set_model(gpx_inv *v, const float *CA, const float *CB, const float *IEA)
+{
+    if (!v || !CA || !CB || !IEA) return ifail(v, GPX_EARG, "gpx_inv_set_model: null argument");
+    hipSetDevice(v->desc.device);
+    INV_TRY(hipMemcpy(v->par, CA, v->ncell * 4, hipMemcpyHostToDevice));
+    INV_TRY(hipMemcpy(v->par + v->ncell, CB, v->ncell * 4, hipMemcpyHostToDevice));
+    INV_TRY(hipMemcpy(v->par + 2 * v->ncell, IEA, v->ncell * 4, hipMemcpyHostToDevice));
+    v->have_model = true;
+    v->have_forward = false;
+    return 0;
+}
+
+static int n_checkpoints(const gpx_inv *v) { return (v->desc.iterations + v->desc.K - 1) / v->desc.K; }
+static long long ck_stride(const gpx_inv *v) { return 6 * v->ntot + v->phi_total; }
+
+static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
+{
+    InvArgs a;
+    memset(&a, 0, sizeof(a));
+    const gpx_inv_desc &g = v->desc;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+    a.ntot = v->ntot;
+    a.d = g.d;
+    const double mu0 = 1.25663706127e-6;
+    a.DB = (float)((1.0 / g.d) * (g.dt / mu0));
+    a.cH = (float)(g.dt / mu0);
+    float *base = adjoint ? v->L : v->F;
+    for (int c = 0; c < 3; ++c) {
+        a.E[c] = base + c * v->ntot;
+        a.H[c] = base + (3 + c) * v->ntot;
+    }
+    a.CA = v->par;
+    a.CB = v->par + v->ncell;
+    a.IEA = v->par + 2 * v->ncell;
+    a.gCA = v->grad;
+    a.gCB = v->grad + v->ncell;
+    a.gIEA = v->grad + 2 * v->ncell;
+    a.nslab = (int)v->slabs.size();
+    for (int s = 0; s < a.nslab; ++s) {
+        a.slab[s] = v->slabs[s].s;
+        float *const *P = adjoint ? v->slabs[s].mu : v->slabs[s].phi;
+        a.slab[s].PE1 = P[0];
+        a.slab[s].PE2 = P[1];
+        a.slab[s].PH1 = P[2];
+        a.slab[s].PH2 = P[3];
+    }
+    a.si = v->si; a.sj = v->sj; a.sk = v->sk;
+    a.sfac = v->sfac;
+    a.waves = v->waves;
+    a.it = it;
+    a.iterations = g.iterations;
+    a.nrx = v->nrx;
+    a.rxc = v->rxc;
+    a.traces = v->traces;
+    a.resid = v->resid;
+    return a;
+}
+
+static int grid_for(const gpx_inv *v)
+{
+    const long long b = (v->ntot + 255) / 256;
+    return (int)(b < 65536 ? b : 65536);
+}
+
+// Copies the state (fields and Phi) to or from checkpoint c.
+static int checkpoint(gpx_inv *v, int c, bool save)
+{
+    float *dst = v->ck + (long long)c * ck_stride(v);
+    auto cp = [&](float *ckp, float *live, long long n) {
+        return save ? hipMemcpyAsync(ckp, live, n * 4, hipMemcpyDeviceToDevice, v->stream)
+                    : hipMemcpyAsync(live, ckp, n * 4, hipMemcpyDeviceToDevice, v->stream);
+    };
+    INV_TRY(cp(dst, v->F, 6 * v->ntot));
+    long long off = 6 * v->ntot;
+    for (auto &S : v->slabs)
+        for (int q = 0; q < 4; ++q) {
+            INV_TRY(cp(dst + off, S.phi[q], S.n[q]));
+            off += S.n[q];
+        }
+    return 0;
+}
+
+static void forward_step(gpx_inv *v, int it, bool rx)
+{
+    InvArgs a = inv_args(v, false, it);
+    if (rx && v->nrx) hipLaunchKernelGGL(k_inv_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
+    hipLaunchKernelGGL(k_inv_h, dim3(grid_for(v)), dim3(256), 0, v->stream, a);
+    hipLaunchKernelGGL(k_inv_e, dim3(grid_for(v)), dim3(256), 0, v->stream, a);
+}
+
+extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, const float *waves, int nrx,
+                               const int *rx, float *traces)
+{
+    if (!v || !waves || nrx < 0 || (nrx > 0 && (!rx || !traces)))
+        return ifail(v, GPX_EARG, "gpx_inv_forward: bad argument");
+    if (!v->have_model) return ifail(v, GPX_ESTATE, "gpx_inv_forward: no model uploaded");
+    const gpx_inv_desc &g = v->desc;
+    if (si < 1 || si >= g.nx || sj < 1 || sj >= g.ny || sk < 0 || sk >= g.nz)
+        return ifail(v, GPX_EARG, "gpx_inv_forward: source edge outside the updated Ez range");
+    for (int r = 0; r < nrx; ++r)
+        if (rx[3 * r] < 0 || rx[3 * r] > g.nx || rx[3 * r + 1] < 0 || rx[3 * r + 1] > g.ny || rx[3 * r + 2] < 0 ||
+            rx[3 * r + 2] > g.nz)
+            return ifail(v, GPX_EARG, "gpx_inv_forward: receiver outside the grid");
+    hipSetDevice(g.device);
+    v->si = si; v->sj = sj; v->sk = sk;
+    v->sfac = sfac;
+    INV_TRY(hipMemcpy(v->waves, waves, (size_t)g.iterations * 4, hipMemcpyHostToDevice));
+    if (nrx != v->nrx || !v->rxc) {
+        if (v->rxc) hipFree(v->rxc);
+        if (v->traces) hipFree(v->traces);
+        if (v->resid) hipFree(v->resid);
+        v->rxc = nullptr;
+        v->traces = v->resid = nullptr;
+        v->nrx = nrx;
+        INV_TRY(hipMalloc((void **)&v->rxc, (size_t)(nrx > 0 ? nrx : 1) * 3 * 4));
+        INV_TRY(hipMalloc((void **)&v->traces, (size_t)(nrx > 0 ? nrx : 1) * g.iterations * 4));
+        INV_TRY(hipMalloc((void **)&v->resid, (size_t)(nrx > 0 ? nrx : 1) * g.iterations * 4));
+    }
+    if (nrx) INV_TRY(hipMemcpy(v->rxc, rx, (size_t)nrx * 3 * 4, hipMemcpyHostToDevice));
+    if (!v->ck) INV_TRY(hipMalloc((void **)&v->ck, (size_t)n_checkpoints(v) * ck_stride(v) * 4));
+    INV_TRY(hipMemsetAsync(v->F, 0, 6 * v->ntot * 4, v->stream));
+    for (auto &S : v->slabs)
+        for (int q = 0; q < 4; ++q) INV_TRY(hipMemsetAsync(S.phi[q], 0, S.n[q] * 4, v->stream));
+    hipEvent_t e0, e1;
+    INV_TRY(hipEventCreate(&e0));
+    INV_TRY(hipEventCreate(&e1));
+    INV_TRY(hipEventRecord(e0, v->stream));
+    for (int it = 0; it < g.iterations; ++it) {
+        if (it % g.K == 0)
+            if (int rc = checkpoint(v, it / g.K, true)) return rc;
+        forward_step(v, it, true);
+    }
+    INV_TRY(hipGetLastError());
+    INV_TRY(hipEventRecord(e1, v->stream));
+    INV_TRY(hipEventSynchronize(e1));
+    INV_TRY(hipEventElapsedTime(&v->ms_fwd, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (nrx) INV_TRY(hipMemcpy(traces, v->traces, (size_t)nrx * g.iterations * 4, hipMemcpyDeviceToHost));
+    v->have_forward = true;
+    return 0;
+}
+
+extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, float *gCB, float *gIEA)
+{
+    if (!v || !gCA || !gCB || !gIEA || (v->nrx > 0 && !resid)) return ifail(v, GPX_EARG, "gpx_inv_backward: bad argument");
+    if (!v->have_forward) return ifail(v, GPX_ESTATE, "gpx_inv_backward: run gpx_inv_forward first");
+    const gpx_inv_desc &g = v->desc;
+    hipSetDevice(g.device);
+    if (v->nrx) INV_TRY(hipMemcpy(v->resid, resid, (size_t)v->nrx * g.iterations * 4, hipMemcpyHostToDevice));
+    INV_TRY(hipMemsetAsync(v->L, 0, 6 * v->ntot * 4, v->stream));
+    INV_TRY(hipMemsetAsync(v->grad, 0, 3 * v->ncell * 4, v->stream));
+    for (auto &S : v->slabs)
+        for (int q = 0; q < 4; ++q) INV_TRY(hipMemsetAsync(S.mu[q], 0, S.n[q] * 4, v->stream));
+    hipEvent_t e0, e1;
+    INV_TRY(hipEventCreate(&e0));
+    INV_TRY(hipEventCreate(&e1));
+    INV_TRY(hipEventRecord(e0, v->stream));
+    const int G = grid_for(v);
+    for (int c = n_checkpoints(v) - 1; c >= 0; --c) {
+        const int n0 = c * g.K, n1 = n0 + g.K < g.iterations ? n0 + g.K : g.iterations;
+        // recompute the segment, storing E^n and H^{n+1}
+        if (int rc = checkpoint(v, c, false)) return rc;
+        for (int it = n0; it < n1; ++it) {
+            float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
+            INV_TRY(hipMemcpyAsync(slot, v->F, 3 * v->ntot * 4, hipMemcpyDeviceToDevice, v->stream));
+            InvArgs a = inv_args(v, false, it);
+            hipLaunchKernelGGL(k_inv_h, dim3(G), dim3(256), 0, v->stream, a);
+            INV_TRY(hipMemcpyAsync(slot + 3 * v->ntot, v->F + 3 * v->ntot, 3 * v->ntot * 4,
+                                   hipMemcpyDeviceToDevice, v->stream));
+            hipLaunchKernelGGL(k_inv_e, dim3(G), dim3(256), 0, v->stream, a);
+        }
+        // adjoint sweep over the segment
+        for (int it = n1 - 1; it >= n0; --it) {
+            const float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
+            InvArgs a = inv_args(v, true, it);
+            for (int q = 0; q < 3; ++q) {
+                a.Es[q] = slot + q * v->ntot;
+                a.Hs[q] = slot + (3 + q) * v->ntot;
+            }
+            hipLaunchKernelGGL(k_adj_e_gather, dim3(G), dim3(256), 0, v->stream, a);
+            hipLaunchKernelGGL(k_adj_e_update, dim3(G), dim3(256), 0, v->stream, a);
+            hipLaunchKernelGGL(k_adj_h_gather, dim3(G), dim3(256), 0, v->stream, a);
+            hipLaunchKernelGGL(k_adj_h_update, dim3(G), dim3(256), 0, v->stream, a);
+            if (v->nrx) hipLaunchKernelGGL(k_adj_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
+        }
+        INV_TRY(hipGetLastError());
+    }
+    INV_TRY(hipEventRecord(e1, v->stream));
+    INV_TRY(hipEventSynchronize(e1));
+    INV_TRY(hipEventElapsedTime(&v->ms_bwd, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    INV_TRY(hipMemcpy(gCA, v->grad, v->ncell * 4, hipMemcpyDeviceToHost));
+    INV_TRY(hipMemcpy(gCB, v->grad + v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
+    INV_TRY(hipMemcpy(gIEA, v->grad + 2 * v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int gpx_inv_timings(const gpx_inv *v, float *ms_forward, float *ms_backward)
+{
+    if (!v) return GPX_EARG;
+    if (ms_forward) *ms_forward = v->ms_fwd;
+    if (ms_backward) *ms_backward = v->ms_bwd;
+    return 0;
+}

@@ -1278,11 +1278,11 @@ constexpr int STEP_LDS_MAX = 64 * 1024;
 
 // The fused kernels are compiled in step_*.hip, one file per PML form.
 namespace gpx {
-step_kernel step_kernel_plain(int id_bytes, int P, bool C);
-step_kernel step_kernel_hori1(int id_bytes, int P, bool C);
-step_kernel step_kernel_hori2(int id_bytes, int P, bool C);
-step_kernel step_kernel_mri1(int id_bytes, int P, bool C);
-step_kernel step_kernel_mri2(int id_bytes, int P, bool C);
+StepKernels step_kernel_plain(int id_bytes, int P, bool C);
+StepKernels step_kernel_hori1(int id_bytes, int P, bool C);
+StepKernels step_kernel_hori2(int id_bytes, int P, bool C);
+StepKernels step_kernel_mri1(int id_bytes, int P, bool C);
+StepKernels step_kernel_mri2(int id_bytes, int P, bool C);
 }  // namespace gpx
 
 // PML order shared by every slab, 0 without slabs, -1 when they differ.
@@ -1296,7 +1296,7 @@ static int slab_order(const gpx_ctx *ctx)
     return o;
 }
 
-static step_kernel pick_step(const gpx_ctx *ctx)
+static StepKernels pick_step(const gpx_ctx *ctx)
 {
     const int P = ctx->desc.maxpoles, idb = ctx->id_bytes;
     const bool C = ctx->disp_complex != 0;
@@ -1440,7 +1440,7 @@ static hipError_t launch_step(gpx_ctx *ctx, int tc, int it)
     }
     FusedArgs a;
     fill_step_args(ctx, tc, it, a);
-    const step_kernel k = pick_step(ctx);
+    const step_kernel k = pick_step(ctx).one;
     const int B = tc / STEP_CELLS_PER_THREAD;
     const size_t lds = step_lds(ctx, tc);
     int grid;
@@ -1770,4 +1770,322 @@ extern "C" long long gpx_interior_cells(const gpx_ctx *ctx)
 extern "C" size_t gpx_memory_bytes(const gpx_ctx *ctx)
 {
     return ctx ? ctx->bytes : 0;
+}
+
+// ====================================================================== batch
+// Many models advanced together: one launch of the fused step per iteration
+// covers the work items of every model, optionally all iterations captured in
+// one HIP graph.  Each model keeps its own gpx_ctx (buffers, sources,
+// receivers); the batch only holds the launch tables.
+struct gpx_batch {
+    std::vector<gpx_ctx *> ctx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int tile = 0, xchunk = 0;      // launch shape; 0 = chosen at run time
+    FusedArgs *dargs = nullptr;    // [2][ntr]
+    int *ditem0 = nullptr;         // [ntr + 1]
+    std::string err;
+};
+
+static int bfail(gpx_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    else g_err = msg;
+    return code;
+}
+
+#define GPX_BTRY(call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return bfail(b, (int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *gpx_batch_last_error(const gpx_batch *b)
+{
+    return b ? b->err.c_str() : g_err.c_str();
+}
+
+// Tile sizes every model of the batch accepts, smallest first.
+static std::vector<int> batch_tiles(const gpx_batch *b)
+{
+    std::vector<int> out;
+    for (int tc : {512, 1024, 2048}) {
+        bool ok = true;
+        for (const gpx_ctx *c : b->ctx) {
+            const int NZ1 = c->desc.nz + 1;
+            if (step_lds(c, tc) > (size_t)STEP_LDS_MAX || 2 * NZ1 > tc) ok = false;
+        }
+        if (ok) out.push_back(tc);
+    }
+    return out;
+}
+
+extern "C" int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out)
+{
+    if (!ctxs || n < 1 || !out) return bfail(nullptr, GPX_EARG, "gpx_batch_create: bad argument");
+    *out = nullptr;
+    const gpx_ctx *c0 = ctxs[0];
+    for (int t = 0; t < n; ++t) {
+        gpx_ctx *c = ctxs[t];
+        if (!c) return bfail(nullptr, GPX_EARG, "gpx_batch_create: null context");
+        if (int rc = check_ready(c)) return bfail(nullptr, rc, "gpx_batch_create: model " + std::to_string(t) + ": " + c->err);
+        if (c->desc.device != c0->desc.device)
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models must be on one device");
+        if (c->desc.iterations != c0->desc.iterations)
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models must have the same number of iterations");
+        if (!c->snaps.empty())
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: models with snapshots are stepped one by one");
+        if (slab_order(c) < 0 || (double)c->fstride * 24.0 >= 4294967295.0 ||
+            (double)c->ntot * 6 * c->id_bytes >= 4294967295.0)
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: model " + std::to_string(t) + " cannot use the fused step");
+        if (pick_step(c).many != pick_step(c0).many)
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models need the same kernel (index width, "
+                                            "poles and their kind, PML formulation and order)");
+    }
+    gpx_batch *b = new gpx_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->device = c0->desc.device;
+    if (batch_tiles(b).empty()) {
+        delete b;
+        return bfail(nullptr, GPX_EARG, "gpx_batch_create: no tile size fits every model");
+    }
+    hipSetDevice(b->device);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->dargs, sizeof(FusedArgs) * 2 * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ditem0, sizeof(int) * (n + 1));
+    if (e != hipSuccess) {
+        if (b->dargs) hipFree(b->dargs);
+        if (b->stream) hipStreamDestroy(b->stream);
+        delete b;
+        return bfail(nullptr, (int)e, std::string("gpx_batch_create: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int gpx_batch_destroy(gpx_batch *b)
+{
+    if (!b) return 0;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->dargs) hipFree(b->dargs);
+    if (b->ditem0) hipFree(b->ditem0);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+extern "C" int gpx_batch_set_shape(gpx_batch *b, int tile_cells, int xchunk)
+{
+    if (!b || xchunk < 0) return bfail(b, GPX_EARG, "gpx_batch_set_shape: bad argument");
+    if (tile_cells) {
+        const std::vector<int> ok = batch_tiles(b);
+        bool found = false;
+        for (int tc : ok) found = found || tc == tile_cells;
+        if (!found) return bfail(b, GPX_EARG, "gpx_batch_set_shape: tile size not usable by every model");
+    }
+    b->tile = tile_cells;
+    b->xchunk = xchunk;
+    return 0;
+}
+
+// Launch tables for tile size `tc`: every model's arguments for both state
+// orders, the first item of each model, and the common x-chunk length that
+// minimises rounds x planes per item over all items.
+struct BatchPlan {
+    int tc = 0, B = 0, xc = 0, nitems = 0, grid = 0;
+    size_t lds = 0;
+    batch_kernel k = nullptr;
+};
+
+static int batch_prepare(gpx_batch *b, int tc, int xchunk, BatchPlan &p)
+{
+    const int n = (int)b->ctx.size();
+    p.tc = tc;
+    p.B = tc / STEP_CELLS_PER_THREAD;
+    p.k = pick_step(b->ctx[0]).many;
+    p.lds = 0;
+    int maxnx1 = 0;
+    for (gpx_ctx *c : b->ctx) {
+        if (c->rare_dirty || !c->rare) {
+            hipError_t e = upload_rare(c);
+            if (e != hipSuccess) return bfail(b, (int)e, std::string("gpx_batch: ") + hipGetErrorString(e));
+        }
+        const size_t l = step_lds(c, tc);
+        if (l > p.lds) p.lds = l;
+        if (c->desc.nx + 1 > maxnx1) maxnx1 = c->desc.nx + 1;
+    }
+    const int resident = resident_blocks(b->ctx[0], (const void *)p.k, p.B, p.lds);
+    std::vector<FusedArgs> args(2 * (size_t)n);
+    for (int t = 0; t < n; ++t) fill_step_args(b->ctx[t], tc, 0, args[t]);
+    int xc = xchunk;
+    if (!xc) {
+        long long best = -1;
+        for (int c = 2; c <= maxnx1; ++c) {
+            long long items = 0;
+            for (int t = 0; t < n; ++t) items += (long long)args[t].ntiles * ((b->ctx[t]->desc.nx + 1 + c - 1) / c);
+            const long long cost = (items + resident - 1) / resident * (long long)(c + 1);
+            if (best < 0 || cost <= best) {
+                best = cost;
+                xc = c;
+            }
+        }
+    }
+    std::vector<int> item0(n + 1, 0);
+    for (int t = 0; t < n; ++t) {
+        FusedArgs &a = args[t];
+        a.xc = xc;
+        a.nitems = a.ntiles * ((b->ctx[t]->desc.nx + 1 + xc - 1) / xc);
+        item0[t + 1] = item0[t] + a.nitems;
+        FusedArgs &s = args[n + t];  // the same model with its two states exchanged
+        s = a;
+        const float *fc = s.Fc;
+        s.Fc = s.Fx;
+        s.Fx = const_cast<float *>(fc);
+        for (int q = 0; q < s.nslab; ++q) {
+            std::swap(s.slabH[q].phi1, s.slabH[q].phw1);
+            std::swap(s.slabH[q].phi2, s.slabH[q].phw2);
+        }
+    }
+    p.xc = xc;
+    p.nitems = item0[n];
+    long long g = resident < p.nitems ? resident : p.nitems;
+    g = (g + 7) / 8 * 8;
+    p.grid = (int)g;
+    hipSetDevice(b->device);
+    GPX_BTRY(hipMemcpyAsync(b->dargs, args.data(), args.size() * sizeof(FusedArgs), hipMemcpyHostToDevice, b->stream));
+    GPX_BTRY(hipMemcpyAsync(b->ditem0, item0.data(), item0.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    GPX_BTRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+static BatchArgs batch_args(const gpx_batch *b, const BatchPlan &p, int step, int it)
+{
+    BatchArgs a;
+    a.args = b->dargs;
+    a.item0 = b->ditem0;
+    a.ntr = (int)b->ctx.size();
+    a.nitems = p.nitems;
+    a.per_xcd = p.grid / 8;
+    a.flip = step & 1;
+    a.it = it;
+    return a;
+}
+
+// After n batched iterations every model's next state is its current one
+// when n is odd.
+static void batch_swap(gpx_batch *b, int n)
+{
+    if (!(n & 1)) return;
+    for (gpx_ctx *c : b->ctx) {
+        for (int q = 0; q < 6; ++q) std::swap(c->F[q], c->Fn[q]);
+        for (auto &S : c->slabs) {
+            std::swap(S.phi[2], S.phin[0]);
+            std::swap(S.phi[3], S.phin[1]);
+        }
+    }
+}
+
+static int batch_steps(gpx_batch *b, const BatchPlan &p, int it0, int n, int graph, float *ms)
+{
+    // Work queued on the models' own streams (uploads) completes first.
+    for (gpx_ctx *c : b->ctx) GPX_BTRY(hipStreamSynchronize(c->stream));
+    hipEvent_t e0, e1;
+    GPX_BTRY(hipEventCreate(&e0));
+    GPX_BTRY(hipEventCreate(&e1));
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) {
+        GPX_BTRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < n; ++s)
+            hipLaunchKernelGGL(p.k, dim3(p.grid), dim3(p.B), p.lds, b->stream, batch_args(b, p, s, it0 + s));
+        GPX_BTRY(hipStreamEndCapture(b->stream, &gr));
+        GPX_BTRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    }
+    GPX_BTRY(hipEventRecord(e0, b->stream));
+    if (graph) {
+        GPX_BTRY(hipGraphLaunch(ge, b->stream));
+    } else {
+        for (int s = 0; s < n; ++s) {
+            hipLaunchKernelGGL(p.k, dim3(p.grid), dim3(p.B), p.lds, b->stream, batch_args(b, p, s, it0 + s));
+            GPX_BTRY(hipGetLastError());
+        }
+    }
+    GPX_BTRY(hipEventRecord(e1, b->stream));
+    GPX_BTRY(hipEventSynchronize(e1));
+    float t = 0;
+    GPX_BTRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    if (ge) hipGraphExecDestroy(ge);
+    if (gr) hipGraphDestroy(gr);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    batch_swap(b, n);
+    return 0;
+}
+
+extern "C" int gpx_batch_run(gpx_batch *b, int it0, int n, int graph, float *ms)
+{
+    if (!b || it0 < 0 || n < 1 || it0 + n > b->ctx[0]->desc.iterations)
+        return bfail(b, GPX_EARG, "gpx_batch_run: bad argument");
+    hipSetDevice(b->device);
+    BatchPlan p;
+    const int tc = b->tile ? b->tile : batch_tiles(b).back();
+    if (int rc = batch_prepare(b, tc, b->xchunk, p)) return rc;
+    return batch_steps(b, p, it0, n, graph, ms);
+}
+
+// Zeroes the whole state of a model: both field copies, Phi, T and the
+// receiver buffer.
+static int zero_state(gpx_ctx *ctx)
+{
+    gpx_ctx *c = ctx;
+    hipStream_t s = c->stream;
+    for (float *m : c->Fmem) GPX_TRY(hipMemsetAsync(m, 0, (size_t)6 * c->fstride * 4, s));
+    for (const Slab &S : c->slabs) {
+        for (int q = 0; q < 4; ++q)
+            GPX_TRY(hipMemsetAsync(S.phi[q], 0, (size_t)S.order * S.dims[q][0] * S.dims[q][1] * S.dims[q][2] * 4, s));
+        for (int q = 0; q < 2; ++q)
+            GPX_TRY(hipMemsetAsync(S.phin[q], 0, (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4, s));
+    }
+    if (c->desc.maxpoles > 0)
+        for (float *T : c->T)
+            if (T) GPX_TRY(hipMemsetAsync(T, 0, (size_t)c->desc.maxpoles * c->ntot * 4 * (c->disp_complex ? 2 : 1), s));
+    if (c->rxbuf) GPX_TRY(hipMemsetAsync(c->rxbuf, 0, (size_t)6 * c->desc.iterations * c->nrx * 4, s));
+    GPX_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xchunk, float *ms_per_step)
+{
+    if (!b || steps < 1 || steps > b->ctx[0]->desc.iterations) return bfail(b, GPX_EARG, "gpx_batch_tune: bad argument");
+    hipSetDevice(b->device);
+    float best = -1.f;
+    int bt = 0, bx = 0;
+    for (int tc : batch_tiles(b)) {
+        BatchPlan p;
+        if (int rc = batch_prepare(b, tc, 0, p)) return rc;
+        const int base = p.xc;
+        for (int xc : {base, 2 * base, (base + 1) / 2}) {
+            if (xc < 2) continue;
+            if (int rc = batch_prepare(b, tc, xc, p)) return rc;
+            float ms = 0;
+            if (int rc = batch_steps(b, p, 0, 1, 0, nullptr)) return rc;  // first launch of a shape
+            if (int rc = batch_steps(b, p, 0, steps, 0, &ms)) return rc;
+            ms /= steps;
+            if (best < 0 || ms < best) {
+                best = ms;
+                bt = tc;
+                bx = xc;
+            }
+        }
+    }
+    for (gpx_ctx *c : b->ctx)
+        if (int rc = zero_state(c)) return bfail(b, rc, "gpx_batch_tune: " + c->err);
+    b->tile = bt;
+    b->xchunk = bx;
+    if (tile_cells) *tile_cells = bt;
+    if (xchunk) *xchunk = bx;
+    if (ms_per_step) *ms_per_step = best;
+    return 0;
 }

@@ -352,30 +352,67 @@ struct FusedArgs {
     const uint32_t *planeflag;
 };
 
-template <class IDT, int P, bool CPLX, int FORM, int ORDER, int CPT>
-__global__ void __launch_bounds__(1024) k_step(FusedArgs a)
+// Pointers into the dynamic LDS of the fused step: both coefficient tables,
+// both R tables, the dispersive coefficients, the H exchange area of the
+// block's cells and the dispersive flags; the slab descriptions are static.
+struct StepLds {
+    float4 *cE, *cH;
+    float *sE, *sH, *lRE, *lRH, *ldisp, *lHx, *lHy, *lHz;
+    uint8_t *lflag;
+    SlabArgs *lsE, *lsH;
+};
+
+template <int P, bool CPLX, int CPT>
+__device__ __forceinline__ StepLds step_layout(float4 *lds4, SlabArgs *lsE, SlabArgs *lsH, const FusedArgs &a)
 {
-    extern __shared__ float4 lds4[];
-    __shared__ SlabArgs lsE[MAX_SLABS], lsH[MAX_SLABS];
-    const int nm = a.nmat, B = blockDim.x, TC = B * CPT;
+    StepLds S;
+    const int nm = a.nmat, TC = blockDim.x * CPT;
     const int nd = P > 0 ? nm * 3 * P * (CPLX ? 2 : 1) : 0;
-    float4 *cE = lds4, *cH = lds4 + nm;
-    float *sE = reinterpret_cast<float *>(lds4 + 2 * nm), *sH = sE + nm;
-    float *lRE = sH + nm, *lRH = lRE + a.nR;
-    float *ldisp = lRH + a.nR;
-    float *lHx = ldisp + nd, *lHy = lHx + TC, *lHz = lHy + TC;
-    uint8_t *lflag = reinterpret_cast<uint8_t *>(lHz + TC);
-    stage_coef(cE, sE, a.coefE, nm);
-    stage_coef(cH, sH, a.coefH, nm);
-    stage(lRE, a.RE, a.nR);
-    stage(lRH, a.RH, a.nR);
-    stage_slabs(lsE, a.slabE, a.nslab);
-    stage_slabs(lsH, a.slabH, a.nslab);
+    S.cE = lds4;
+    S.cH = lds4 + nm;
+    S.sE = reinterpret_cast<float *>(lds4 + 2 * nm);
+    S.sH = S.sE + nm;
+    S.lRE = S.sH + nm;
+    S.lRH = S.lRE + a.nR;
+    S.ldisp = S.lRH + a.nR;
+    S.lHx = S.ldisp + nd;
+    S.lHy = S.lHx + TC;
+    S.lHz = S.lHy + TC;
+    S.lflag = reinterpret_cast<uint8_t *>(S.lHz + TC);
+    S.lsE = lsE;
+    S.lsH = lsH;
+    return S;
+}
+
+// Copies one model's tables to LDS (the caller synchronises).
+template <int P, bool CPLX>
+__device__ __forceinline__ void step_stage(const StepLds &S, const FusedArgs &a)
+{
+    const int nm = a.nmat;
+    stage_coef(S.cE, S.sE, a.coefE, nm);
+    stage_coef(S.cH, S.sH, a.coefH, nm);
+    stage(S.lRE, a.RE, a.nR);
+    stage(S.lRH, a.RH, a.nR);
+    stage_slabs(S.lsE, a.slabE, a.nslab);
+    stage_slabs(S.lsH, a.slabH, a.nslab);
     if (P > 0) {
-        stage(ldisp, a.disp, nd);
-        for (int q = threadIdx.x; q < nm; q += B) lflag[q] = a.dispflag[q];
+        stage(S.ldisp, a.disp, nm * 3 * P * (CPLX ? 2 : 1));
+        for (int q = threadIdx.x; q < nm; q += blockDim.x) S.lflag[q] = a.dispflag[q];
     }
-    __syncthreads();
+}
+
+// One work item (tile, x chunk) of iteration `it` of one model.
+template <class IDT, int P, bool CPLX, int FORM, int ORDER, int CPT>
+__device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, int item, int it)
+{
+    const int B = blockDim.x;
+    const float4 *cE = S.cE, *cH = S.cH;
+    const float *sE = S.sE, *sH = S.sH, *lRE = S.lRE, *lRH = S.lRH, *ldisp = S.ldisp;
+    float *lHx = S.lHx, *lHy = S.lHy, *lHz = S.lHz;
+    const uint8_t *lflag = S.lflag;
+    const SlabArgs *lsE = S.lsE, *lsH = S.lsH;
+    (void)ldisp;
+    (void)lflag;
 
     const int nx = a.nx, ny = a.ny, nz = a.nz, NZ1 = nz + 1;
     const size_t N = a.ntot, Pl = a.plane;
@@ -385,8 +422,7 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
     const rsrc_t rF = rsrc(a.Fc), wF = rsrc(a.Fx), rI = rsrc(a.ID);
     const uint32_t cs = a.cs, ci = (uint32_t)N * (uint32_t)sizeof(IDT);
     const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
-
-    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
+    {
         const int tile = item % a.ntiles, chunk = item / a.ntiles;
         const int q0 = tile * a.L - NZ1;
         const int qe = min(tile * a.L + a.L, a.plane);
@@ -529,11 +565,11 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
                     const SrcArgs &M = a.rare->srcM;
                     for (int s = 0; s < M.n; ++s) {
                         const int *in = M.info + 6 * s;
-                        if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                        if (in[0] != i || it < in[4] || it > in[5]) continue;
                         if (in[1] == j && in[2] == k) {
                             const int pol = in[3];
                             const uint32_t m = pol == 0 ? m3[u] : (pol == 1 ? m4[u] : m5[u]);
-                            const float w = M.waves[(size_t)s * a.iterations + a.it];
+                            const float w = M.waves[(size_t)s * a.iterations + it];
                             const float v = sH[m] * w * M.fac[2 * s];
                             if (pol == 0) nhx = nhx - v;
                             else if (pol == 1) nhy = nhy - v;
@@ -545,7 +581,7 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
                     const StepRare &Q = *a.rare;
                     for (int r = 0; r < Q.nrx; ++r) {
                         if (Q.rxc[3 * r] != i || Q.rxc[3 * r + 1] != j || Q.rxc[3 * r + 2] != k) continue;
-                        float *o = Q.rxbuf + (size_t)a.it * Q.nrx + r;
+                        float *o = Q.rxbuf + (size_t)it * Q.nrx + r;
                         const size_t cs = (size_t)a.iterations * Q.nrx;
                         o[0] = ex[u]; o[cs] = ey[u]; o[2 * cs] = ez[u];
                         o[3 * cs] = hx[u]; o[4 * cs] = hy[u]; o[5 * cs] = hz[u];
@@ -649,12 +685,12 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
                         const SrcArgs &V = a.rare->srcV, &D = a.rare->srcD;
                         for (int s = 0; s < V.n; ++s) {  // gprMax/sources.py:81-120
                             const int *in = V.info + 6 * s;
-                            if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                            if (in[0] != i || it < in[4] || it > in[5]) continue;
                             if (in[1] == j && in[2] == k) {
                                 const int pol = in[3] & 3;
                                 const bool hard = in[3] >= 4;
                                 const uint32_t m = pol == 0 ? m0[u] : (pol == 1 ? m1[u] : m2[u]);
-                                const float w = V.waves[(size_t)s * a.iterations + a.it];
+                                const float w = V.waves[(size_t)s * a.iterations + it];
                                 float *F = pol == 0 ? &nex : (pol == 1 ? &ney : &nez);
                                 if (hard) *F = -w / V.fac[2 * s];
                                 else *F = *F - sE[m] * w * V.fac[2 * s];
@@ -662,11 +698,11 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
                         }
                         for (int s = 0; s < D.n; ++s) {  // gprMax/sources.py:163-193
                             const int *in = D.info + 6 * s;
-                            if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                            if (in[0] != i || it < in[4] || it > in[5]) continue;
                             if (in[1] == j && in[2] == k) {
                                 const int pol = in[3];
                                 const uint32_t m = pol == 0 ? m0[u] : (pol == 1 ? m1[u] : m2[u]);
-                                const float w = D.waves[(size_t)s * a.iterations + a.it];
+                                const float w = D.waves[(size_t)s * a.iterations + it];
                                 const float v = sE[m] * w * D.fac[2 * s] * D.fac[2 * s + 1];
                                 if (pol == 0) nex = nex - v;
                                 else if (pol == 1) ney = ney - v;
@@ -697,32 +733,90 @@ __global__ void __launch_bounds__(1024) k_step(FusedArgs a)
     }
 }
 
+template <class IDT, int P, bool CPLX, int FORM, int ORDER, int CPT>
+__global__ void __launch_bounds__(1024) k_step(FusedArgs a)
+{
+    extern __shared__ float4 lds4[];
+    __shared__ SlabArgs lsE[MAX_SLABS], lsH[MAX_SLABS];
+    const StepLds S = step_layout<P, CPLX, CPT>(lds4, lsE, lsH, a);
+    step_stage<P, CPLX>(S, a);
+    __syncthreads();
+    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x)
+        step_item<IDT, P, CPLX, FORM, ORDER, CPT>(a, S, item, a.it);
+}
+
+// The fused step of many models in one launch.  `args` holds every model's
+// arguments twice, [2][ntr]: for its current state and with the two states
+// exchanged; `flip` picks the row, so consecutive iterations alternate rows
+// without host updates.  Model t owns items [item0[t], item0[t+1]).
+struct BatchArgs {
+    const FusedArgs *args;
+    const int *item0;
+    int ntr, nitems, per_xcd;
+    int flip, it;
+};
+
+template <class IDT, int P, bool CPLX, int FORM, int ORDER, int CPT>
+__global__ void __launch_bounds__(1024) k_step_batch(BatchArgs b)
+{
+    extern __shared__ float4 lds4[];
+    __shared__ SlabArgs lsE[MAX_SLABS], lsH[MAX_SLABS];
+    int staged = -1;
+    for (int g = logical_block(b.per_xcd); g < b.nitems; g += gridDim.x) {
+        int lo = 0, hi = b.ntr - 1;  // last model whose first item <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (b.item0[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const FusedArgs &a = b.args[b.flip * b.ntr + lo];
+        const StepLds S = step_layout<P, CPLX, CPT>(lds4, lsE, lsH, a);
+        if (lo != staged) {
+            __syncthreads();
+            step_stage<P, CPLX>(S, a);
+            __syncthreads();
+            staged = lo;
+        }
+        step_item<IDT, P, CPLX, FORM, ORDER, CPT>(a, S, g - b.item0[lo], b.it);
+    }
+}
+
 typedef void (*step_kernel)(FusedArgs);
+typedef void (*batch_kernel)(BatchArgs);
 
 // Cells per thread of the fused step: a block of B threads holds B x this.
 constexpr int STEP_CELLS_PER_THREAD = 2;
 
-// Fused kernel for the given material-index width, poles and coefficient kind.
+// Fused kernels (one model per launch, and many) for the given
+// material-index width, poles and coefficient kind.
+struct StepKernels {
+    step_kernel one;
+    batch_kernel many;
+};
+
 template <int FORM, int ORDER>
-step_kernel step_kernel_table(int id_bytes, int P, bool C)
+StepKernels step_kernel_table(int id_bytes, int P, bool C)
 {
     constexpr int K = STEP_CELLS_PER_THREAD;
+#define GPX_SK(IDT, PP, CC) \
+    StepKernels{k_step<IDT, PP, CC, FORM, ORDER, K>, k_step_batch<IDT, PP, CC, FORM, ORDER, K>}
     if (id_bytes == 2) {
         switch (P) {
-        case 0: return k_step<uint16_t, 0, false, FORM, ORDER, K>;
-        case 1: return C ? k_step<uint16_t, 1, true, FORM, ORDER, K> : k_step<uint16_t, 1, false, FORM, ORDER, K>;
-        case 2: return C ? k_step<uint16_t, 2, true, FORM, ORDER, K> : k_step<uint16_t, 2, false, FORM, ORDER, K>;
-        case 3: return C ? k_step<uint16_t, 3, true, FORM, ORDER, K> : k_step<uint16_t, 3, false, FORM, ORDER, K>;
-        default: return C ? k_step<uint16_t, 4, true, FORM, ORDER, K> : k_step<uint16_t, 4, false, FORM, ORDER, K>;
+        case 0: return GPX_SK(uint16_t, 0, false);
+        case 1: return C ? GPX_SK(uint16_t, 1, true) : GPX_SK(uint16_t, 1, false);
+        case 2: return C ? GPX_SK(uint16_t, 2, true) : GPX_SK(uint16_t, 2, false);
+        case 3: return C ? GPX_SK(uint16_t, 3, true) : GPX_SK(uint16_t, 3, false);
+        default: return C ? GPX_SK(uint16_t, 4, true) : GPX_SK(uint16_t, 4, false);
         }
     }
     switch (P) {
-    case 0: return k_step<uint32_t, 0, false, FORM, ORDER, K>;
-    case 1: return C ? k_step<uint32_t, 1, true, FORM, ORDER, K> : k_step<uint32_t, 1, false, FORM, ORDER, K>;
-    case 2: return C ? k_step<uint32_t, 2, true, FORM, ORDER, K> : k_step<uint32_t, 2, false, FORM, ORDER, K>;
-    case 3: return C ? k_step<uint32_t, 3, true, FORM, ORDER, K> : k_step<uint32_t, 3, false, FORM, ORDER, K>;
-    default: return C ? k_step<uint32_t, 4, true, FORM, ORDER, K> : k_step<uint32_t, 4, false, FORM, ORDER, K>;
+    case 0: return GPX_SK(uint32_t, 0, false);
+    case 1: return C ? GPX_SK(uint32_t, 1, true) : GPX_SK(uint32_t, 1, false);
+    case 2: return C ? GPX_SK(uint32_t, 2, true) : GPX_SK(uint32_t, 2, false);
+    case 3: return C ? GPX_SK(uint32_t, 3, true) : GPX_SK(uint32_t, 3, false);
+    default: return C ? GPX_SK(uint32_t, 4, true) : GPX_SK(uint32_t, 4, false);
     }
+#undef GPX_SK
 }
 
 }  // namespace gpx

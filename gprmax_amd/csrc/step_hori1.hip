@@ -3,7 +3,7 @@
 #include "gpx_step.h"
 
 namespace gpx {
-step_kernel step_kernel_hori1(int id_bytes, int P, bool C)
+StepKernels step_kernel_hori1(int id_bytes, int P, bool C)
 {
     return step_kernel_table<0, 1>(id_bytes, P, C);
 }

@@ -3,7 +3,7 @@
 #include "gpx_step.h"
 
 namespace gpx {
-step_kernel step_kernel_mri1(int id_bytes, int P, bool C)
+StepKernels step_kernel_mri1(int id_bytes, int P, bool C)
 {
     return step_kernel_table<1, 1>(id_bytes, P, C);
 }

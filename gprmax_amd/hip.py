@@ -61,6 +61,12 @@ SIGNATURES = {
     'gpx_device_info': ([_I, ctypes.c_char_p, _I, ctypes.POINTER(ctypes.c_size_t),
                          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I)], _I),
     'gpx_memory_bytes': ([_P], ctypes.c_size_t),
+    'gpx_batch_create': ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
+    'gpx_batch_destroy': ([_P], _I),
+    'gpx_batch_last_error': ([_P], ctypes.c_char_p),
+    'gpx_batch_set_shape': ([_P, _I, _I], _I),
+    'gpx_batch_tune': ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_F)], _I),
+    'gpx_batch_run': ([_P, _I, _I, _I, ctypes.POINTER(_F)], _I),
 }
 
 _lib = None
@@ -257,3 +263,55 @@ def device_info(device=0):
     if rc:
         raise GeneralError('gpx_device_info failed: {}'.format(lib.gpx_last_error(None).decode()))
     return {'name': name.value.decode(), 'totalmem': total.value, 'freemem': free.value, 'cus': cus.value}
+
+
+class HipBatch:
+    """Several HipSolver models advanced together, one fused launch per
+    iteration for all of them (optionally all iterations in one HIP graph)."""
+
+    def __init__(self, solvers):
+        self.lib = library()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.ctx.value for s in self.solvers])
+        b = ctypes.c_void_p()
+        rc = self.lib.gpx_batch_create(arr, len(self.solvers), ctypes.byref(b))
+        if rc:
+            raise GeneralError('gpx_batch_create failed ({}): {}'.format(rc, self.lib.gpx_batch_last_error(None).decode()))
+        self.b = b
+
+    def _check(self, rc):
+        if rc:
+            raise GeneralError('HIP batch error {}: {}'.format(rc, self.lib.gpx_batch_last_error(self.b).decode()))
+
+    def close(self):
+        if getattr(self, 'b', None):
+            self.lib.gpx_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_shape(self, tile_cells=0, xchunk=0):
+        self._check(self.lib.gpx_batch_set_shape(self.b, int(tile_cells), int(xchunk)))
+
+    def tune(self, steps=3):
+        """Pick the fastest launch shape; returns (tile_cells, xchunk, ms per
+        iteration).  Leaves every model at the zero state."""
+        t, x, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self._check(self.lib.gpx_batch_tune(self.b, int(steps), ctypes.byref(t), ctypes.byref(x), ctypes.byref(ms)))
+        return t.value, x.value, ms.value
+
+    def run(self, it0, n, graph=True):
+        """Advance every model by n iterations from it0; returns device ms."""
+        ms = ctypes.c_float()
+        self._check(self.lib.gpx_batch_run(self.b, int(it0), int(n), int(bool(graph)), ctypes.byref(ms)))
+        return ms.value

@@ -101,6 +101,32 @@ def dispersive_soil(iterations=3000, shape=(400, 400, 300), seed=1003, pml=20):
     return G
 
 
+def bscan_trace(t, seed=1004, iterations=1500, sides=(128, 256)):
+    """Trace t of the uneven B-scan: its own domain, each side drawn from
+    `sides` cells at 1 mm, 10-cell PML, lower half soil with a per-cell random
+    choice of 64 lossless materials (er ~ U[3, 15]), a z-directed Hertzian
+    dipole and an Ez receiver 10 cells apart in x, 5 cells above the soil,
+    stepped 2 cells per trace (wrapping inside the domain)."""
+    rng = np.random.default_rng([seed, t])
+    nx, ny, nz = (int(v) for v in rng.integers(sides[0], sides[1] + 1, 3))
+    b = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, iterations, pml_cells=10)
+    er = rng.uniform(3, 15, 64)
+    mats = [b.add_material(float(er[q]), 0, 1, 0, 'soil{}'.format(q)) for q in range(64)]
+    soil = np.zeros((nx, ny, nz), dtype=bool)
+    soil[:, :, :nz // 2] = True
+    ids = np.ones((nx, ny, nz), dtype=np.uint32)
+    ids[soil] = rng.integers(mats[0].numID, mats[0].numID + 64, int(soil.sum()), dtype=np.uint32)
+    b.paint_voxels(soil, ids)
+    b.add_waveform('ricker', 1, 1.5e9, 'ricker15')
+    xs = 15 + (2 * t) % (nx - 40)
+    zs = nz // 2 + 5
+    b.add_hertzian_dipole('z', (xs, ny // 2, zs), 'ricker15')
+    b.add_rx((xs + 10, ny // 2, zs))
+    G = b.finalise()
+    G.title = 'bscan_trace{}_{}x{}x{}'.format(t, nx, ny, nz)
+    return G
+
+
 WORKLOADS = {
     'freespace_ricker': freespace_ricker,
     'heterogeneous_halfspace': heterogeneous_halfspace,

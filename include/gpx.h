@@ -154,6 +154,78 @@ int gpx_device_info(int device, char *name, int namelen, size_t *totalmem, size_
 /* Bytes of device memory held by ctx. */
 size_t gpx_memory_bytes(const gpx_ctx *ctx);
 
+/* ---- Batched B-scan: many models in one launch per iteration.
+ *
+ * Replaces the reference's sequential trace loop, in which every B-scan trace
+ * re-runs run_model and so a whole solve_gpu (gprMax/gprMax.py:222-232 with
+ * gprMax/model_build_run.py:84-406); here the traces stay separate gpx_ctx
+ * models, each built and uploaded as usual, and the batch advances all of them
+ * with the fused step.  Requirements: one device, the same iteration count, no
+ * snapshots, and the same kernel (index width, poles and their kind, PML
+ * formulation and order) for every model.  The contexts must outlive the
+ * batch; results are read from each context (gpx_download_rx, ...). */
+typedef struct gpx_batch gpx_batch;
+
+int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out);
+int gpx_batch_destroy(gpx_batch *b);
+/* Message of the last failure (NULL: of the last gpx_batch_create). */
+const char *gpx_batch_last_error(const gpx_batch *b);
+/* Cells per block (512, 1024 or 2048; 0 = largest usable) and planes per x
+ * chunk (0 = chosen for the batch) of the batched launch. */
+int gpx_batch_set_shape(gpx_batch *b, int tile_cells, int xchunk);
+/* Autotuner: times `steps` iterations for each usable tile size and three
+ * chunk lengths around the planned one, keeps the fastest shape, reports it
+ * and its time per iteration, then resets every model to the zero state
+ * (fields, Phi, T, receiver buffer).  Call before stepping. */
+int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xchunk, float *ms_per_step);
+/* Iterations [it0, it0 + n) of every model; graph != 0 captures the n
+ * launches in one HIP graph and launches that.  Waits; *ms (optional) is the
+ * device time measured with HIP events around the launches. */
+int gpx_batch_run(gpx_batch *b, int it0, int n, int graph, float *ms);
+
+/* ---- Differentiable solve: forward with checkpoints and its adjoint.
+ *
+ * No counterpart in the reference (gprMax has no gradients; its optimiser is
+ * gradient-free, gprMax/optimisation_taguchi.py).  The forward scheme is the
+ * reference's solver loop (gprMax/model_build_run.py:424-470) with every cell
+ * its own material: per-cell CA, CB = CBx = CBy = CBz (cubic cells) and
+ * IEA = srce = 1/EA (gprMax/materials.py:178-201), mu_r = 1 and no magnetic
+ * loss, HORIPML with one CFS term, one z-directed Hertzian dipole and Ez
+ * receivers.  gpx_inv_backward returns dL/dCA, dL/dCB and dL/dIEA per cell
+ * for L with dL/d(trace) = resid; the PML correction's use of IEA is not
+ * differentiated, so gradients of cells inside a PML slab are incomplete. */
+typedef struct gpx_inv gpx_inv;
+
+typedef struct gpx_inv_desc {
+    int nx, ny, nz;       /* cells */
+    int iterations;
+    int K;                /* checkpoint interval (iterations) */
+    int device;
+    float d;              /* cell size (dx = dy = dz) */
+    float dt;             /* time step */
+} gpx_inv_desc;
+
+int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out);
+int gpx_inv_destroy(gpx_inv *v);
+const char *gpx_inv_last_error(const gpx_inv *v);
+/* One HORIPML slab (order 1), as gpx_add_pml; eR, hR = RA, RB, RE, RF [t]. */
+int gpx_inv_add_pml(gpx_inv *v, int direction, int xs, int xf, int ys, int yf, int zs, int zf,
+                    const float *eR, const float *hR);
+/* Per-cell coefficients, float32 [nx][ny][nz] each. */
+int gpx_inv_set_model(gpx_inv *v, const float *CA, const float *CB, const float *IEA);
+/* All iterations from the zero state with a z-directed Hertzian dipole on
+ * the Ez edge (si, sj, sk), Ez -= IEA * waves[n] * sfac; receivers rx int32
+ * [nrx][3]; traces float32 [nrx][iterations] (Ez at the start of each
+ * iteration).  Keeps a checkpoint of the state every K iterations. */
+int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, const float *waves, int nrx,
+                    const int *rx, float *traces);
+/* Adjoint of the last forward: resid float32 [nrx][iterations] = dL/d(trace);
+ * outputs float32 [nx][ny][nz].  Recomputes each checkpoint segment. */
+int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, float *gCB, float *gIEA);
+/* Device time of the last forward and of the last backward (recompute
+ * included), ms. */
+int gpx_inv_timings(const gpx_inv *v, float *ms_forward, float *ms_backward);
+
 #ifdef __cplusplus
 }
 #endif

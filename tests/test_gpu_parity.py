@@ -274,3 +274,58 @@ def test_hip_rejects_bad_input():
     G.ID[0, 0, 0, 0] = len(G.updatecoeffsE) + 5
     with pytest.raises(GeneralError, match='material index'):
         _hip()(G)
+
+
+def _single_runs(models, n):
+    out = []
+    for G in models:
+        with _hip()(G) as s:
+            s.run(0, n)
+            out.append((s.download_fields(), s.download_rx()))
+    return out
+
+
+def _batch_models(iterations):
+    shapes = [WIDE, (26, 22, 19), (30, 17, 25)]
+    return [rich_model(seed=30 + n, shape=sh, iterations=iterations, snapshots=False, poles=1, order=1)
+            for n, sh in enumerate(shapes)]
+
+
+def test_batch_matches_single_runs():
+    """Three models of different sizes advanced in one launch per iteration,
+    first eagerly and then as one captured HIP graph: every model ends in the
+    state and with the receiver traces of its own run."""
+    from gprmax_amd.hip import HipBatch
+    models = _batch_models(20)
+    ref = _single_runs(models, 20)
+    solvers = [_hip()(G) for G in models]
+    try:
+        with HipBatch(solvers) as b:
+            b.run(0, 7, graph=False)
+            b.run(7, 13, graph=True)
+        for s, (F, rx) in zip(solvers, ref):
+            assert np.array_equal(s.download_fields(), F)
+            assert np.array_equal(s.download_rx(), rx)
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_batch_autotuner_picks_a_shape_and_resets_the_state():
+    from gprmax_amd.hip import HipBatch
+    models = _batch_models(16)
+    ref = _single_runs(models, 16)
+    solvers = [_hip()(G) for G in models]
+    try:
+        with HipBatch(solvers) as b:
+            tile, xchunk, ms = b.tune(steps=2)
+            assert tile in (512, 1024, 2048) and xchunk >= 2 and ms > 0
+            for s in solvers:
+                assert not s.download_fields().any() and not s.download_rx().any()
+            b.run(0, 16, graph=True)
+        for s, (F, rx) in zip(solvers, ref):
+            assert np.array_equal(s.download_fields(), F)
+            assert np.array_equal(s.download_rx(), rx)
+    finally:
+        for s in solvers:
+            s.close()

@@ -1,0 +1,129 @@
+"""Gradients of a trace misfit with respect to per-cell permittivity and
+conductivity: the GPU adjoint (gprmax_amd.inversion) against a float64
+pure-PyTorch autograd reference of the same discrete forward
+(tests/torch_fdtd.py) and against central finite differences.
+
+The reference itself is pinned first: its autograd gradient equals central
+differences of its own float64 forward (CPU test)."""
+
+import numpy as np
+import pytest
+import torch
+
+from gprmax_amd.inversion import InversionGeometry
+from torch_fdtd import forward_traces, misfit
+
+SHAPE = (16, 15, 14)
+PML = 4
+
+
+def _setup(seed=0, iterations=70, freq=4e9):
+    rng = np.random.default_rng(seed)
+    geo = InversionGeometry(SHAPE, 2e-3, iterations, pml=PML, er_pml=7.0, K=16)
+    waves = geo.waveform(freq=freq)
+    er_true = 4 + 6 * rng.random(SHAPE)
+    se_true = 1e-3 + 1.9e-2 * rng.random(SHAPE)
+    sources = [(7, 7, 6), (9, 6, 8)]
+    receivers = [(10, 7, 6), (5, 8, 7), (8, 10, 9), (6, 5, 8)]
+    er0 = 7.0 + 0.3 * rng.standard_normal(SHAPE)
+    se0 = 1.05e-2 + 1e-3 * rng.standard_normal(SHAPE)
+    return geo, waves, er_true, se_true, er0, se0, sources, receivers
+
+
+def _observed(geo, waves, er, se, sources, receivers):
+    with torch.no_grad():
+        return [forward_traces(torch.tensor(er), torch.tensor(se), geo, s, receivers, waves).numpy()
+                for s in sources]
+
+
+def _autograd(geo, waves, er0, se0, sources, receivers, obs):
+    er = torch.tensor(er0, requires_grad=True)
+    se = torch.tensor(se0, requires_grad=True)
+    L = misfit(er, se, geo, sources, receivers, obs, waves)
+    L.backward()
+    return L.item(), er.grad.numpy(), se.grad.numpy()
+
+
+def _fd(geo, waves, er0, se0, sources, receivers, obs, cell, which, rel=1e-4):
+    base = (er0 if which == 'er' else se0)
+    h = rel * abs(base[cell])
+
+    def loss(delta):
+        e, s = er0.copy(), se0.copy()
+        (e if which == 'er' else s)[cell] += delta
+        with torch.no_grad():
+            return misfit(torch.tensor(e), torch.tensor(s), geo, sources, receivers, obs, waves).item()
+    return (loss(h) - loss(-h)) / (2 * h)
+
+
+def test_torch_reference_gradient_matches_finite_differences():
+    geo, waves, er_t, se_t, er0, se0, src, rx = _setup(iterations=40)
+    src, rx = src[:1], rx[:2]
+    obs = _observed(geo, waves, er_t, se_t, src, rx)
+    L, ger, gse = _autograd(geo, waves, er0, se0, src, rx, obs)
+    assert L > 0
+    for cell in [(7, 7, 7), (9, 6, 6), (6, 8, 8)]:
+        assert abs(_fd(geo, waves, er0, se0, src, rx, obs, cell, 'er') - ger[cell]) < 1e-6 * abs(ger).max()
+        assert abs(_fd(geo, waves, er0, se0, src, rx, obs, cell, 'se') - gse[cell]) < 1e-6 * abs(gse).max()
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_float64_reference():
+    from gprmax_amd.inversion import InversionGrid
+    geo, waves, er_t, se_t, er0, se0, src, rx = _setup()
+    grid = InversionGrid(SHAPE, 2e-3, geo.iterations, pml=PML, er_pml=7.0, K=16)
+    try:
+        grid.set_model(er_t, se_t)
+        for s in src:
+            ours = grid.forward(s, rx, waves).astype(np.float64)
+            with torch.no_grad():
+                ref = forward_traces(torch.tensor(er_t), torch.tensor(se_t), geo, s, rx, waves).numpy()
+            assert np.abs(ours - ref).max() <= 1e-4 * np.abs(ref).max()
+    finally:
+        grid.close()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_float64_autograd_and_finite_differences():
+    """Checkpoint interval 16 over 70 iterations (the last segment partial),
+    two sources, four receivers, PML on every face."""
+    from gprmax_amd.inversion import InversionGrid, misfit_and_gradient
+    geo, waves, er_t, se_t, er0, se0, src, rx = _setup()
+    obs = _observed(geo, waves, er_t, se_t, src, rx)
+    L_ref, ger_ref, gse_ref = _autograd(geo, waves, er0, se0, src, rx, obs)
+    grid = InversionGrid(SHAPE, 2e-3, geo.iterations, pml=PML, er_pml=7.0, K=16)
+    try:
+        L, ger, gse = misfit_and_gradient(grid, er0, se0, src, rx, obs, waves)
+    finally:
+        grid.close()
+    m = geo.interior
+    assert abs(L - L_ref) <= 1e-4 * L_ref
+    for ours, ref in ((ger, ger_ref), (gse, gse_ref)):
+        err = np.linalg.norm(ours[m] - ref[m]) / np.linalg.norm(ref[m])
+        assert err < 1e-3, err
+    rng = np.random.default_rng(4)
+    idx = np.argwhere(m)
+    for n in rng.choice(len(idx), 6, replace=False):
+        cell = tuple(idx[n])
+        for which, g in (('er', ger), ('se', gse)):
+            fd = _fd(geo, waves, er0, se0, src, rx, obs, cell, which)
+            assert abs(g[cell] - fd) <= 1e-2 * abs(fd) + 1e-3 * np.abs(g[m]).max(), (cell, which, g[cell], fd)
+
+
+@pytest.mark.gpu
+def test_misfit_autograd_function():
+    from gprmax_amd.inversion import InversionGrid, misfit as gpu_misfit
+    geo, waves, er_t, se_t, er0, se0, src, rx = _setup(iterations=40)
+    obs = _observed(geo, waves, er_t, se_t, src, rx)
+    grid = InversionGrid(SHAPE, 2e-3, geo.iterations, pml=PML, er_pml=7.0, K=16)
+    try:
+        er = torch.tensor(er0, requires_grad=True)
+        se = torch.tensor(se0, requires_grad=True)
+        L = gpu_misfit(er, se, grid, src, rx, obs, waves)
+        (2.0 * L).backward()
+    finally:
+        grid.close()
+    _, ger_ref, gse_ref = _autograd(geo, waves, er0, se0, src, rx, obs)
+    m = geo.interior
+    assert np.linalg.norm(er.grad.numpy()[m] - 2 * ger_ref[m]) < 1e-3 * np.linalg.norm(2 * ger_ref[m])
+    assert np.linalg.norm(se.grad.numpy()[m] - 2 * gse_ref[m]) < 1e-3 * np.linalg.norm(2 * gse_ref[m])
