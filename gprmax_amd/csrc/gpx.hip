@@ -987,7 +987,7 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
     if (!ctx || (fused != 0 && fused != 1) ||
-        !(tile_cells == 0 || tile_cells == 512 || tile_cells == 1024 || tile_cells == 2048))
+        !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
     ctx->tile_cells = tile_cells;
@@ -1334,7 +1334,7 @@ static int step_tile(const gpx_ctx *ctx)
     auto ok = [&](int tc) { return step_lds(ctx, tc) <= (size_t)STEP_LDS_MAX && 2 * NZ1 <= tc; };
     if (ctx->tile_cells) return ok(ctx->tile_cells) ? ctx->tile_cells : 0;
     int best = 0;
-    for (int tc : {512, 1024, 2048}) {
+    for (int tc : {256, 512, 1024}) {
         if (!ok(tc)) continue;
         best = tc;
         if (8 * NZ1 <= tc) break;
@@ -1809,7 +1809,7 @@ extern "C" const char *gpx_batch_last_error(const gpx_batch *b)
 static std::vector<int> batch_tiles(const gpx_batch *b)
 {
     std::vector<int> out;
-    for (int tc : {512, 1024, 2048}) {
+    for (int tc : {256, 512, 1024}) {
         bool ok = true;
         for (const gpx_ctx *c : b->ctx) {
             const int NZ1 = c->desc.nz + 1;

@@ -519,6 +519,9 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
                 m0[u] = cur[u].m0; m1[u] = cur[u].m1; m2[u] = cur[u].m2;
                 m3[u] = cur[u].m3; m4[u] = cur[u].m4; m5[u] = cur[u].m5;
             }
+            // The next plane's loads fly during this whole plane: both updates
+            // and both barriers.
+            if (i + 1 < xb) load(i + 1, cur);
 
             // ---- H: update_magnetic, H-PML slabs in order, magnetic dipoles
 #pragma unroll
@@ -600,8 +603,6 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
                 hy[u] = nhy;
                 hz[u] = nhz;
             }
-            // The next plane's loads fly while the block waits and updates E.
-            if (i + 1 < xb) load(i + 1, cur);
             __syncthreads();
 
             // ---- E: update_electric or dispersive A, E-PML slabs in order,
@@ -785,7 +786,7 @@ typedef void (*step_kernel)(FusedArgs);
 typedef void (*batch_kernel)(BatchArgs);
 
 // Cells per thread of the fused step: a block of B threads holds B x this.
-constexpr int STEP_CELLS_PER_THREAD = 2;
+constexpr int STEP_CELLS_PER_THREAD = 1;
 
 // Fused kernels (one model per launch, and many) for the given
 // material-index width, poles and coefficient kind.

@@ -194,7 +194,7 @@ def test_launch_shapes_do_not_change_results(xchunk, block):
 WIDE = (23, 45, 37)
 
 
-@pytest.mark.parametrize('tile,xchunk', [(1024, 3), (2048, 2), (1024, 0), (512, 5)])
+@pytest.mark.parametrize('tile,xchunk', [(1024, 3), (512, 2), (1024, 0), (256, 5)])
 @pytest.mark.parametrize('case', ['horipml1', 'mripml2', 'debye3', 'lorentz2'])
 def test_fused_step_from_random_state(case, tile, xchunk):
     """Fused iterations from a random state (fields, Phi, T) with several tiles
@@ -319,7 +319,7 @@ def test_batch_autotuner_picks_a_shape_and_resets_the_state():
     try:
         with HipBatch(solvers) as b:
             tile, xchunk, ms = b.tune(steps=2)
-            assert tile in (512, 1024, 2048) and xchunk >= 2 and ms > 0
+            assert tile in (256, 512, 1024) and xchunk >= 2 and ms > 0
             for s in solvers:
                 assert not s.download_fields().any() and not s.download_rx().any()
             b.run(0, 16, graph=True)

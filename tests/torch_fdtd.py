@@ -115,7 +115,7 @@ def forward_traces(er, se, grid, source, receivers, waves):
             axis, cells, nbrs, shape, n = ge
             c = IEA[cells]
             for w, (comp, src, sign) in enumerate(_E_PAIRS[axis]):
-                dd = (H[src][cells] - H[src][nbrs]) / d
+                dd = (nH[src][cells] - nH[src][nbrs]) / d
                 ph = phiE[w]
                 corr = c * ((R['ERA'].reshape(shape) - 1.0) * dd + R['ERB'].reshape(shape) * ph)
                 phiE[w] = R['ERE'].reshape(shape) * ph - R['ERF'].reshape(shape) * dd
