@@ -247,13 +247,14 @@ def main():
         n_prof = min(50, G.iterations - W - K)
         ms4 = s.run_profiled(W + K, n_prof) if n_prof > 0 else (0.0,) * 4
         interior = s.interior_cells()
+        box = s.interior_box()
         mem = s.memory_bytes()
     cells = G.nx * G.ny * G.nz
     mcells = cells * K / (ms * 1e-3) / 1e6
     if fused:
         kernels = {'step': (algorithmic_bytes_per_step(G), ms4[0])}
     else:
-        bh, be = kernel_bytes(G, 1, interior), kernel_bytes(G, 0, interior)
+        bh, be = kernel_bytes(G, 1, box), kernel_bytes(G, 0, box)
         kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
                    'e_interior': (be[0], ms4[2]), 'e_boundary': (be[1], ms4[3])}
     dom = max(kernels, key=lambda k: kernels[k][1])

@@ -16,13 +16,15 @@
 //
 // Iteration n maps the state s^n = (E^n, H^n, PhiE^n, PhiH^n) to s^{n+1}
 // linearly.  Its adjoint, applied from n = N-1 down to 0 to the adjoint state
-// (lE, lH, mE, mH) = dL/ds^{n+1}, is four gather kernels:
-//   k_adj_e_gather : lH += (dE^{n+1}/dH^{n+1})^T lE      (curl and E-PML terms)
-//   k_adj_e_update : gradients from lE and the stored E^n, H^{n+1};
-//                    mE = (dE/dPhiE)^T lE + RE mE;  lE = CA lE
-//   k_adj_h_gather : lE += (dH^{n+1}/dE^n)^T lH         (curl and H-PML terms)
-//   k_adj_h_update : mH = (dH/dPhiH)^T lH + RE mH       (lH = DA lH = lH)
-// then the receiver residual of iteration n is added to lE.  Forward states
+// (lE, lH, mE, mH) = dL/ds^{n+1}, is two gather kernels:
+//   k_adj_e : lH += (dE^{n+1}/dH^{n+1})^T lE  (curl and E-PML terms); gradients
+//             from lE and the stored E^n, H^{n+1}; mE = (dE/dPhiE)^T lE + RE mE;
+//             lE = CA lE
+//   k_adj_h : lE += (dH^{n+1}/dE^n)^T lH      (curl and H-PML terms);
+//             mH = (dH/dPhiH)^T lH + RE mH; lH = DA lH = lH
+// then the receiver residual of iteration n is added to lE.  lE, mE and mH
+// are double-buffered (read one copy, write the other), so each kernel's
+// gathers see the values of the previous sub-step only.  Forward states
 // come from checkpoints every K iterations; each segment is recomputed
 // forward, storing E^n and H^{n+1} for every iteration of it, before the
 // adjoint sweeps back over it.
@@ -54,15 +56,21 @@ struct InvSlab {
     const float *RE;                     // E: RA, RB, RE, RF, each [t]
     const float *RH;                     // H: the same
     int t;
-    float *PE1, *PE2, *PH1, *PH2;        // Phi (forward) or their adjoints
+    float *PE1, *PE2, *PH1, *PH2;        // Phi (forward) or their adjoints (read)
+    float *QE1, *QE2, *QH1, *QH2;        // adjoint: the adjoints written
+    // Points whose adjoint gathers see this slab: the E box and the box one
+    // step down the normal (E), the H box and the box one step up (H).
+    int gE_lo[3], gE_hi[3], gH_lo[3], gH_hi[3];
 };
 
 struct InvArgs {
     int nx, ny, nz;
     long long ntot;
-    float d, DB, cH;                     // 1/d? no: d; DB = dt/(mu0 d); cH = dt/mu0
+    float d, DB, cH;                     // cell size; DB = dt/(mu0 d); cH = dt/mu0
     float *E[3], *H[3];                  // fields (forward) or adjoint fields
-    const float *Es[3], *Hs[3];          // stored forward E^n and H^{n+1} (adjoint)
+    float *En[3];                        // adjoint: lE^n written here (lE^{n+1} is read from E)
+    float *Es[3], *Hs[3];                // forward: where to save E^n and H^{n+1} (or null);
+                                         // adjoint: the saved E^n and H^{n+1}
     const float *CA, *CB, *IEA;          // per cell [nx][ny][nz]
     float *gCA, *gCB, *gIEA;             // gradient accumulators (adjoint)
     int nslab;
@@ -75,15 +83,25 @@ struct InvArgs {
     const int *rxc;                      // [nrx][3]
     float *traces;                       // [nrx][iterations]
     const float *resid;                  // [nrx][iterations], dL/d(trace)
+    // Grid points [fast[0], fast[1]) x [fast[2], fast[3]) x [fast[4], fast[5])
+    // have every neighbour of the adjoint gathers inside the grid and updated,
+    // and no PML term: the adjoint kernels take a short path there.
+    int fast[6];
+    int shell[6][6];                     // [lo x, y, z, hi x, y, z] of the shell boxes
 };
 
-__device__ __forceinline__ long long fidx(const InvArgs &a, int i, int j, int k)
+// Grid offsets are 32-bit: gpx_inv_create requires fewer than 2^31 points.
+__device__ __forceinline__ int fidx(const InvArgs &a, int i, int j, int k)
 {
-    return ((long long)i * (a.ny + 1) + j) * (a.nz + 1) + k;
+    return (i * (a.ny + 1) + j) * (a.nz + 1) + k;
 }
-__device__ __forceinline__ long long cidx(const InvArgs &a, int i, int j, int k)
+__device__ __forceinline__ int cidx(const InvArgs &a, int i, int j, int k)
 {
-    return ((long long)i * a.ny + j) * a.nz + k;
+    return (i * a.ny + j) * a.nz + k;
+}
+__device__ __forceinline__ bool in_box3(const int *lo, const int *hi, int i, int j, int k)
+{
+    return i >= lo[0] && i < hi[0] && j >= lo[1] && j < hi[1] && k >= lo[2] && k < hi[2];
 }
 
 // Reference update ranges (gprMax/fields_updates_ext.pyx): component c of E or H
@@ -113,8 +131,8 @@ __device__ __forceinline__ bool in_h(const InvSlab &s, int i, int j, int k)
 }
 
 // Profile index and Phi index of cell (i, j, k) for the given field's box.
-__device__ __forceinline__ void pidx(const InvSlab &s, bool efield, int i, int j, int k, int &p, long long &q1,
-                                     long long &q2)
+__device__ __forceinline__ void pidx(const InvSlab &s, bool efield, int i, int j, int k, int &p, int &q1,
+                                     int &q2)
 {
     int c[3] = {i - s.org[0], j - s.org[1], k - s.org[2]};
     const int along = s.axis == 0 ? i : (s.axis == 1 ? j : k);
@@ -122,8 +140,8 @@ __device__ __forceinline__ void pidx(const InvSlab &s, bool efield, int i, int j
     p = s.minus ? pend - along : along - pend;
     c[s.axis] = p;
     const int *d1 = efield ? s.dE1 : s.dH1, *d2 = efield ? s.dE2 : s.dH2;
-    q1 = ((long long)c[0] * d1[1] + c[1]) * d1[2] + c[2];
-    q2 = ((long long)c[0] * d2[1] + c[1]) * d2[2] + c[2];
+    q1 = (c[0] * d1[1] + c[1]) * d1[2] + c[2];
+    q2 = (c[0] * d2[1] + c[1]) * d2[2] + c[2];
 }
 
 // E-PML pairs per axis: (corrected E component, source H component, sign) for
@@ -152,12 +170,12 @@ __device__ __forceinline__ int axis_of(const InvSlab &s, int i, int j, int k)
 // H update and H-PML of one iteration; one thread per grid point.
 __global__ void k_inv_h(InvArgs a)
 {
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
-        const long long o = g, oi = fidx(a, i + 1, j, k), oj = fidx(a, i, j + 1, k), ok = o + 1;
+    {
+        // launched over (k blocks, j, i): no index division
+        const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+        if (k > a.nz) return;
+        const int g = fidx(a, i, j, k);
+        const int o = g, oi = fidx(a, i + 1, j, k), oj = fidx(a, i, j + 1, k), ok = o + 1;
         float h[3] = {a.H[0][o], a.H[1][o], a.H[2][o]};
         if (h_upd(a, 0, i, j, k))
             h[0] = 1.0f * h[0] - a.DB * (a.E[2][oj] - a.E[2][o]) + a.DB * (a.E[1][ok] - a.E[1][o]);
@@ -169,11 +187,11 @@ __global__ void k_inv_h(InvArgs a)
             const InvSlab &S = a.slab[s];
             if (!in_h(S, i, j, k)) continue;
             int p;
-            long long q[2];
+            int q[2];
             pidx(S, false, i, j, k, p, q[0], q[1]);
             const float RA01 = S.RH[p] - 1.0f, RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p],
                         RF0 = S.RH[3 * S.t + p];
-            const long long on = S.axis == 0 ? oi : (S.axis == 1 ? oj : ok);
+            const int on = S.axis == 0 ? oi : (S.axis == 1 ? oj : ok);
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
                 h_pair(S.axis, w, comp, src, sign);
@@ -188,6 +206,11 @@ __global__ void k_inv_h(InvArgs a)
         a.H[0][o] = h[0];
         a.H[1][o] = h[1];
         a.H[2][o] = h[2];
+        if (a.Hs[0]) {
+            a.Hs[0][o] = h[0];
+            a.Hs[1][o] = h[1];
+            a.Hs[2][o] = h[2];
+        }
     }
 }
 
@@ -203,16 +226,21 @@ __global__ void k_inv_rx(InvArgs a)
 // E update, E-PML and the Hertzian dipole of one iteration.
 __global__ void k_inv_e(InvArgs a)
 {
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
-        const long long o = g;
-        const long long om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
+    {
+        // launched over (k blocks, j, i): no index division
+        const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+        if (k > a.nz) return;
+        const int g = fidx(a, i, j, k);
+        const int o = g;
+        const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
         float e[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+        if (a.Es[0]) {
+            a.Es[0][o] = e[0];
+            a.Es[1][o] = e[1];
+            a.Es[2][o] = e[2];
+        }
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
-        const long long oc = cell ? cidx(a, i, j, k) : 0;
+        const int oc = cell ? cidx(a, i, j, k) : 0;
         const float CA = cell ? a.CA[oc] : 0.f, CB = cell ? a.CB[oc] : 0.f, IEA = cell ? a.IEA[oc] : 0.f;
         if (e_upd(a, 0, i, j, k))
             e[0] = CA * e[0] + CB * (a.H[2][o] - a.H[2][om[1]]) - CB * (a.H[1][o] - a.H[1][om[2]]);
@@ -224,7 +252,7 @@ __global__ void k_inv_e(InvArgs a)
             const InvSlab &S = a.slab[s];
             if (!in_e(S, i, j, k)) continue;
             int p;
-            long long q[2];
+            int q[2];
             pidx(S, true, i, j, k, p, q[0], q[1]);
             const float RA01 = S.RE[p] - 1.0f, RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p],
                         RF0 = S.RE[3 * S.t + p];
@@ -258,7 +286,7 @@ __device__ __forceinline__ float e_pml_ldd(const InvArgs &a, const InvSlab &S, i
     int comp, src, sign;
     e_pair(S.axis, w, comp, src, sign);
     int p;
-    long long q[2];
+    int q[2];
     pidx(S, true, i, j, k, p, q[0], q[1]);
     const float RA01 = S.RE[p] - 1.0f, RF0 = S.RE[3 * S.t + p];
     const float IEA = (i < a.nx && j < a.ny && k < a.nz) ? a.IEA[cidx(a, i, j, k)] : 0.f;
@@ -273,7 +301,7 @@ __device__ __forceinline__ float h_pml_ldd(const InvArgs &a, const InvSlab &S, i
     int comp, src, sign;
     h_pair(S.axis, w, comp, src, sign);
     int p;
-    long long q[2];
+    int q[2];
     pidx(S, false, i, j, k, p, q[0], q[1]);
     const float RA01 = S.RH[p] - 1.0f, RF0 = S.RH[3 * S.t + p];
     const float lF = a.H[comp][fidx(a, i, j, k)];
@@ -288,25 +316,82 @@ __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k
     return a.CB[cidx(a, i, j, k)] * a.E[c][fidx(a, i, j, k)];
 }
 
-// lH += (dE^{n+1}/dH^{n+1})^T lE.
-__global__ void k_adj_e_gather(InvArgs a)
+// Reverse of the E half of iteration n, one thread per grid point:
+//   lH    += (dE^{n+1}/dH^{n+1})^T lE^{n+1}      (curl and E-PML terms, a gather)
+//   lE^n   = CA lE^{n+1} on updated edges (else unchanged), into En
+//   mE^n   = RE mE^{n+1} + (dE^{n+1}/dPhiE^n)^T lE^{n+1}, into the Q arrays
+//   gradients of the cell from lE^{n+1}, the saved E^n and H^{n+1}.
+// lE^{n+1} and mE^{n+1} are only read, so neighbours see consistent values.
+// FAST = true runs over the points of the fast box only (every lane of a wave
+// takes the short path); FAST = false over all other points.
+template <bool FAST>
+__global__ void k_adj_e(InvArgs a)
 {
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+    {
+        // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
+        int i, j, k;
+        if (FAST) {
+            k = a.fast[4] + blockIdx.x * blockDim.x + threadIdx.x;
+            j = a.fast[2] + blockIdx.y;
+            i = a.fast[0] + blockIdx.z;
+            if (k >= a.fast[5]) return;
+        } else {
+            // the shell around the fast box, as up to six disjoint boxes
+            // (blockIdx.y), each enumerated flat so every lane has a point
+            const int *lo = a.shell[blockIdx.y], *hi = a.shell[blockIdx.y] + 3;
+            const int sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+            const int t = blockIdx.x * blockDim.x + threadIdx.x;
+            if (sx <= 0 || sy <= 0 || sz <= 0 || t >= sx * sy * sz) return;
+            k = lo[2] + t % sz;
+            j = lo[1] + (t / sz) % sy;
+            i = lo[0] + t / (sz * sy);
+        }
+        const int g = fidx(a, i, j, k);
+        if (FAST) {
+            // The same arithmetic with direct offsets: every edge involved is
+            // updated and no PML term exists here.
+            const int o = g;
+            const int NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
+            const int oc = cidx(a, i, j, k), CY = a.nz, CX = a.ny * a.nz;
+            const float *CB = a.CB;
+            auto XI = [&](int c, int off, int coff) { return CB[oc + coff] * a.E[c][o + off]; };
+            const float dh0 = XI(1, 0, 0) - XI(1, 1, 1) - XI(2, 0, 0) + XI(2, NZ1, CY);
+            const float dh1 = XI(2, 0, 0) - XI(2, PL, CX) - XI(0, 0, 0) + XI(0, 1, 1);
+            const float dh2 = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
+            a.H[0][o] += dh0;
+            a.H[1][o] += dh1;
+            a.H[2][o] += dh2;
+            const float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+            const float CA = a.CA[oc];
+            const float curl[3] = {
+                (a.Hs[2][o] - a.Hs[2][o - NZ1]) - (a.Hs[1][o] - a.Hs[1][o - 1]),
+                (a.Hs[0][o] - a.Hs[0][o - 1]) - (a.Hs[2][o] - a.Hs[2][o - PL]),
+                (a.Hs[1][o] - a.Hs[1][o - PL]) - (a.Hs[0][o] - a.Hs[0][o - NZ1])};
+            float gca = 0.f, gcb = 0.f;
+            for (int c = 0; c < 3; ++c) {
+                gca += lE[c] * a.Es[c][o];
+                gcb += lE[c] * curl[c];
+            }
+            a.gCA[oc] += gca;
+            a.gCB[oc] += gcb;
+            if (i == a.si && j == a.sj && k == a.sk) a.gIEA[oc] += -lE[2] * a.waves[a.it] * a.sfac;
+            a.En[0][o] = CA * lE[0];
+            a.En[1][o] = CA * lE[1];
+            a.En[2][o] = CA * lE[2];
+            return;
+        }
         // Ex(i,j,k) = ... + CB (Hz(i,j,k) - Hz(i,j-1,k)) - CB (Hy(i,j,k) - Hy(i,j,k-1))
         // Ey(i,j,k) = ... + CB (Hx(i,j,k) - Hx(i,j,k-1)) - CB (Hz(i,j,k) - Hz(i-1,j,k))
         // Ez(i,j,k) = ... + CB (Hy(i,j,k) - Hy(i-1,j,k)) - CB (Hx(i,j,k) - Hx(i,j-1,k))
-        float dx = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
-        float dy = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
-        float dz = xi(a, 0, i, j, k) - xi(a, 0, i, j + 1, k) - xi(a, 1, i, j, k) + xi(a, 1, i + 1, j, k);
-        float dh[3] = {dx, dy, dz};
+        float dh[3];
+        dh[0] = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
+        dh[1] = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
+        dh[2] = xi(a, 0, i, j, k) - xi(a, 0, i, j + 1, k) - xi(a, 1, i, j, k) + xi(a, 1, i + 1, j, k);
         // E-PML: dd = (S(e) - S(e - axis)) / d uses H at the edge and at edge - axis,
         // so H here feeds the edge here (+) and the edge one step up the axis (-).
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
+            if (!in_box3(S.gE_lo, S.gE_hi, i, j, k)) continue;
             const int ip = i + (S.axis == 0), jp = j + (S.axis == 1), kp = k + (S.axis == 2);
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
@@ -314,59 +399,53 @@ __global__ void k_adj_e_gather(InvArgs a)
                 dh[src] += (e_pml_ldd(a, S, w, i, j, k) - e_pml_ldd(a, S, w, ip, jp, kp)) / a.d;
             }
         }
-        a.H[0][g] += dh[0];
-        a.H[1][g] += dh[1];
-        a.H[2][g] += dh[2];
-    }
-}
+        const int o = g;
+        a.H[0][o] += dh[0];
+        a.H[1][o] += dh[1];
+        a.H[2][o] += dh[2];
 
-// Gradients, mE and lE <- CA lE, from lE = dL/dE^{n+1}.
-__global__ void k_adj_e_update(InvArgs a)
-{
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
-        const bool cell = i < a.nx && j < a.ny && k < a.nz;
-        if (!cell) continue;  // edges outside the cells are never updated
-        const long long o = g, oc = cidx(a, i, j, k);
-        const long long om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
-        const float CA = a.CA[oc], IEA = a.IEA[oc];
-        float gca = 0.f, gcb = 0.f, giea = 0.f;
         float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
-        // E-PML adjoint Phi (before lE changes) and the gradient terms.
-        for (int s = 0; s < a.nslab; ++s) {
-            const InvSlab &S = a.slab[s];
-            if (!in_e(S, i, j, k)) continue;
-            int p;
-            long long q[2];
-            pidx(S, true, i, j, k, p, q[0], q[1]);
-            const float RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p];
-            for (int w = 0; w < 2; ++w) {
-                int comp, src, sign;
-                e_pair(S.axis, w, comp, src, sign);
-                float *mu = w == 0 ? S.PE1 : S.PE2;
-                mu[q[w]] = RE0 * mu[q[w]] + (float)sign * IEA * RB0 * lE[comp];
+        float out[3] = {lE[0], lE[1], lE[2]};
+        const bool cell = i < a.nx && j < a.ny && k < a.nz;
+        if (cell) {  // edges outside the cells are never updated
+            const int oc = cidx(a, i, j, k);
+            const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o,
+                                     k > 0 ? o - 1 : o};
+            const float CA = a.CA[oc], IEA = a.IEA[oc];
+            for (int s = 0; s < a.nslab; ++s) {
+                const InvSlab &S = a.slab[s];
+                if (!in_e(S, i, j, k)) continue;
+                int p;
+                int q[2];
+                pidx(S, true, i, j, k, p, q[0], q[1]);
+                const float RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p];
+                for (int w = 0; w < 2; ++w) {
+                    int comp, src, sign;
+                    e_pair(S.axis, w, comp, src, sign);
+                    const float *mu = w == 0 ? S.PE1 : S.PE2;
+                    float *mo = w == 0 ? S.QE1 : S.QE2;
+                    mo[q[w]] = RE0 * mu[q[w]] + (float)sign * IEA * RB0 * lE[comp];
+                }
             }
+            const float curl[3] = {
+                (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
+                (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
+                (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
+            float gca = 0.f, gcb = 0.f, giea = 0.f;
+            for (int c = 0; c < 3; ++c) {
+                if (!e_upd(a, c, i, j, k)) continue;
+                gca += lE[c] * a.Es[c][o];
+                gcb += lE[c] * curl[c];
+                out[c] = CA * lE[c];
+            }
+            if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
+            a.gCA[oc] += gca;
+            a.gCB[oc] += gcb;
+            if (giea != 0.f) a.gIEA[oc] += giea;
         }
-        const float curl[3] = {
-            (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
-            (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
-            (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
-        for (int c = 0; c < 3; ++c) {
-            if (!e_upd(a, c, i, j, k)) continue;
-            gca += lE[c] * a.Es[c][o];
-            gcb += lE[c] * curl[c];
-            lE[c] = CA * lE[c];
-        }
-        if (i == a.si && j == a.sj && k == a.sk) giea += -a.E[2][o] * a.waves[a.it] * a.sfac;
-        a.gCA[oc] += gca;
-        a.gCB[oc] += gcb;
-        a.gIEA[oc] += giea;
-        a.E[0][o] = lE[0];
-        a.E[1][o] = lE[1];
-        a.E[2][o] = lE[2];
+        a.En[0][o] = out[0];
+        a.En[1][o] = out[1];
+        a.En[2][o] = out[2];
     }
 }
 
@@ -377,25 +456,59 @@ __device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int 
     return a.DB * a.H[c][fidx(a, i, j, k)];
 }
 
-// lE += (dH^{n+1}/dE^n)^T lH.
-__global__ void k_adj_h_gather(InvArgs a)
+// Reverse of the H half of iteration n (lH = dL/dH^{n+1}, complete):
+//   lE^n  += (dH^{n+1}/dE^n)^T lH                  (curl and H-PML terms, a gather)
+//   mH^n   = RE mH^{n+1} + (dH^{n+1}/dPhiH^n)^T lH, into the Q arrays
+// and lH^n = DA lH = lH.
+// FAST = true runs over the points of the fast box only (every lane of a wave
+// takes the short path); FAST = false over all other points.
+template <bool FAST>
+__global__ void k_adj_h(InvArgs a)
 {
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
+    {
+        // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
+        int i, j, k;
+        if (FAST) {
+            k = a.fast[4] + blockIdx.x * blockDim.x + threadIdx.x;
+            j = a.fast[2] + blockIdx.y;
+            i = a.fast[0] + blockIdx.z;
+            if (k >= a.fast[5]) return;
+        } else {
+            // the shell around the fast box, as up to six disjoint boxes
+            // (blockIdx.y), each enumerated flat so every lane has a point
+            const int *lo = a.shell[blockIdx.y], *hi = a.shell[blockIdx.y] + 3;
+            const int sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+            const int t = blockIdx.x * blockDim.x + threadIdx.x;
+            if (sx <= 0 || sy <= 0 || sz <= 0 || t >= sx * sy * sz) return;
+            k = lo[2] + t % sz;
+            j = lo[1] + (t / sz) % sy;
+            i = lo[0] + t / (sz * sy);
+        }
+        const int g = fidx(a, i, j, k);
+        if (FAST) {
+            const int o = g;
+            const int NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
+            auto ETA = [&](int c, int off) { return a.DB * a.H[c][o + off]; };
+            const float de0 = ETA(1, 0) - ETA(1, -1) - ETA(2, 0) + ETA(2, -NZ1);
+            const float de1 = ETA(2, 0) - ETA(2, -PL) - ETA(0, 0) + ETA(0, -1);
+            const float de2 = ETA(0, 0) - ETA(0, -NZ1) - ETA(1, 0) + ETA(1, -PL);
+            a.E[0][o] += de0;
+            a.E[1][o] += de1;
+            a.E[2][o] += de2;
+            return;
+        }
         // Hx(i,j,k) = Hx - DB (Ez(i,j+1,k) - Ez(i,j,k)) + DB (Ey(i,j,k+1) - Ey(i,j,k))
         // Hy(i,j,k) = Hy - DB (Ex(i,j,k+1) - Ex(i,j,k)) + DB (Ez(i+1,j,k) - Ez(i,j,k))
         // Hz(i,j,k) = Hz - DB (Ey(i+1,j,k) - Ey(i,j,k)) + DB (Ex(i,j+1,k) - Ex(i,j,k))
-        float dx = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
-        float dy = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
-        float dz = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
-        float de[3] = {dx, dy, dz};
+        float de[3];
+        de[0] = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
+        de[1] = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
+        de[2] = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
         // H-PML: dd = (S(h + axis) - S(h)) / d, so E here feeds the cell here (-)
         // and the cell one step down the axis (+).
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
+            if (!in_box3(S.gH_lo, S.gH_hi, i, j, k)) continue;
             const int im = i - (S.axis == 0), jm = j - (S.axis == 1), km = k - (S.axis == 2);
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
@@ -406,29 +519,19 @@ __global__ void k_adj_h_gather(InvArgs a)
         a.E[0][g] += de[0];
         a.E[1][g] += de[1];
         a.E[2][g] += de[2];
-    }
-}
-
-// mH from lH = dL/dH^{n+1}; lH itself is unchanged (DA = 1).
-__global__ void k_adj_h_update(InvArgs a)
-{
-    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < a.ntot;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % (a.nz + 1));
-        const int j = (int)((g / (a.nz + 1)) % (a.ny + 1));
-        const int i = (int)(g / ((long long)(a.nz + 1) * (a.ny + 1)));
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
             if (!in_h(S, i, j, k)) continue;
             int p;
-            long long q[2];
+            int q[2];
             pidx(S, false, i, j, k, p, q[0], q[1]);
             const float RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p];
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
                 h_pair(S.axis, w, comp, src, sign);
-                float *mu = w == 0 ? S.PH1 : S.PH2;
-                mu[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * a.H[comp][g];
+                const float *mu = w == 0 ? S.PH1 : S.PH2;
+                float *mo = w == 0 ? S.QH1 : S.QH2;
+                mo[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * a.H[comp][g];
             }
         }
     }
@@ -452,6 +555,7 @@ struct gpx_inv {
     hipStream_t stream = nullptr;
     float *F = nullptr;            // E, H: 6 x ntot
     float *L = nullptr;            // adjoint lE, lH: 6 x ntot
+    float *L2 = nullptr;           // the second copy of lE: 3 x ntot
     float *ck = nullptr;           // checkpoints: nck x (6 ntot + phi)
     float *seg = nullptr;          // segment store: K x 6 x ntot
     float *par = nullptr;          // CA, CB, IEA: 3 x ncell
@@ -467,10 +571,14 @@ struct gpx_inv {
         long long n[4] = {0, 0, 0, 0};   // Phi sizes
         float *phi[4] = {nullptr, nullptr, nullptr, nullptr};
         float *mu[4] = {nullptr, nullptr, nullptr, nullptr};
+        float *mu2[4] = {nullptr, nullptr, nullptr, nullptr};  // second copies
         float *R = nullptr;              // E then H tables, 8 t floats
     };
     std::vector<Slab> slabs;
     long long phi_total = 0;
+    int fast[6] = {0, 0, 0, 0, 0, 0};  // see InvArgs
+    int shell[6][6] = {};              // see InvArgs
+    int shell_max = 0;                 // points of the largest shell box
     float ms_fwd = 0, ms_bwd = 0;
     std::string err;
 };
@@ -495,12 +603,55 @@ extern "C" const char *gpx_inv_last_error(const gpx_inv *v)
     return v ? v->err.c_str() : g_inv_err.c_str();
 }
 
+// The box of grid points where the adjoint kernels need no range check and
+// no PML term: points 1..n-2 on every axis, shrunk past every slab's E box
+// and the edges one below it, and its H box and the cells one above it.
+static void update_fast(gpx_inv *v)
+{
+    const int n[3] = {v->desc.nx, v->desc.ny, v->desc.nz};
+    int lo[3] = {1, 1, 1}, hi[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
+    for (const auto &S : v->slabs) {
+        const InvSlab &s = S.s;
+        const int ax = s.axis;
+        if (s.minus) {
+            const int end = s.hiE[ax] > s.hiH[ax] + 1 ? s.hiE[ax] : s.hiH[ax] + 1;
+            if (end > lo[ax]) lo[ax] = end;
+        } else {
+            const int start = s.loE[ax] - 1 < s.loH[ax] ? s.loE[ax] - 1 : s.loH[ax];
+            if (start < hi[ax]) hi[ax] = start;
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        v->fast[2 * q] = lo[q];
+        v->fast[2 * q + 1] = hi[q] > lo[q] ? hi[q] : lo[q];
+    }
+    // The grid points [0, n] minus the fast box: below / above it in x (all
+    // y, z), then in y within its x range (all z), then in z within its x
+    // and y ranges.
+    const int N[3] = {n[0] + 1, n[1] + 1, n[2] + 1};
+    const int *f = v->fast;
+    const int boxes[6][6] = {
+        {0, 0, 0, f[0], N[1], N[2]},          {f[1], 0, 0, N[0], N[1], N[2]},
+        {f[0], 0, 0, f[1], f[2], N[2]},       {f[0], f[3], 0, f[1], N[1], N[2]},
+        {f[0], f[2], 0, f[1], f[3], f[4]},    {f[0], f[2], f[5], f[1], f[3], N[2]}};
+    memcpy(v->shell, boxes, sizeof(boxes));
+    v->shell_max = 0;
+    for (auto &b : boxes) {
+        const long long vol = (long long)(b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+        if (vol > v->shell_max) v->shell_max = (int)vol;
+    }
+}
+
 extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
 {
     if (!d || !out) return ifail(nullptr, GPX_EARG, "gpx_inv_create: null argument");
     *out = nullptr;
     if (d->nx < 2 || d->ny < 2 || d->nz < 2 || d->iterations < 1 || d->K < 1 || !(d->d > 0) || !(d->dt > 0))
         return ifail(nullptr, GPX_EARG, "gpx_inv_create: bad descriptor");
+    if (d->nx + 1 > 65535 || d->ny + 1 > 65535)
+        return ifail(nullptr, GPX_ELIMIT, "gpx_inv_create: at most 65534 cells along x and y");
+    if ((double)(d->nx + 1) * (d->ny + 1) * (d->nz + 1) >= 2147483647.0)
+        return ifail(nullptr, GPX_ELIMIT, "gpx_inv_create: at most 2^31 grid points");
     if (hipSetDevice(d->device) != hipSuccess) return ifail(nullptr, GPX_EARG, "gpx_inv_create: bad device");
     gpx_inv *v = new gpx_inv();
     v->desc = *d;
@@ -509,6 +660,7 @@ extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
     hipError_t e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&v->F, 6 * v->ntot * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->L, 6 * v->ntot * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->L2, 3 * v->ntot * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->seg, (size_t)d->K * 6 * v->ntot * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->par, 3 * v->ncell * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->grad, 3 * v->ncell * 4);
@@ -517,6 +669,7 @@ extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
         gpx_inv_destroy(v);
         return ifail(nullptr, (int)e, std::string("gpx_inv_create: ") + hipGetErrorString(e));
     }
+    update_fast(v);
     *out = v;
     return 0;
 }
@@ -526,12 +679,13 @@ extern "C" int gpx_inv_destroy(gpx_inv *v)
     if (!v) return 0;
     hipSetDevice(v->desc.device);
     if (v->stream) hipStreamSynchronize(v->stream);
-    for (float *p : {v->F, v->L, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid})
+    for (float *p : {v->F, v->L, v->L2, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid})
         if (p) hipFree(p);
     if (v->rxc) hipFree(v->rxc);
     for (auto &S : v->slabs) {
         for (float *p : S.phi) if (p) hipFree(p);
         for (float *p : S.mu) if (p) hipFree(p);
+        for (float *p : S.mu2) if (p) hipFree(p);
         if (S.R) hipFree(S.R);
     }
     if (v->stream) hipStreamDestroy(v->stream);
@@ -562,7 +716,8 @@ extern "C" int gpx_inv_add_pml(gpx_inv *v, int dir, int xs, int xf, int ys, int 
         s.hiE[n] = s.hiH[n] = en[n];
         s.org[n] = st[n];
     }
-    if (minus) {  // E cells f - p, H cells f - (p + 1)
+    if (false) {
+    } else if (minus) {  // E cells f - p, H cells f - (p + 1)
         s.loE[axis] = en[axis] - t + 1;
         s.hiE[axis] = en[axis] + 1;
         s.pendE = en[axis];
@@ -573,6 +728,12 @@ extern "C" int gpx_inv_add_pml(gpx_inv *v, int dir, int xs, int xf, int ys, int 
         s.loE[axis] = s.loH[axis] = st[axis];
         s.hiE[axis] = s.hiH[axis] = st[axis] + t;
         s.pendE = s.pendH = st[axis];
+    }
+    for (int q = 0; q < 3; ++q) {
+        s.gE_lo[q] = s.loE[q] - (q == axis);
+        s.gE_hi[q] = s.hiE[q];
+        s.gH_lo[q] = s.loH[q];
+        s.gH_hi[q] = s.hiH[q] + (q == axis);
     }
     // Phi shapes of gprMax/pml.py:202-219 (order 1)
     const int nx = xf - xs, ny = yf - ys, nz = zf - zs;
@@ -597,6 +758,7 @@ extern "C" int gpx_inv_add_pml(gpx_inv *v, int dir, int xs, int xf, int ys, int 
         S.n[q] = (long long)dims[q][0] * dims[q][1] * dims[q][2];
         INV_TRY(hipMalloc((void **)&S.phi[q], S.n[q] * 4));
         INV_TRY(hipMalloc((void **)&S.mu[q], S.n[q] * 4));
+        INV_TRY(hipMalloc((void **)&S.mu2[q], S.n[q] * 4));
         v->phi_total += S.n[q];
     }
     INV_TRY(hipMalloc((void **)&S.R, (size_t)8 * t * 4));
@@ -605,6 +767,7 @@ extern "C" int gpx_inv_add_pml(gpx_inv *v, int dir, int xs, int xf, int ys, int 
     s.RE = S.R;
     s.RH = S.R + 4 * t;
     v->slabs.push_back(S);
+    update_fast(v);
     if (v->ck) {  // checkpoint layout depends on the Phi sizes
         hipFree(v->ck);
         v->ck = nullptr;
@@ -668,13 +831,28 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     a.rxc = v->rxc;
     a.traces = v->traces;
     a.resid = v->resid;
+    memcpy(a.fast, v->fast, sizeof(a.fast));
+    memcpy(a.shell, v->shell, sizeof(a.shell));
     return a;
 }
 
-static int grid_for(const gpx_inv *v)
+// Launch shapes: 128 threads along k, one block row per (j, i).
+static dim3 grid_all(const gpx_inv *v)
 {
-    const long long b = (v->ntot + 255) / 256;
-    return (int)(b < 65536 ? b : 65536);
+    return dim3((v->desc.nz + 1 + 127) / 128, v->desc.ny + 1, v->desc.nx + 1);
+}
+static dim3 grid_fast(const gpx_inv *v)
+{
+    const int fx = v->fast[1] - v->fast[0], fy = v->fast[3] - v->fast[2], fz = v->fast[5] - v->fast[4];
+    return dim3(fz > 0 ? (fz + 127) / 128 : 1, fy > 0 ? fy : 1, fx > 0 ? fx : 1);
+}
+static dim3 grid_shell(const gpx_inv *v)
+{
+    return dim3(v->shell_max > 0 ? (v->shell_max + 127) / 128 : 1, 6, 1);
+}
+static bool fast_empty(const gpx_inv *v)
+{
+    return v->fast[1] <= v->fast[0] || v->fast[3] <= v->fast[2] || v->fast[5] <= v->fast[4];
 }
 
 // Copies the state (fields and Phi) to or from checkpoint c.
@@ -699,8 +877,8 @@ static void forward_step(gpx_inv *v, int it, bool rx)
 {
     InvArgs a = inv_args(v, false, it);
     if (rx && v->nrx) hipLaunchKernelGGL(k_inv_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
-    hipLaunchKernelGGL(k_inv_h, dim3(grid_for(v)), dim3(256), 0, v->stream, a);
-    hipLaunchKernelGGL(k_inv_e, dim3(grid_for(v)), dim3(256), 0, v->stream, a);
+    hipLaunchKernelGGL(k_inv_h, grid_all(v), dim3(128), 0, v->stream, a);
+    hipLaunchKernelGGL(k_inv_e, grid_all(v), dim3(128), 0, v->stream, a);
 }
 
 extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, const float *waves, int nrx,
@@ -764,40 +942,59 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     hipSetDevice(g.device);
     if (v->nrx) INV_TRY(hipMemcpy(v->resid, resid, (size_t)v->nrx * g.iterations * 4, hipMemcpyHostToDevice));
     INV_TRY(hipMemsetAsync(v->L, 0, 6 * v->ntot * 4, v->stream));
+    INV_TRY(hipMemsetAsync(v->L2, 0, 3 * v->ntot * 4, v->stream));
     INV_TRY(hipMemsetAsync(v->grad, 0, 3 * v->ncell * 4, v->stream));
     for (auto &S : v->slabs)
-        for (int q = 0; q < 4; ++q) INV_TRY(hipMemsetAsync(S.mu[q], 0, S.n[q] * 4, v->stream));
+        for (int q = 0; q < 4; ++q) {
+            INV_TRY(hipMemsetAsync(S.mu[q], 0, S.n[q] * 4, v->stream));
+            INV_TRY(hipMemsetAsync(S.mu2[q], 0, S.n[q] * 4, v->stream));
+        }
+    float *lE[2] = {v->L, v->L2};  // current and next copy of lE
+    int cur = 0;
     hipEvent_t e0, e1;
     INV_TRY(hipEventCreate(&e0));
     INV_TRY(hipEventCreate(&e1));
     INV_TRY(hipEventRecord(e0, v->stream));
-    const int G = grid_for(v);
+    const dim3 G = grid_all(v), GF = grid_fast(v), GS = grid_shell(v);
+    const bool nofast = fast_empty(v);
     for (int c = n_checkpoints(v) - 1; c >= 0; --c) {
         const int n0 = c * g.K, n1 = n0 + g.K < g.iterations ? n0 + g.K : g.iterations;
         // recompute the segment, storing E^n and H^{n+1}
         if (int rc = checkpoint(v, c, false)) return rc;
         for (int it = n0; it < n1; ++it) {
             float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
-            INV_TRY(hipMemcpyAsync(slot, v->F, 3 * v->ntot * 4, hipMemcpyDeviceToDevice, v->stream));
             InvArgs a = inv_args(v, false, it);
-            hipLaunchKernelGGL(k_inv_h, dim3(G), dim3(256), 0, v->stream, a);
-            INV_TRY(hipMemcpyAsync(slot + 3 * v->ntot, v->F + 3 * v->ntot, 3 * v->ntot * 4,
-                                   hipMemcpyDeviceToDevice, v->stream));
-            hipLaunchKernelGGL(k_inv_e, dim3(G), dim3(256), 0, v->stream, a);
+            for (int q = 0; q < 3; ++q) {  // the kernels save E^n and H^{n+1} here
+                a.Es[q] = slot + q * v->ntot;
+                a.Hs[q] = slot + (3 + q) * v->ntot;
+            }
+            hipLaunchKernelGGL(k_inv_h, G, dim3(128), 0, v->stream, a);
+            hipLaunchKernelGGL(k_inv_e, G, dim3(128), 0, v->stream, a);
         }
         // adjoint sweep over the segment
         for (int it = n1 - 1; it >= n0; --it) {
-            const float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
+            float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
             InvArgs a = inv_args(v, true, it);
             for (int q = 0; q < 3; ++q) {
                 a.Es[q] = slot + q * v->ntot;
                 a.Hs[q] = slot + (3 + q) * v->ntot;
+                a.E[q] = lE[cur] + q * v->ntot;
+                a.En[q] = lE[cur ^ 1] + q * v->ntot;
             }
-            hipLaunchKernelGGL(k_adj_e_gather, dim3(G), dim3(256), 0, v->stream, a);
-            hipLaunchKernelGGL(k_adj_e_update, dim3(G), dim3(256), 0, v->stream, a);
-            hipLaunchKernelGGL(k_adj_h_gather, dim3(G), dim3(256), 0, v->stream, a);
-            hipLaunchKernelGGL(k_adj_h_update, dim3(G), dim3(256), 0, v->stream, a);
+            for (int s = 0; s < a.nslab; ++s) {
+                const auto &S = v->slabs[s];
+                float *const *P = cur == 0 ? S.mu : S.mu2;
+                float *const *Q = cur == 0 ? S.mu2 : S.mu;
+                a.slab[s].PE1 = P[0]; a.slab[s].PE2 = P[1]; a.slab[s].PH1 = P[2]; a.slab[s].PH2 = P[3];
+                a.slab[s].QE1 = Q[0]; a.slab[s].QE2 = Q[1]; a.slab[s].QH1 = Q[2]; a.slab[s].QH2 = Q[3];
+            }
+            if (!nofast) hipLaunchKernelGGL(k_adj_e<true>, GF, dim3(128), 0, v->stream, a);
+            hipLaunchKernelGGL(k_adj_e<false>, GS, dim3(128), 0, v->stream, a);
+            for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
+            if (!nofast) hipLaunchKernelGGL(k_adj_h<true>, GF, dim3(128), 0, v->stream, a);
+            hipLaunchKernelGGL(k_adj_h<false>, GS, dim3(128), 0, v->stream, a);
             if (v->nrx) hipLaunchKernelGGL(k_adj_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
+            cur ^= 1;
         }
         INV_TRY(hipGetLastError());
     }

@@ -1759,6 +1759,13 @@ extern "C" int gpx_device_info(int device, char *name, int namelen, size_t *tota
     return 0;
 }
 
+extern "C" int gpx_interior_box(const gpx_ctx *ctx, int *box6)
+{
+    if (!ctx || !box6) return GPX_EARG;
+    memcpy(box6, ctx->box, sizeof(ctx->box));
+    return 0;
+}
+
 extern "C" long long gpx_interior_cells(const gpx_ctx *ctx)
 {
     if (!ctx) return 0;

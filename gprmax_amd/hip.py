@@ -45,6 +45,7 @@ SIGNATURES = {
     'gpx_run_timed': ([_P, _I, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
     'gpx_run_profiled': ([_P, _I, _I, ctypes.POINTER(_F)], _I),
     'gpx_interior_cells': ([_P], ctypes.c_longlong),
+    'gpx_interior_box': ([_P, ctypes.POINTER(_I)], _I),
     'gpx_part': ([_P, _I, _I, _I], _I),
     'gpx_set_launch': ([_P, _I, _I], _I),
     'gpx_set_step': ([_P, _I, _I], _I),
@@ -197,6 +198,11 @@ class HipSolver:
 
     def interior_cells(self):
         return int(self.lib.gpx_interior_cells(self.ctx))
+
+    def interior_box(self):
+        b = (ctypes.c_int * 6)()
+        self._check(self.lib.gpx_interior_box(self.ctx, b))
+        return tuple(b)
 
     def part(self, name, arg=0, it=0):
         self._check(self.lib.gpx_part(self.ctx, PART[name], arg, it))

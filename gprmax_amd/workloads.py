@@ -159,14 +159,24 @@ def pass_bytes(G, field):
     return b
 
 
-def kernel_bytes(G, field, interior_cells):
+def kernel_bytes(G, field, box):
     """Algorithmic bytes of the (interior, boundary) kernels of one pass.
 
-    The interior kernel updates `interior_cells` cells away from the faces and
-    the PML slabs at 48 B each (plus 24 B x poles for E); the boundary kernel
-    updates the remaining cells and carries all the PML Phi traffic, so the
-    two add up to ``pass_bytes``.
+    The interior kernel updates the cells of `box` ([x0, x1, y0, y1, z0, z1)
+    in field-array indices) at 48 B each (plus 24 B x poles for E) plus the
+    Phi traffic of the PML cells inside the box; the boundary kernel does the
+    rest, so the two add up to ``pass_bytes``.
     """
+    x0, x1, y0, y1, z0, z1 = box
+    lo = (x0, y0, z0)
+    hi = (min(x1, G.nx), min(y1, G.ny), min(z1, G.nz))   # cells of the grid
+    cells = max(0, hi[0] - lo[0]) * max(0, hi[1] - lo[1]) * max(0, hi[2] - lo[2])
+    order = len(G.cfs) if G.pmls else 0
+    pml_in = 0
+    for p in G.pmls:
+        a = (max(lo[0], p.xs), max(lo[1], p.ys), max(lo[2], p.zs))
+        b = (min(hi[0], p.xf), min(hi[1], p.yf), min(hi[2], p.zf))
+        pml_in += max(0, b[0] - a[0]) * max(0, b[1] - a[1]) * max(0, b[2] - a[2])
     per_cell = 48 + (24 * G.maxpoles if field == 0 else 0)
-    inner = per_cell * interior_cells
+    inner = per_cell * cells + 16 * order * pml_in
     return inner, pass_bytes(G, field) - inner

@@ -114,8 +114,10 @@ int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
  * boundary = the rest).  Waits after each iteration. */
 int gpx_run_profiled(gpx_ctx *ctx, int it0, int n, float *ms4);
 
-/* Cells of the interior box (the rest are boundary cells). */
+/* Cells of the interior box (the rest are boundary cells), and the box
+ * itself as [x0, x1, y0, y1, z0, z1), cell indices of the field arrays. */
 long long gpx_interior_cells(const gpx_ctx *ctx);
+int gpx_interior_box(const gpx_ctx *ctx, int *box6);
 
 /* One unfused sub-step (enum gpx_part_id) at iteration it. */
 int gpx_part(gpx_ctx *ctx, int part, int arg, int it);
