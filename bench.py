@@ -209,6 +209,7 @@ def main():
     ap.add_argument('--block', type=int, default=0)
     ap.add_argument('--tile', type=int, default=0, help='cells per fused block (0 = auto)')
     ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
+    ap.add_argument('--overlap', action='store_true', help='H and E passes with interior and boundary kernels side by side')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
@@ -236,7 +237,7 @@ def main():
     info = device_info(0)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
-        s.set_step(1 if args.fused else 0, args.tile)
+        s.set_step(1 if args.fused else (2 if args.overlap else 0), args.tile)
         fused = s.step_kind()
         s.run(0, W)
         s.synchronize()

@@ -606,7 +606,10 @@ struct gpx_ctx {
     int *frame = nullptr;            // plane offsets outside the interior rectangle
     int nframe = 0;
     int xchunk = 0, block = 0;
-    int fused = 0;                   // 1: one fused kernel per step when eligible; 0: two passes
+    int fused = 0;                   // 1: one fused kernel per step when eligible; 0: two passes;
+                                     // 2: two passes, interior and boundary kernels side by side
+    hipStream_t stream2 = nullptr;   // second stream of the side-by-side form
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int tile_cells = 0;              // cells per fused block (0 = auto)
     int cus = 0;                     // compute units of the device
     mutable std::map<std::pair<const void *, size_t>, int> occupancy;
@@ -748,6 +751,10 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->rare) hipFree(ctx->rare);
     if (ctx->frame) hipFree(ctx->frame);
     for (auto &s : ctx->snaps) if (s.out) hipFree(s.out);
+    if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) hipStreamDestroy(ctx->stream2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -986,7 +993,7 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
-    if (!ctx || (fused != 0 && fused != 1) ||
+    if (!ctx || fused < 0 || fused > 2 ||
         !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
@@ -1219,17 +1226,25 @@ static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, si
 
 typedef void (*pass_kernel)(StepArgs);
 
-static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd, int field, int flags, int it)
+// `share` < 1 plans the kernel for that fraction of the resident blocks, so
+// that another kernel can run beside it.
+static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd, int field, int flags, int it,
+                                hipStream_t stream = nullptr, double share = 1.0)
 {
     StepArgs a;
     int block, grid;
     size_t lds;
     base_args(ctx, field, a, block, lds);
     if (count_items(ctx, bnd, block, 1, nullptr) <= 0) return hipSuccess;
-    plan(ctx, bnd, resident_blocks(ctx, (const void *)kernel, block, lds), block, a, grid);
+    int resident = resident_blocks(ctx, (const void *)kernel, block, lds);
+    if (share < 1.0) {
+        resident = (int)(resident * share) / 8 * 8;
+        if (resident < 8) resident = 8;
+    }
+    plan(ctx, bnd, resident, block, a, grid);
     a.flags = flags;
     a.it = it;
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, ctx->stream, a);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream ? stream : ctx->stream, a);
     return hipGetLastError();
 }
 
@@ -1271,6 +1286,32 @@ static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it, 
     if (e != hipSuccess) return e;
     if (ev) e = hipEventRecord(ev[2], ctx->stream);
     return e;
+}
+
+// The same pass with the interior and boundary kernels running side by side:
+// the boundary kernel on the second stream, each planned for its share of
+// the resident blocks (by cell count, boundary cells weighted 2x).
+static hipError_t launch_pass_overlap(gpx_ctx *ctx, int field, int flags, int it)
+{
+    hipError_t e;
+    if (!ctx->stream2) {
+        if ((e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
+    }
+    const double ni = (double)(ctx->box[1] - ctx->box[0]) * (ctx->box[3] - ctx->box[2]) * (ctx->box[5] - ctx->box[4]);
+    const double nb = (double)ctx->ntot - (ni > 0 ? ni : 0);
+    const double share = ni > 0 ? ni / (ni + 2.0 * nb) : 0.0;
+    if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0)) != hipSuccess) return e;
+    if (ni > 0 && flags) {
+        e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it, ctx->stream, share);
+        if (e != hipSuccess) return e;
+    }
+    e = launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it, ctx->stream2, 1.0 - share);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(ctx->ev_join, ctx->stream2)) != hipSuccess) return e;
+    return hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
 }
 
 // ---------------------------------------------------------------- fused step
@@ -1325,7 +1366,7 @@ static size_t step_lds(const gpx_ctx *ctx, int tc) { return step_table_lds(ctx) 
 // then the two-pass kernels run instead.
 static int step_tile(const gpx_ctx *ctx)
 {
-    if (!ctx->fused || slab_order(ctx) < 0) return 0;
+    if (ctx->fused != 1 || slab_order(ctx) < 0) return 0;
     const int NZ1 = ctx->desc.nz + 1;
     // Streamed offsets are 32-bit: a state of six components and the six
     // index arrays must each stay below 4 GiB.
@@ -1517,6 +1558,11 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
                 GPX_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
                 acc[0] += t;
             }
+            continue;
+        }
+        if (ctx->fused == 2 && !ms4) {
+            GPX_TRY(launch_pass_overlap(ctx, 1, hf, it));
+            GPX_TRY(launch_pass_overlap(ctx, 0, ef, it));
             continue;
         }
         GPX_TRY(launch_pass(ctx, 1, hf, it, ms4 ? ev : nullptr));

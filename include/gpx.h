@@ -128,7 +128,9 @@ int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
 
 /* Step form of gpx_run.  fused = 1: one kernel per iteration that does the
  * H and the E update with the state double-buffered on the device; fused = 0
- * (default): an H pass and an E pass per iteration.  tile_cells = cells per
+ * (default): an H pass and an E pass per iteration; fused = 2: the passes
+ * with their interior and boundary kernels running side by side on two
+ * streams (gpx_run_profiled always runs them one after the other).  tile_cells = cells per
  * fused block (512, 1024 or 2048; 0 = auto).  Both forms give identical
  * results; the fused form needs a tile of at least twice nz+1 cells within
  * the LDS budget, and the passes run when it does not fit. */
