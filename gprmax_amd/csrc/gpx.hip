@@ -658,8 +658,13 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
     gpx_ctx *ctx = nullptr;
     if (!d || !out) return fail(nullptr, GPX_EARG, "gpx_create: null argument");
     *out = nullptr;
-    if (d->nx < 2 || d->ny < 2 || d->nz < 2)
-        return fail(nullptr, GPX_EARG, "gpx_create: only 3D grids (nx, ny, nz >= 2) are supported");
+    // One axis of one cell is a 2-D model (TMx, TMy, TMz).  The 3-D update
+    // ranges then give the results of the reference's 2-D branches
+    // (gprMax/fields_updates_ext.pyx:55-77, 376-400): the E components the
+    // 2-D branch skips have empty 3-D ranges, and the H cells only the 3-D
+    // ranges add lie on outer faces whose E never leaves zero.
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1 || (d->nx == 1) + (d->ny == 1) + (d->nz == 1) > 1)
+        return fail(nullptr, GPX_EARG, "gpx_create: a grid needs at least 2 cells on two axes and 1 on the third");
     if (d->iterations < 1 || d->nmat < 1)
         return fail(nullptr, GPX_EARG, "gpx_create: iterations and nmat must be positive");
     if (d->maxpoles < 0 || d->maxpoles > MAX_POLES)

@@ -26,7 +26,7 @@ from .pml import CFS, build_pmls, default_pml_thickness
 from .receivers import Rx
 from .snapshots import Snapshot
 from .sources import HertzianDipole, MagneticDipole, VoltageSource
-from .utilities import GeneralError, courant_dt, iterations_for
+from .utilities import GeneralError, courant_dt, iterations_for, round_value
 from .waveforms import Waveform
 
 
@@ -113,13 +113,16 @@ class ModelBuilder:
         G.dt, G.mode = courant_dt(G.nx, G.ny, G.nz, G.dx, G.dy, G.dz)
         if dt is not None:
             G.dt = dt
-        if G.mode != '3D':
-            raise GeneralError('only 3D models are supported by this solver core (mode {})'.format(G.mode))
         G.iterations, G.timewindow = iterations_for(time_window, G.dt)
         if isinstance(pml_cells, dict):
             G.pmlthickness = OrderedDict((k, int(pml_cells.get(k, 0))) for k in default_pml_thickness())
         else:
+            # A thickness for every face, as without #pml_cells: a 2-D model
+            # has none on its invariant axis (input_cmds_singleuse.py:145-159).
             G.pmlthickness = default_pml_thickness(int(pml_cells))
+            if G.mode != '3D':
+                axis = G.mode[-1]
+                G.pmlthickness[axis + '0'] = G.pmlthickness[axis + 'max'] = 0
         for key, n in G.pmlthickness.items():
             size = {'x': G.nx, 'y': G.ny, 'z': G.nz}[key[0]]
             if 2 * n >= size:
@@ -206,6 +209,66 @@ class ModelBuilder:
             G.ID[4, xs:xf, ys:yf + 1, zs:zf] = n
             G.ID[5, xs:xf, ys:yf, zs:zf + 1] = n
 
+    def cylinder(self, x1, y1, z1, x2, y2, z2, r, material, averaging=True):
+        """#cylinder between face centres (x1, y1, z1) and (x2, y2, z2) in
+        metres, radius r, aligned with a grid axis
+        (gprMax/input_cmds_geometry.py:527-599, build_cylinder in
+        gprMax/geometry_primitives_ext.pyx:692-822).
+
+        The face centres snap to the grid; the builder then works in float32
+        (its arguments are C floats) except the distance test, which is
+        double: cell (i, j) is inside when
+        sqrt((f32(i*dx) + 0.5*dx - x1)^2 + (f32(j*dy) + 0.5*dy - y1)^2) <= r.
+        """
+        G = self.G
+        d = (G.dx, G.dy, G.dz)
+        p1 = [round_value(float(v) / h) * h for v, h in zip((x1, y1, z1), d)]
+        p2 = [round_value(float(v) / h) * h for v, h in zip((x2, y2, z2), d)]
+        if r <= 0:
+            raise GeneralError('the cylinder radius should be a positive value')
+        f = np.float32
+        p1f, p2f, df, rf = [f(v) for v in p1], [f(v) for v in p2], [f(v) for v in d], f(r)
+        snap = [(round_value(float(a / h)), round_value(float(b / h))) for a, b, h in zip(p1f, p2f, df)]
+        if snap[1][0] == snap[1][1] and snap[2][0] == snap[2][1]:
+            axis = 0
+        elif snap[0][0] == snap[0][1] and snap[2][0] == snap[2][1]:
+            axis = 1
+        elif snap[0][0] == snap[0][1] and snap[1][0] == snap[1][1]:
+            axis = 2
+        else:
+            raise GeneralError('only cylinders aligned with a grid axis are supported')
+        bounds = []
+        for n, size in enumerate((G.nx, G.ny, G.nz)):
+            a, b = sorted((p1f[n], p2f[n]))
+            if n == axis:
+                s, e = round_value(float(a / df[n])), round_value(float(b / df[n]))
+            else:
+                s, e = round_value(float((a - rf) / df[n])) - 1, round_value(float((b + rf) / df[n])) + 1
+            bounds.append((max(s, 0), min(e, size)))
+        mask = np.zeros((G.nx, G.ny, G.nz), dtype=bool)
+        if any(e <= s for s, e in bounds):
+            return
+        u, v = [n for n in range(3) if n != axis]
+
+        def centre(n):
+            idx = np.arange(*bounds[n], dtype=np.float32)
+            return (idx * df[n]).astype(np.float64) + 0.5 * np.float64(df[n]) - np.float64(p1f[n])
+
+        cu, cv = centre(u), centre(v)
+        inside = np.sqrt(cu[:, None] ** 2 + cv[None, :] ** 2) <= np.float64(rf)
+        sl = [slice(*b) for b in bounds]
+        shape = [1, 1, 1]
+        shape[u], shape[v] = inside.shape
+        view = mask[tuple(sl)]
+        view |= inside.reshape(shape)
+        averaging = averaging and material.averagable
+        if averaging:
+            G.solid[mask] = material.numID
+            G.rigidcellE[mask] = False
+            G.rigidcellH[mask] = False
+        else:
+            self.paint_voxels(mask, material.numID)
+
     def paint_voxels(self, mask, numids):
         """Rigid (non-averaged) voxels painted in (i, j, k) order with the
         semantics of build_voxel (gprMax/geometry_primitives_ext.pyx:314-376).
@@ -252,6 +315,7 @@ class ModelBuilder:
             G.pmls = []
         build_electric_components(G)
         build_magnetic_components(G)
+        pec_invariant_axis(G)
         for vs in G.voltagesources:
             vs.create_material(G)
         process_materials(G)
@@ -326,6 +390,20 @@ def _edge_build(G, comp, ranges, cell_offsets, rigidcells, electric):
             names_of = [m.ID for m in G.materials]
         names = [names_of[n] for n in ids]
         target[a, b, c] = _average_material(G, names, ids, cache, electric)
+
+
+def pec_invariant_axis(G):
+    """PEC on the two E components across the invariant axis of a 2-D model,
+    at its two grid planes (gprMax/model_build_run.py:222-241)."""
+    planes = {'2D TMx': (0, (1, 2)), '2D TMy': (1, (0, 2)), '2D TMz': (2, (0, 1))}
+    if G.mode not in planes:
+        return
+    axis, comps = planes[G.mode]
+    for c in comps:
+        for plane in (0, 1):
+            idx = [c, slice(None), slice(None), slice(None)]
+            idx[1 + axis] = plane
+            G.ID[tuple(idx)] = 0
 
 
 def build_electric_components(G):

@@ -65,7 +65,9 @@ enum gpx_part_id {
 enum gpx_source_kind { GPX_SRC_HERTZIAN = 0, GPX_SRC_MAGNETIC = 1, GPX_SRC_VOLTAGE = 2 };
 
 /* Context lifetime.  Replaces the device set-up of solve_gpu
- * (gprMax/model_build_run.py:477-583). */
+ * (gprMax/model_build_run.py:477-583).  A grid of one cell along one axis is
+ * a 2-D model (TMx, TMy or TMz, gprMax/input_cmds_singleuse.py:145-159);
+ * more than one such axis is GPX_EARG. */
 int gpx_create(const gpx_desc *desc, gpx_ctx **out);
 int gpx_destroy(gpx_ctx *ctx);
 const char *gpx_last_error(const gpx_ctx *ctx);

@@ -44,7 +44,10 @@ typedef REAL real;
 #define CO(m, c) coeffs[(size_t)(m) * 5 + (c)]
 
 /* ------------------------------------------------------------------------
- * update_electric, 3-D branch (gprMax/fields_updates_ext.pyx:30-107)
+ * update_electric (gprMax/fields_updates_ext.pyx:30-107).  A grid one cell
+ * thick along an axis is a 2-D model (TMx, TMy, TMz) and updates only the
+ * E component along that axis, over the ranges of lines 55-77; otherwise the
+ * 3-D branch of lines 79-107.
  * ---------------------------------------------------------------------- */
 void FN(update_electric)(int nx, int ny, int nz, int nthreads, const real *coeffs,
                          const uint32_t *ID, real *Ex, real *Ey, real *Ez,
@@ -53,6 +56,40 @@ void FN(update_electric)(int nx, int ny, int nz, int nthreads, const real *coeff
     const long NY1 = ny + 1, NZ1 = nz + 1;
     const size_t NTOT = (size_t)(nx + 1) * NY1 * NZ1;
     long i, j, k;
+
+    if (nx == 1) {          /* TMx: Ex, i in [0, nx), j in [1, ny), k in [1, nz) */
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+        for (i = 0; i < nx; i++)
+            for (j = 1; j < ny; j++)
+                for (k = 1; k < nz; k++) {
+                    uint32_t m = ID[IDX(0, i, j, k)];
+                    Ex[F3(i, j, k)] = CO(m, 0) * Ex[F3(i, j, k)] + CO(m, 2) * (Hz[F3(i, j, k)] - Hz[F3(i, j - 1, k)])
+                                    - CO(m, 3) * (Hy[F3(i, j, k)] - Hy[F3(i, j, k - 1)]);
+                }
+        return;
+    }
+    if (ny == 1) {          /* TMy: Ey, i in [1, nx), j in [0, ny), k in [1, nz) */
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+        for (i = 1; i < nx; i++)
+            for (j = 0; j < ny; j++)
+                for (k = 1; k < nz; k++) {
+                    uint32_t m = ID[IDX(1, i, j, k)];
+                    Ey[F3(i, j, k)] = CO(m, 0) * Ey[F3(i, j, k)] + CO(m, 3) * (Hx[F3(i, j, k)] - Hx[F3(i, j, k - 1)])
+                                    - CO(m, 1) * (Hz[F3(i, j, k)] - Hz[F3(i - 1, j, k)]);
+                }
+        return;
+    }
+    if (nz == 1) {          /* TMz: Ez, i in [1, nx), j in [1, ny), k in [0, nz) */
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+        for (i = 1; i < nx; i++)
+            for (j = 1; j < ny; j++)
+                for (k = 0; k < nz; k++) {
+                    uint32_t m = ID[IDX(2, i, j, k)];
+                    Ez[F3(i, j, k)] = CO(m, 0) * Ez[F3(i, j, k)] + CO(m, 1) * (Hy[F3(i, j, k)] - Hy[F3(i - 1, j, k)])
+                                    - CO(m, 2) * (Hx[F3(i, j, k)] - Hx[F3(i, j - 1, k)]);
+                }
+        return;
+    }
 
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 1; i < nx; i++)
@@ -94,7 +131,10 @@ void FN(update_electric)(int nx, int ny, int nz, int nthreads, const real *coeff
 }
 
 /* ------------------------------------------------------------------------
- * update_magnetic, 3-D branch (gprMax/fields_updates_ext.pyx:352-412)
+ * update_magnetic (gprMax/fields_updates_ext.pyx:352-412).  A 2-D model
+ * updates Hx when ny or nz is 1, Hy when nx or nz is 1 and Hz when nx or ny
+ * is 1, each with the coefficient and field at the same (i, j, k) and the
+ * ranges of lines 376-400; otherwise the 3-D branch (lines 402-412).
  * ---------------------------------------------------------------------- */
 void FN(update_magnetic)(int nx, int ny, int nz, int nthreads, const real *coeffs,
                          const uint32_t *ID, const real *Ex, const real *Ey, const real *Ez,
@@ -103,6 +143,40 @@ void FN(update_magnetic)(int nx, int ny, int nz, int nthreads, const real *coeff
     const long NY1 = ny + 1, NZ1 = nz + 1;
     const size_t NTOT = (size_t)(nx + 1) * NY1 * NZ1;
     long i, j, k;
+
+    if (nx == 1 || ny == 1 || nz == 1) {
+        if (ny == 1 || nz == 1) {
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+            for (i = 1; i < nx; i++)
+                for (j = 0; j < ny; j++)
+                    for (k = 0; k < nz; k++) {
+                        uint32_t m = ID[IDX(3, i, j, k)];
+                        Hx[F3(i, j, k)] = CO(m, 0) * Hx[F3(i, j, k)] - CO(m, 2) * (Ez[F3(i, j + 1, k)] - Ez[F3(i, j, k)])
+                                        + CO(m, 3) * (Ey[F3(i, j, k + 1)] - Ey[F3(i, j, k)]);
+                    }
+        }
+        if (nx == 1 || nz == 1) {
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+            for (i = 0; i < nx; i++)
+                for (j = 1; j < ny; j++)
+                    for (k = 0; k < nz; k++) {
+                        uint32_t m = ID[IDX(4, i, j, k)];
+                        Hy[F3(i, j, k)] = CO(m, 0) * Hy[F3(i, j, k)] - CO(m, 3) * (Ex[F3(i, j, k + 1)] - Ex[F3(i, j, k)])
+                                        + CO(m, 1) * (Ez[F3(i + 1, j, k)] - Ez[F3(i, j, k)]);
+                    }
+        }
+        if (nx == 1 || ny == 1) {
+#pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
+            for (i = 0; i < nx; i++)
+                for (j = 0; j < ny; j++)
+                    for (k = 1; k < nz; k++) {
+                        uint32_t m = ID[IDX(5, i, j, k)];
+                        Hz[F3(i, j, k)] = CO(m, 0) * Hz[F3(i, j, k)] - CO(m, 1) * (Ey[F3(i + 1, j, k)] - Ey[F3(i, j, k)])
+                                        + CO(m, 2) * (Ex[F3(i, j + 1, k)] - Ex[F3(i, j, k)]);
+                    }
+        }
+        return;
+    }
 
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 0; i < nx; i++)
@@ -162,7 +236,10 @@ void FN(update_electric_dispersive_A)(int nx, int ny, int nz, int nthreads, int 
     const long NY1 = ny + 1, NZ1 = nz + 1;
     const size_t NTOT = (size_t)(nx + 1) * NY1 * NZ1;
     long i, j, k;
+    /* Each component is skipped when its two transverse axes are both one
+     * cell (fields_updates_ext.pyx:146, 158, 170). */
     /* Ex: i in [0, nx), j, k in [1, n) */
+    if (ny != 1 || nz != 1) {
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 0; i < nx; i++)
         for (j = 1; j < ny; j++)
@@ -173,7 +250,9 @@ void FN(update_electric_dispersive_A)(int nx, int ny, int nz, int nthreads, int 
                 Ex[f] = CO(m, 0) * Ex[f] + CO(m, 2) * (Hz[f] - Hz[F3(i, j - 1, k)])
                       - CO(m, 3) * (Hy[f] - Hy[F3(i, j, k - 1)]) - CO(m, 4) * phi;
             }
+    }
     /* Ey: i in [1, nx), j in [0, ny), k in [1, nz) */
+    if (nx != 1 || nz != 1) {
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 1; i < nx; i++)
         for (j = 0; j < ny; j++)
@@ -184,7 +263,9 @@ void FN(update_electric_dispersive_A)(int nx, int ny, int nz, int nthreads, int 
                 Ey[f] = CO(m, 0) * Ey[f] + CO(m, 3) * (Hx[f] - Hx[F3(i, j, k - 1)])
                       - CO(m, 1) * (Hz[f] - Hz[F3(i - 1, j, k)]) - CO(m, 4) * phi;
             }
+    }
     /* Ez: i, j in [1, n), k in [0, nz) */
+    if (nx != 1 || ny != 1) {
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 1; i < nx; i++)
         for (j = 1; j < ny; j++)
@@ -195,6 +276,7 @@ void FN(update_electric_dispersive_A)(int nx, int ny, int nz, int nthreads, int 
                 Ez[f] = CO(m, 0) * Ez[f] + CO(m, 1) * (Hy[f] - Hy[F3(i - 1, j, k)])
                       - CO(m, 2) * (Hx[f] - Hx[F3(i, j - 1, k)]) - CO(m, 4) * phi;
             }
+    }
 }
 
 static inline void disp_B_cell(int P, const real *disp, uint32_t m, real *T, size_t NTOT, size_t f, real E)
@@ -213,16 +295,20 @@ void FN(update_electric_dispersive_B)(int nx, int ny, int nz, int nthreads, int 
     const long NY1 = ny + 1, NZ1 = nz + 1;
     const size_t NTOT = (size_t)(nx + 1) * NY1 * NZ1;
     long i, j, k;
+    /* Guards of fields_updates_ext.pyx:211, 220, 229 */
+    if (ny != 1 || nz != 1)
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 0; i < nx; i++)
         for (j = 1; j < ny; j++)
             for (k = 1; k < nz; k++)
                 disp_B_cell(P, disp, ID[IDX(0, i, j, k)], Tx, NTOT, F3(i, j, k), Ex[F3(i, j, k)]);
+    if (nx != 1 || nz != 1)
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 1; i < nx; i++)
         for (j = 0; j < ny; j++)
             for (k = 1; k < nz; k++)
                 disp_B_cell(P, disp, ID[IDX(1, i, j, k)], Ty, NTOT, F3(i, j, k), Ey[F3(i, j, k)]);
+    if (nx != 1 || ny != 1)
 #pragma omp parallel for schedule(static) num_threads(nthreads) private(j, k)
     for (i = 1; i < nx; i++)
         for (j = 1; j < ny; j++)

@@ -55,3 +55,64 @@ def rich_model(seed=0, shape=(26, 22, 19), iterations=40, pml=4, order=1, formul
         bld.add_snapshot(0, 0, 0, nx, ny, nz, 1, 1, 1, iterations // 2)
         bld.add_snapshot(2, 3, 1, 2 + 2 * 5, 3 + 3 * 4, 1 + 2 * 6, 2, 3, 2, iterations - 1)
     return bld.finalise()
+
+
+def plane_model(seed=0, axis=2, size=(30, 24), iterations=40, pml=4, order=1, formulation='HORIPML',
+                poles=0, kind='debye', nmat=7, snapshots=True):
+    """A 2-D (one cell along `axis`) companion of rich_model: random materials,
+    a PEC block, PML on the four in-plane faces, Hertzian, voltage and
+    magnetic sources in the plane, receivers and snapshots."""
+    rng = np.random.default_rng(seed)
+    shape = [0, 0, 0]
+    inplane = [a for a in range(3) if a != axis]
+    shape[axis] = 1
+    shape[inplane[0]], shape[inplane[1]] = size
+    nx, ny, nz = shape
+    cfs = None
+    if order == 2:
+        a, b = CFS(), CFS()
+        b.alpha.max = 0.05
+        b.kappa.scalingprofile, b.kappa.min, b.kappa.max = 'quadratic', 1, 3
+        cfs = [a, b]
+    bld = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, iterations, pml_cells=pml,
+                       pml_formulation=formulation, cfs=cfs)
+    assert bld.G.mode == '2D TM' + 'xyz'[axis]
+    for n in range(nmat):
+        m = bld.add_material(rng.uniform(1, 9), rng.uniform(0, 0.05), rng.uniform(1, 2), rng.uniform(0, 1e3),
+                             'm{}'.format(n))
+        if poles and n % 2 == 0:
+            if kind == 'debye':
+                add_dispersion(m, 'debye', list(rng.uniform(0.5, 5, poles)), list(10 ** rng.uniform(-11, -9, poles)))
+            elif kind == 'lorentz':
+                add_dispersion(m, 'lorentz', list(rng.uniform(0.5, 5, poles)), list(rng.uniform(1e9, 5e9, poles)),
+                               list(rng.uniform(1e8, 5e8, poles)))
+            else:
+                add_dispersion(m, 'drude', None, list(rng.uniform(1e9, 3e9, poles)), list(rng.uniform(1e9, 5e9, poles)))
+    mask = rng.random((nx, ny, nz)) < 0.6
+    ids = rng.integers(2, 2 + nmat, (nx, ny, nz)).astype(np.uint32)
+    bld.paint_voxels(mask, ids)
+
+    def cell(u, v):
+        c = [0, 0, 0]
+        c[inplane[0]], c[inplane[1]] = u, v
+        return tuple(c)
+
+    lo, hi = [0, 0, 0], [nx, ny, nz]
+    lo[inplane[0]], hi[inplane[0]] = size[0] // 2 - 3, size[0] // 2
+    lo[inplane[1]], hi[inplane[1]] = 2, 5
+    bld.box(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], bld.G.materials[0], averaging=False)
+    bld.add_waveform('ricker', 1, 20e9, 'w1')
+    bld.add_waveform('gaussiandot', 2.0, 15e9, 'w2')
+    pol = 'xyz'[axis]
+    u0, v0 = size[0] // 2, size[1] // 2
+    bld.add_hertzian_dipole(pol, cell(u0, v0), 'w1')
+    bld.add_hertzian_dipole(pol, cell(u0 - 3, v0 + 2), 'w2', start=5e-12, stop=40e-12)
+    bld.add_magnetic_dipole('xyz'[inplane[0]], cell(u0 + 2, v0 - 3), 'w1')
+    bld.add_voltage_source(pol, cell(u0 - 4, v0 - 2), 50.0, 'w2')
+    bld.add_voltage_source(pol, cell(u0 + 4, v0 + 3), 0.0, 'w1')
+    for r in range(4):
+        bld.add_rx(cell(int(rng.integers(pml, size[0] - pml)), int(rng.integers(pml, size[1] - pml))))
+    bld.add_rx(cell(u0, v0))
+    if snapshots:
+        bld.add_snapshot(0, 0, 0, nx, ny, nz, 1, 1, 1, iterations // 2)
+    return bld.finalise()

@@ -61,9 +61,62 @@ def hertzian_dipole_dispersive(precision='single', n=100, **kw):
     return b.finalise()
 
 
+def _plane(axis, precision='single', n=100, **kw):
+    """tests/models_basic/2D_{ExHyHz,EyHxHz,EzHxHy}/*.in: a 100 x 100 cell
+    plane, one cell thick along `axis`, default PML (none on that axis),
+    gaussiandot 1 GHz dipole along `axis` at the centre, rx 20 cells further
+    along both in-plane axes."""
+    size = [n, n, n]
+    size[axis] = 1
+    b = ModelBuilder(*size, 0.001, 0.001, 0.001, 3e-9, precision=precision, **kw)
+    src = [n // 2] * 3
+    rx = [n // 2 + n // 5] * 3
+    src[axis] = rx[axis] = 0
+    b.add_waveform('gaussiandot', 1, 1e9, 'myWave')
+    b.add_hertzian_dipole('xyz'[axis], tuple(src), 'myWave')
+    b.add_rx(tuple(rx))
+    return b.finalise()
+
+
+def plane_ExHyHz(precision='single', **kw):
+    return _plane(0, precision, **kw)
+
+
+def plane_EyHxHz(precision='single', **kw):
+    return _plane(1, precision, **kw)
+
+
+def plane_EzHxHy(precision='single', **kw):
+    return _plane(2, precision, **kw)
+
+
+def cylinder_Ascan_2D(precision='single', **kw):
+    """tests/models_basic/cylinder_Ascan_2D/cylinder_Ascan_2D.in: 240 x 210 mm
+    TMz plane of 2 mm cells, er=6 half-space below y = 170 mm holding a PEC
+    cylinder of 10 mm radius at (120, 80) mm, ricker 1.5 GHz dipole at
+    (100, 170) mm, rx at (140, 170) mm."""
+    b = ModelBuilder(120, 105, 1, 0.002, 0.002, 0.002, 3e-9, precision=precision, **kw)
+    hs = b.add_material(6, 0, 1, 0, 'half_space')
+    b.add_waveform('ricker', 1, 1.5e9, 'my_ricker')
+    b.add_hertzian_dipole('z', (50, 85, 0), 'my_ricker')
+    b.add_rx((70, 85, 0))
+    b.box(0, 120, 0, 85, 0, 1, hs, averaging=True)
+    b.cylinder(0.120, 0.080, 0, 0.120, 0.080, 0.002, 0.010, b.G.materials[0])
+    return b.finalise()
+
+
+PLANE_MODELS = {
+    '2D_ExHyHz': plane_ExHyHz,
+    '2D_EyHxHz': plane_EyHxHz,
+    '2D_EzHxHy': plane_EzHxHy,
+    'cylinder_Ascan_2D': cylinder_Ascan_2D,
+}
+
+
 BASIC_MODELS = {
     'hertzian_dipole_fs': hertzian_dipole_fs,
     'magnetic_dipole_fs': magnetic_dipole_fs,
     'hertzian_dipole_hs': hertzian_dipole_hs,
     'hertzian_dipole_dispersive': hertzian_dipole_dispersive,
 }
+BASIC_MODELS.update(PLANE_MODELS)

@@ -11,8 +11,8 @@ import pytest
 
 import oracle
 from oracle import solver as osolver
-from gpu_models import rich_model
-from models import BASIC_MODELS
+from gpu_models import plane_model, rich_model
+from models import BASIC_MODELS, PLANE_MODELS
 
 pytestmark = pytest.mark.gpu
 
@@ -141,6 +141,23 @@ def test_each_substep_matches_oracle(case):
                 assert np.array_equal(out[c], snap.fields[c]), ('snapshot', n, c)
 
 
+def _compare_runs(F, rxb, snaps, phis, T, Gc):
+    for n, c in enumerate(COMPS):
+        assert np.array_equal(F[n], getattr(Gc, c)), c
+    for r, rx in enumerate(Gc.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
+    for n, snap in enumerate(Gc.snapshots):
+        for c in range(6):
+            assert np.array_equal(snaps[n][c], snap.fields[c]), ('snapshot', n, c)
+    for n, pml in enumerate(Gc.pmls):
+        for w, name in enumerate(('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2')):
+            assert np.array_equal(phis[n][w], getattr(pml, name)), (pml.ID, name)
+    if T is not None:
+        for n, name in enumerate(('Tx', 'Ty', 'Tz')):
+            assert np.array_equal(T[n], getattr(Gc, name)), name
+
+
 @pytest.mark.parametrize('fused', [1, 0, 2], ids=['fused-step', 'two-pass', 'side-by-side'])
 @pytest.mark.parametrize('case', list(CASES))
 def test_fused_run_matches_oracle(case, fused):
@@ -158,20 +175,51 @@ def test_fused_run_matches_oracle(case, fused):
         phis = [[s.download_phi(n, w) for w in range(4)] for n in range(len(G.pmls))]
         T = s.download_T() if G.maxpoles else None
     osolver.solve_cpu(1, 1, Gc, nthreads=8)
-    for n, c in enumerate(COMPS):
-        assert np.array_equal(F[n], getattr(Gc, c)), c
-    for r, rx in enumerate(Gc.rxs):
-        for c, name in enumerate(COMPS):
-            assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
-    for n, snap in enumerate(Gc.snapshots):
-        for c in range(6):
-            assert np.array_equal(snaps[n][c], snap.fields[c]), ('snapshot', n, c)
-    for n, pml in enumerate(Gc.pmls):
-        for w, name in enumerate(('EPhi1', 'EPhi2', 'HPhi1', 'HPhi2')):
-            assert np.array_equal(phis[n][w], getattr(pml, name)), (pml.ID, name)
-    if T is not None:
-        for n, name in enumerate(('Tx', 'Ty', 'Tz')):
-            assert np.array_equal(T[n], getattr(Gc, name)), name
+    _compare_runs(F, rxb, snaps, phis, T, Gc)
+
+
+PLANE_CASES = {
+    'horipml1': dict(order=1),
+    'mripml2': dict(order=2, formulation='MRIPML'),
+    'debye2': dict(poles=2, kind='debye'),
+    'lorentz1': dict(poles=1, kind='lorentz', order=2),
+}
+
+
+@pytest.mark.parametrize('fused', [1, 0, 2], ids=['fused-step', 'two-pass', 'side-by-side'])
+@pytest.mark.parametrize('axis', [0, 1, 2], ids=['TMx', 'TMy', 'TMz'])
+@pytest.mark.parametrize('case', list(PLANE_CASES))
+def test_2d_run_matches_oracle(case, axis, fused):
+    """2-D models (one cell along `axis`): the GPU runs the 3-D update ranges,
+    the oracle the reference's 2-D branches; final state, Phi, T, receiver
+    traces and snapshots are the same bits."""
+    G = plane_model(seed=13 + axis, axis=axis, **PLANE_CASES[case])
+    Gc = plane_model(seed=13 + axis, axis=axis, **PLANE_CASES[case])
+    with _hip()(G) as s:
+        s.set_step(fused)
+        assert (s.step_kind() > 0) == (fused == 1)
+        s.run(0, G.iterations)
+        F = s.download_fields()
+        rxb = s.download_rx()
+        snaps = [s.download_snapshot(n) for n in range(len(G.snapshots))]
+        phis = [[s.download_phi(n, w) for w in range(4)] for n in range(len(G.pmls))]
+        T = s.download_T() if G.maxpoles else None
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    assert np.abs(np.stack([rx.outputs[COMPS[axis]] for rx in Gc.rxs])).max() > 0
+    _compare_runs(F, rxb, snaps, phis, T, Gc)
+
+
+@pytest.mark.parametrize('name', list(PLANE_MODELS))
+def test_reference_2d_models_match_oracle(name):
+    """The reference's 2-D test models through solve_hip: receiver traces equal
+    to the oracle's (which the CPU tests pin to the reference's files)."""
+    from gprmax_amd.solver import solve_hip
+    G = PLANE_MODELS[name]()
+    solve_hip(1, 1, G)
+    Gc = PLANE_MODELS[name]()
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    for c in COMPS:
+        assert np.array_equal(G.rxs[0].outputs[c], Gc.rxs[0].outputs[c]), c
 
 
 @pytest.mark.parametrize('xchunk,block', [(1, 64), (7, 128), (0, 256), (300, 512), (5, 1024)])

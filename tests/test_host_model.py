@@ -182,3 +182,60 @@ def test_averaged_half_space_edges():
     # The reference's magnetic match accepts a material holding neither name an
     # even number of times (zero), so the first such material (pec) is taken.
     assert int(G.ID[5, 5, 5, 10]) == 0
+
+
+@pytest.mark.parametrize('axis', [0, 1, 2])
+def test_2d_model_has_no_pml_across_and_pec_on_its_planes(axis):
+    """A grid one cell thick is a 2-D model: the Courant limit of the other two
+    axes, no PML slab on the invariant axis, and the two E components across
+    that axis PEC on both of its grid planes (model_build_run.py:222-241)."""
+    shape = [12, 12, 12]
+    shape[axis] = 1
+    b = ModelBuilder(*shape, 1e-3, 2e-3, 3e-3, 10, pml_cells=3)
+    G = b.finalise()
+    assert G.mode == '2D TM' + 'xyz'[axis]
+    d = [1e-3, 2e-3, 3e-3]
+    d.pop(axis)
+    raw = 1 / (c * np.sqrt(1 / d[0] ** 2 + 1 / d[1] ** 2))
+    assert G.dt <= raw and raw - G.dt < 1e-26
+    assert sorted(p.direction[0] for p in G.pmls) == sorted(['xyz'[a] for a in range(3) if a != axis] * 2)
+    for comp in range(3):
+        for plane in (0, 1):
+            sl = [slice(None)] * 3
+            sl[axis] = plane
+            ids = G.ID[comp][tuple(sl)]
+            assert (not ids.any()) == (comp != axis), (comp, plane)
+    # the in-plane H and the axis E keep free space
+    assert (G.ID[axis] == 1).any()
+
+
+def test_axis_aligned_cylinder_matches_brute_force():
+    """#cylinder: the cells whose centre lies within r of the axis, tested with
+    the reference's float32 cell positions (geometry_primitives_ext.pyx:
+    816-822), for each alignment; a pure-Python loop over all cells."""
+    f = np.float32
+    for axis, (p1, p2) in enumerate((((0.0, 0.021, 0.017), (0.030, 0.021, 0.017)),
+                                     ((0.013, 0.0, 0.019), (0.013, 0.040, 0.019)),
+                                     ((0.019, 0.018, 0.0), (0.019, 0.018, 0.024)))):
+        b = ModelBuilder(40, 41, 42, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+        m = b.add_material(4, 0, 1, 0, 'rod')
+        r = 0.0071
+        b.cylinder(*p1, *p2, r, m, averaging=False)
+        got = b.G.solid == m.numID
+        u, v = [a for a in range(3) if a != axis]
+        want = np.zeros_like(got)
+        for i in range(40):
+            for j in range(41):
+                for k in range(42):
+                    idx = (i, j, k)
+                    lo = round_value(p1[axis] / 1e-3)
+                    hi = round_value(p2[axis] / 1e-3)
+                    if not (lo <= idx[axis] < hi):
+                        continue
+                    du = float(f(idx[u]) * f(1e-3)) + 0.5 * float(f(1e-3)) - float(f(p1[u]))
+                    dv = float(f(idx[v]) * f(1e-3)) + 0.5 * float(f(1e-3)) - float(f(p1[v]))
+                    want[idx] = np.sqrt(du * du + dv * dv) <= float(f(r))
+        assert want.sum() > 100
+        assert np.array_equal(got, want), axis
+        # a rigid cylinder writes every edge of its cells
+        assert (b.G.ID[axis][tuple(np.argwhere(want)[0])] == m.numID)

@@ -34,7 +34,8 @@ def test_create_rejects_bad_descriptors_before_touching_a_device():
     from gprmax_amd.hip import GpxDesc, library
     lib = library()
     ctx = ctypes.c_void_p()
-    bad = [GpxDesc(1, 10, 10, 5, 2, 0, 0, 0),     # 2-D grid
+    bad = [GpxDesc(1, 1, 10, 5, 2, 0, 0, 0),      # one cell on two axes
+           GpxDesc(0, 10, 10, 5, 2, 0, 0, 0),     # no cells
            GpxDesc(10, 10, 10, 0, 2, 0, 0, 0),    # no iterations
            GpxDesc(10, 10, 10, 5, 2, 9, 0, 0),    # too many poles
            GpxDesc(10, 10, 10, 5, 2, 0, 7, 0),    # unknown PML formulation
