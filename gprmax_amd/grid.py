@@ -57,6 +57,8 @@ class FDTDGrid:
         self.transmissionlines = []
         self.rxs = []
         self.snapshots = []
+        self.srcsteps = [0, 0, 0]
+        self.rxsteps = [0, 0, 0]
         self.maxpoles = 0
         self.updatecoeffsdispersive = None
         # float32/complex64 by default; float64/complex128 reproduces the
