@@ -156,8 +156,12 @@ struct ELoad {
 // update_magnetic (gprMax/fields_updates_ext.pyx:404-412), then each H-PML
 // slab in order, then magnetic dipoles; receivers sample the state at the
 // start of the iteration.
+// The pass bodies run items first, first + stride, ... of the model `a`; `it`
+// is the iteration.  k_*_pass hold one model's arguments by value; the
+// k_*_pass_many kernels find theirs, one item per block, in a device table of
+// several models (a batch of B-scan traces).
 template <class IDT, bool BND>
-__global__ void k_h_pass(StepArgs a)
+__device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int stride, int it)
 {
     extern __shared__ float4 lds4[];
     __shared__ SlabArgs lslab[MAX_SLABS];
@@ -182,7 +186,7 @@ __global__ void k_h_pass(StepArgs a)
     const rsrc_t rI3 = rsrc(ID + 3 * N), rI4 = rsrc(ID + 4 * N), rI5 = rsrc(ID + 5 * N);
     const uint32_t P4 = (uint32_t)a.plane * 4u, Pid = (uint32_t)a.plane * (uint32_t)sizeof(IDT);
 
-    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
+    for (int item = first; item < a.nitems; item += stride) {
         const Cell c = item_cell<BND>(a, item);
         if (!c.live) continue;
         const int j = c.j, k = c.k;
@@ -277,11 +281,11 @@ __global__ void k_h_pass(StepArgs a)
             if ((flags & F_SRC) && (pf & PF_SRCH)) {  // magnetic dipoles, gprMax/sources.py:202-232
                 for (int s = 0; s < a.src0.n; ++s) {
                     const int *in = a.src0.info + 6 * s;
-                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[0] != i || it < in[4] || it > in[5]) continue;
                     if (in[1] == j && in[2] == k) {
                         const int pol = in[3];
                         const uint32_t m = pol == 0 ? m3 : (pol == 1 ? m4 : m5);
-                        const float w = a.src0.waves[(size_t)s * a.iterations + a.it];
+                        const float w = a.src0.waves[(size_t)s * a.iterations + it];
                         const float v = ls[m] * w * a.src0.fac[2 * s];
                         if (pol == 0) hx = hx - v;
                         else if (pol == 1) hy = hy - v;
@@ -300,7 +304,7 @@ __global__ void k_h_pass(StepArgs a)
                 for (int r = 0; r < a.nrx; ++r) {
                     if (a.rxc[3 * r] != i) continue;
                     if (a.rxc[3 * r + 1] == j && a.rxc[3 * r + 2] == k) {
-                        float *o = a.rxbuf + (size_t)a.it * a.nrx + r;
+                        float *o = a.rxbuf + (size_t)it * a.nrx + r;
                         const size_t cs = (size_t)a.iterations * a.nrx;
                         o[0] = ex; o[cs] = ey; o[2 * cs] = ez;
                         o[3 * cs] = hx0; o[4 * cs] = hy0; o[5 * cs] = hz0;
@@ -314,13 +318,49 @@ __global__ void k_h_pass(StepArgs a)
     }
 }
 
+template <class IDT, bool BND>
+__global__ void k_h_pass(StepArgs a)
+{
+    h_pass_run<IDT, BND>(a, logical_block(a.per_xcd), gridDim.x, a.it);
+}
+
+struct PassMany {
+    const int *item0;      // [ntr + 1], first item of each model
+    int ntr, nitems, per_xcd, it;
+};
+
+__device__ __forceinline__ int model_of(const PassMany &m, int item)
+{
+    int lo = 0, hi = m.ntr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m.item0[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// `args` is a kernel parameter of its own, read-only and unaliased, so that
+// the model's arguments are read with scalar loads as by-value ones are.
+template <class IDT, bool BND>
+__global__ void k_h_pass_many(const StepArgs *__restrict__ args, PassMany m)
+{
+    const int item = logical_block(m.per_xcd);
+    if (item >= m.nitems) return;
+    const int t = model_of(m, item);
+    const StepArgs &a = args[t];
+    const int local = item - m.item0[t];
+    h_pass_run<IDT, BND>(a, local, a.nitems, m.it);
+}
+
+
 // ---------------------------------------------------------------- E pass
 
 // update_electric or dispersive part A (gprMax/fields_updates_ext.pyx:81-107,
 // 146-179), each E-PML slab in order, voltage sources, Hertzian dipoles and
 // dispersive part B, all on the cell's own thread.
 template <class IDT, int P, bool CPLX, bool BND>
-__global__ void k_e_pass(StepArgs a)
+__device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int stride, int it)
 {
     extern __shared__ float4 lds4[];
     __shared__ SlabArgs lslab[MAX_SLABS];
@@ -352,7 +392,7 @@ __global__ void k_e_pass(StepArgs a)
     const rsrc_t rI0 = rsrc(ID), rI1 = rsrc(ID + N), rI2 = rsrc(ID + 2 * N);
     const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
 
-    for (int item = logical_block(a.per_xcd); item < a.nitems; item += gridDim.x) {
+    for (int item = first; item < a.nitems; item += stride) {
         const Cell c = item_cell<BND>(a, item);
         if (!c.live) continue;
         const int j = c.j, k = c.k;
@@ -472,12 +512,12 @@ __global__ void k_e_pass(StepArgs a)
                 // Voltage sources, then Hertzian dipoles (gprMax/model_build_run.py:458-461).
                 for (int s = 0; s < a.src0.n; ++s) {  // gprMax/sources.py:81-120
                     const int *in = a.src0.info + 6 * s;
-                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[0] != i || it < in[4] || it > in[5]) continue;
                     if (in[1] == j && in[2] == k) {
                         const int pol = in[3] & 3;
                         const bool hard = in[3] >= 4;
                         const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
-                        const float w = a.src0.waves[(size_t)s * a.iterations + a.it];
+                        const float w = a.src0.waves[(size_t)s * a.iterations + it];
                         float *F = pol == 0 ? &ex : (pol == 1 ? &ey : &ez);
                         if (hard) *F = -w / a.src0.fac[2 * s];
                         else *F = *F - ls[m] * w * a.src0.fac[2 * s];
@@ -485,11 +525,11 @@ __global__ void k_e_pass(StepArgs a)
                 }
                 for (int s = 0; s < a.src1.n; ++s) {  // gprMax/sources.py:163-193
                     const int *in = a.src1.info + 6 * s;
-                    if (in[0] != i || a.it < in[4] || a.it > in[5]) continue;
+                    if (in[0] != i || it < in[4] || it > in[5]) continue;
                     if (in[1] == j && in[2] == k) {
                         const int pol = in[3];
                         const uint32_t m = pol == 0 ? m0 : (pol == 1 ? m1 : m2);
-                        const float w = a.src1.waves[(size_t)s * a.iterations + a.it];
+                        const float w = a.src1.waves[(size_t)s * a.iterations + it];
                         const float v = ls[m] * w * a.src1.fac[2 * s] * a.src1.fac[2 * s + 1];
                         if (pol == 0) ex = ex - v;
                         else if (pol == 1) ey = ey - v;
@@ -518,6 +558,23 @@ __global__ void k_e_pass(StepArgs a)
             if (!BND && P == 0) cur = nxt;
         }
     }
+}
+
+template <class IDT, int P, bool CPLX, bool BND>
+__global__ void k_e_pass(StepArgs a)
+{
+    e_pass_run<IDT, P, CPLX, BND>(a, logical_block(a.per_xcd), gridDim.x, a.it);
+}
+
+template <class IDT, int P, bool CPLX, bool BND>
+__global__ void k_e_pass_many(const StepArgs *__restrict__ args, PassMany m)
+{
+    const int item = logical_block(m.per_xcd);
+    if (item >= m.nitems) return;
+    const int t = model_of(m, item);
+    const StepArgs &a = args[t];
+    const int local = item - m.item0[t];
+    e_pass_run<IDT, P, CPLX, BND>(a, local, a.nitems, m.it);
 }
 
 
@@ -1265,6 +1322,30 @@ static pass_kernel pick_kernel(const gpx_ctx *ctx, int field)
     return ctx->id_bytes == 2 ? e_kernel<uint16_t, BND>(P, C) : e_kernel<uint32_t, BND>(P, C);
 }
 
+typedef void (*many_kernel)(const StepArgs *, PassMany);
+
+template <class IDT, bool BND>
+static many_kernel e_many(int P, bool C)
+{
+    switch (P) {
+    case 0: return k_e_pass_many<IDT, 0, false, BND>;
+    case 1: return C ? k_e_pass_many<IDT, 1, true, BND> : k_e_pass_many<IDT, 1, false, BND>;
+    case 2: return C ? k_e_pass_many<IDT, 2, true, BND> : k_e_pass_many<IDT, 2, false, BND>;
+    case 3: return C ? k_e_pass_many<IDT, 3, true, BND> : k_e_pass_many<IDT, 3, false, BND>;
+    default: return C ? k_e_pass_many<IDT, 4, true, BND> : k_e_pass_many<IDT, 4, false, BND>;
+    }
+}
+
+template <bool BND>
+static many_kernel pick_many(const gpx_ctx *ctx, int field)
+{
+    if (field == 1)
+        return ctx->id_bytes == 2 ? k_h_pass_many<uint16_t, BND> : k_h_pass_many<uint32_t, BND>;
+    const int P = ctx->desc.maxpoles;
+    const bool C = ctx->disp_complex != 0;
+    return ctx->id_bytes == 2 ? e_many<uint16_t, BND>(P, C) : e_many<uint32_t, BND>(P, C);
+}
+
 // One pass = interior kernel (skipped when only PML work is asked for) then
 // boundary kernel; they touch disjoint cells.  With `ev`, an event is recorded
 // before, between and after the two launches.
@@ -1829,9 +1910,13 @@ struct gpx_batch {
     std::vector<gpx_ctx *> ctx;
     int device = 0;
     hipStream_t stream = nullptr;
-    int tile = 0, xchunk = 0;      // launch shape; 0 = chosen at run time
+    int tile = 0, xchunk = 0;      // fused-step tile and chunk; tile 0 = the H and E passes
+    bool fused_ok = false;         // every model can take the same fused step
+    double group_bytes = 192.0 * 1024 * 1024;  // state bytes per group of models (passes)
     FusedArgs *dargs = nullptr;    // [2][ntr]
     int *ditem0 = nullptr;         // [ntr + 1]
+    StepArgs *dpass = nullptr;     // [4][ntr]: H interior, H boundary, E interior, E boundary
+    int *dpitem = nullptr;         // [4][ntr + groups]: each group's first item per model
     std::string err;
 };
 
@@ -1857,6 +1942,7 @@ extern "C" const char *gpx_batch_last_error(const gpx_batch *b)
 static std::vector<int> batch_tiles(const gpx_batch *b)
 {
     std::vector<int> out;
+    if (!b->fused_ok) return out;
     for (int tc : {256, 512, 1024}) {
         bool ok = true;
         for (const gpx_ctx *c : b->ctx) {
@@ -1867,6 +1953,8 @@ static std::vector<int> batch_tiles(const gpx_batch *b)
     }
     return out;
 }
+
+extern "C" int gpx_batch_destroy(gpx_batch *b);
 
 extern "C" int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out)
 {
@@ -1883,29 +1971,31 @@ extern "C" int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out)
             return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models must have the same number of iterations");
         if (!c->snaps.empty())
             return bfail(nullptr, GPX_EARG, "gpx_batch_create: models with snapshots are stepped one by one");
-        if (slab_order(c) < 0 || (double)c->fstride * 24.0 >= 4294967295.0 ||
-            (double)c->ntot * 6 * c->id_bytes >= 4294967295.0)
-            return bfail(nullptr, GPX_EARG, "gpx_batch_create: model " + std::to_string(t) + " cannot use the fused step");
-        if (pick_step(c).many != pick_step(c0).many)
+        if (c->id_bytes != c0->id_bytes || c->desc.maxpoles != c0->desc.maxpoles ||
+            c->disp_complex != c0->disp_complex || c->desc.pml_formulation != c0->desc.pml_formulation)
             return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models need the same kernel (index width, "
-                                            "poles and their kind, PML formulation and order)");
+                                            "poles and their kind, PML formulation)");
     }
     gpx_batch *b = new gpx_batch();
     b->ctx.assign(ctxs, ctxs + n);
     b->device = c0->desc.device;
-    if (batch_tiles(b).empty()) {
-        delete b;
-        return bfail(nullptr, GPX_EARG, "gpx_batch_create: no tile size fits every model");
-    }
+    // The fused step as a second choice: every model eligible, one kernel.
+    b->fused_ok = true;
+    for (gpx_ctx *c : b->ctx)
+        if (slab_order(c) < 0 || (double)c->fstride * 24.0 >= 4294967295.0 ||
+            (double)c->ntot * 6 * c->id_bytes >= 4294967295.0 || pick_step(c).many != pick_step(c0).many)
+            b->fused_ok = false;
+    if (b->fused_ok && batch_tiles(b).empty()) b->fused_ok = false;
     hipSetDevice(b->device);
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&b->dargs, sizeof(FusedArgs) * 2 * n);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ditem0, sizeof(int) * (n + 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->dpass, sizeof(StepArgs) * 4 * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->dpitem, sizeof(int) * 4 * (2 * n));
     if (e != hipSuccess) {
-        if (b->dargs) hipFree(b->dargs);
-        if (b->stream) hipStreamDestroy(b->stream);
-        delete b;
-        return bfail(nullptr, (int)e, std::string("gpx_batch_create: ") + hipGetErrorString(e));
+        hipError_t e2 = e;
+        gpx_batch_destroy(b);
+        return bfail(nullptr, (int)e2, std::string("gpx_batch_create: ") + hipGetErrorString(e2));
     }
     *out = b;
     return 0;
@@ -1918,6 +2008,8 @@ extern "C" int gpx_batch_destroy(gpx_batch *b)
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->dargs) hipFree(b->dargs);
     if (b->ditem0) hipFree(b->ditem0);
+    if (b->dpass) hipFree(b->dpass);
+    if (b->dpitem) hipFree(b->dpitem);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return 0;
@@ -1926,7 +2018,7 @@ extern "C" int gpx_batch_destroy(gpx_batch *b)
 extern "C" int gpx_batch_set_shape(gpx_batch *b, int tile_cells, int xchunk)
 {
     if (!b || xchunk < 0) return bfail(b, GPX_EARG, "gpx_batch_set_shape: bad argument");
-    if (tile_cells) {
+    if (tile_cells) {  // 0 = the H and E passes
         const std::vector<int> ok = batch_tiles(b);
         bool found = false;
         for (int tc : ok) found = found || tc == tile_cells;
@@ -2072,14 +2164,170 @@ static int batch_steps(gpx_batch *b, const BatchPlan &p, int it0, int n, int gra
     return 0;
 }
 
+// The H and E passes of every model, as four launches per iteration (H
+// interior, H boundary, E interior, E boundary), each over the items of a group
+// of models: one block per item, the block finding its model in the table.
+// Groups are consecutive models whose state fits together in the 256 MB
+// last-level cache (192 MB budget; a larger model is a group of its own); a
+// group runs all its iterations before the next one starts, so that its state
+// stays cached between passes, as it does when the models run one by one.
+static const double GROUP_BYTES = 192.0 * 1024 * 1024;
+
+extern "C" int gpx_batch_set_group_bytes(gpx_batch *b, double bytes)
+{
+    if (!b || !(bytes >= 0)) return bfail(b, GPX_EARG, "gpx_batch_set_group_bytes: bad argument");
+    b->group_bytes = bytes > 0 ? bytes : GROUP_BYTES;
+    return 0;
+}
+
+struct PassGroup {
+    int g0 = 0, gn = 0;
+    int nitems[4] = {0, 0, 0, 0}, grid[4] = {0, 0, 0, 0};
+    size_t lds[4] = {0, 0, 0, 0};
+    size_t item_off[4] = {0, 0, 0, 0};  // of the group's item0 table in dpitem
+};
+
+struct PassPlan {
+    many_kernel k[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<PassGroup> groups;
+    int B = 256;
+};
+
+static int batch_prepare_passes(gpx_batch *b, PassPlan &p)
+{
+    const int n = (int)b->ctx.size();
+    for (int kind = 0; kind < 4; ++kind)
+        p.k[kind] = (kind & 1) ? pick_many<true>(b->ctx[0], kind < 2 ? 1 : 0)
+                               : pick_many<false>(b->ctx[0], kind < 2 ? 1 : 0);
+    // greedy groups in model order
+    p.groups.clear();
+    double acc = 0;
+    for (int t = 0; t < n; ++t) {
+        const gpx_ctx *c = b->ctx[t];
+        const double bytes = (double)c->bytes - 6.0 * c->fstride * 4;  // the second state is not used
+        if (p.groups.empty() || acc + bytes > b->group_bytes) {
+            p.groups.push_back(PassGroup());
+            p.groups.back().g0 = t;
+            acc = 0;
+        }
+        p.groups.back().gn++;
+        acc += bytes;
+    }
+    const int G = (int)p.groups.size();
+    std::vector<StepArgs> args(4 * (size_t)n);
+    std::vector<int> item0(4 * (size_t)(n + G), 0);
+    for (int gi = 0; gi < G; ++gi) {
+        PassGroup &g = p.groups[gi];
+        for (int kind = 0; kind < 4; ++kind) {
+            const int field = kind < 2 ? 1 : 0;
+            const bool bnd = kind & 1;
+            g.item_off[kind] = (size_t)kind * (n + G) + g.g0 + gi;
+            int *i0 = &item0[g.item_off[kind]];
+            for (int t = 0; t < g.gn; ++t) {
+                gpx_ctx *c = b->ctx[g.g0 + t];
+                StepArgs &a = args[(size_t)kind * n + g.g0 + t];
+                int block, grid;
+                size_t lds;
+                base_args(c, field, a, block, lds);
+                plan(c, bnd, 0, p.B, a, grid);
+                const int f = field == 1 ? h_flags(c) : e_flags(c);
+                a.flags = bnd ? f : f & ~F_PML_ALL;
+                a.it = 0;
+                if (lds > g.lds[kind]) g.lds[kind] = lds;
+                i0[t + 1] = i0[t] + a.nitems;
+            }
+            g.nitems[kind] = i0[g.gn];
+            g.grid[kind] = (g.nitems[kind] + 7) / 8 * 8;
+        }
+    }
+    hipSetDevice(b->device);
+    GPX_BTRY(hipMemcpyAsync(b->dpass, args.data(), args.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+    GPX_BTRY(hipMemcpyAsync(b->dpitem, item0.data(), item0.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    GPX_BTRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+static void batch_pass_launch(const gpx_batch *b, const PassPlan &p, const PassGroup &g, int it)
+{
+    const int n = (int)b->ctx.size();
+    for (int kind = 0; kind < 4; ++kind) {
+        if (!g.nitems[kind]) continue;
+        PassMany m;
+        const StepArgs *args = b->dpass + (size_t)kind * n + g.g0;
+        m.item0 = b->dpitem + g.item_off[kind];
+        m.ntr = g.gn;
+        m.nitems = g.nitems[kind];
+        m.per_xcd = g.grid[kind] / 8;
+        m.it = it;
+        hipLaunchKernelGGL(p.k[kind], dim3(g.grid[kind]), dim3(p.B), g.lds[kind], b->stream, args, m);
+    }
+}
+
+static int batch_pass_steps(gpx_batch *b, const PassPlan &p, int it0, int n, int graph, float *ms)
+{
+    for (gpx_ctx *c : b->ctx) GPX_BTRY(hipStreamSynchronize(c->stream));
+    hipEvent_t e0, e1;
+    GPX_BTRY(hipEventCreate(&e0));
+    GPX_BTRY(hipEventCreate(&e1));
+    std::vector<hipGraph_t> gr;
+    std::vector<hipGraphExec_t> ge;
+    int rc = 0;
+    if (graph) {
+        for (const PassGroup &g : p.groups) {
+            hipGraph_t h = nullptr;
+            hipGraphExec_t x = nullptr;
+            hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                for (int s = 0; s < n; ++s) batch_pass_launch(b, p, g, it0 + s);
+                e = hipStreamEndCapture(b->stream, &h);
+            }
+            if (e == hipSuccess) gr.push_back(h);
+            if (e == hipSuccess) e = hipGraphInstantiate(&x, h, nullptr, nullptr, 0);
+            if (e == hipSuccess) ge.push_back(x);
+            if (e != hipSuccess) {
+                rc = bfail(b, (int)e, std::string("gpx_batch: graph: ") + hipGetErrorString(e));
+                break;
+            }
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipEventRecord(e0, b->stream);
+        for (size_t gi = 0; e == hipSuccess && gi < p.groups.size(); ++gi) {
+            if (graph) {
+                e = hipGraphLaunch(ge[gi], b->stream);
+            } else {
+                for (int s = 0; e == hipSuccess && s < n; ++s) {
+                    batch_pass_launch(b, p, p.groups[gi], it0 + s);
+                    e = hipGetLastError();
+                }
+            }
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = bfail(b, (int)e, std::string("gpx_batch: ") + hipGetErrorString(e));
+        else if (ms) *ms = t;
+    }
+    for (hipGraphExec_t x : ge) hipGraphExecDestroy(x);
+    for (hipGraph_t h : gr) hipGraphDestroy(h);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
 extern "C" int gpx_batch_run(gpx_batch *b, int it0, int n, int graph, float *ms)
 {
     if (!b || it0 < 0 || n < 1 || it0 + n > b->ctx[0]->desc.iterations)
         return bfail(b, GPX_EARG, "gpx_batch_run: bad argument");
     hipSetDevice(b->device);
+    if (!b->tile) {
+        PassPlan p;
+        if (int rc = batch_prepare_passes(b, p)) return rc;
+        return batch_pass_steps(b, p, it0, n, graph, ms);
+    }
     BatchPlan p;
-    const int tc = b->tile ? b->tile : batch_tiles(b).back();
-    if (int rc = batch_prepare(b, tc, b->xchunk, p)) return rc;
+    if (int rc = batch_prepare(b, b->tile, b->xchunk, p)) return rc;
     return batch_steps(b, p, it0, n, graph, ms);
 }
 
@@ -2110,6 +2358,14 @@ extern "C" int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xch
     hipSetDevice(b->device);
     float best = -1.f;
     int bt = 0, bx = 0;
+    {
+        PassPlan p;
+        if (int rc = batch_prepare_passes(b, p)) return rc;
+        float ms = 0;
+        if (int rc = batch_pass_steps(b, p, 0, 1, 0, nullptr)) return rc;  // first launch
+        if (int rc = batch_pass_steps(b, p, 0, steps, 0, &ms)) return rc;
+        best = ms / steps;
+    }
     for (int tc : batch_tiles(b)) {
         BatchPlan p;
         if (int rc = batch_prepare(b, tc, 0, p)) return rc;

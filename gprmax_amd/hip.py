@@ -66,6 +66,7 @@ SIGNATURES = {
     'gpx_batch_destroy': ([_P], _I),
     'gpx_batch_last_error': ([_P], ctypes.c_char_p),
     'gpx_batch_set_shape': ([_P, _I, _I], _I),
+    'gpx_batch_set_group_bytes': ([_P, ctypes.c_double], _I),
     'gpx_batch_tune': ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_F)], _I),
     'gpx_batch_run': ([_P, _I, _I, _I, ctypes.POINTER(_F)], _I),
 }
@@ -272,8 +273,10 @@ def device_info(device=0):
 
 
 class HipBatch:
-    """Several HipSolver models advanced together, one fused launch per
-    iteration for all of them (optionally all iterations in one HIP graph)."""
+    """Several HipSolver models advanced together: per iteration, four launches
+    (the H and E passes, interior and boundary) each over all models, or one
+    launch of the fused step over all models; optionally all iterations in one
+    HIP graph."""
 
     def __init__(self, solvers):
         self.lib = library()
@@ -307,11 +310,18 @@ class HipBatch:
         self.close()
 
     def set_shape(self, tile_cells=0, xchunk=0):
+        """tile_cells 0: the H and E passes; 256, 512, 1024: the fused step."""
         self._check(self.lib.gpx_batch_set_shape(self.b, int(tile_cells), int(xchunk)))
+
+    def set_group_bytes(self, nbytes=0):
+        """State bytes per group of models the passes advance together
+        (0: 192 MiB)."""
+        self._check(self.lib.gpx_batch_set_group_bytes(self.b, float(nbytes)))
 
     def tune(self, steps=3):
         """Pick the fastest launch shape; returns (tile_cells, xchunk, ms per
-        iteration).  Leaves every model at the zero state."""
+        iteration), tile_cells 0 for the passes.  Leaves every model at the
+        zero state."""
         t, x, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         self._check(self.lib.gpx_batch_tune(self.b, int(steps), ctypes.byref(t), ctypes.byref(x), ctypes.byref(ms)))
         return t.value, x.value, ms.value

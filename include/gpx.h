@@ -160,15 +160,18 @@ int gpx_device_info(int device, char *name, int namelen, size_t *totalmem, size_
 /* Bytes of device memory held by ctx. */
 size_t gpx_memory_bytes(const gpx_ctx *ctx);
 
-/* ---- Batched B-scan: many models in one launch per iteration.
+/* ---- Batched B-scan: many models advanced by shared launches.
  *
  * Replaces the reference's sequential trace loop, in which every B-scan trace
  * re-runs run_model and so a whole solve_gpu (gprMax/gprMax.py:222-232 with
  * gprMax/model_build_run.py:84-406); here the traces stay separate gpx_ctx
- * models, each built and uploaded as usual, and the batch advances all of them
- * with the fused step.  Requirements: one device, the same iteration count, no
- * snapshots, and the same kernel (index width, poles and their kind, PML
- * formulation and order) for every model.  The contexts must outlive the
+ * models, each built and uploaded as usual.  Each iteration of the batch is
+ * either four launches (H interior, H boundary, E interior, E boundary), each
+ * over the work of every model (tile 0, the default), or one launch of the
+ * fused step over every model (tile > 0).  Requirements: one device, the same
+ * iteration count, no snapshots, and the same kernel (index width, poles and
+ * their kind, PML formulation) for every model; the fused step also needs one
+ * PML order for all slabs of all models.  The contexts must outlive the
  * batch; results are read from each context (gpx_download_rx, ...). */
 typedef struct gpx_batch gpx_batch;
 
@@ -176,14 +179,21 @@ int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out);
 int gpx_batch_destroy(gpx_batch *b);
 /* Message of the last failure (NULL: of the last gpx_batch_create). */
 const char *gpx_batch_last_error(const gpx_batch *b);
-/* Cells per block (512, 1024 or 2048; 0 = largest usable) and planes per x
- * chunk (0 = chosen for the batch) of the batched launch. */
+/* tile_cells 0: the H and E passes; 256, 512 or 1024: the fused step with
+ * that many cells per block and `xchunk` planes per x chunk (0 = chosen for
+ * the batch). */
 int gpx_batch_set_shape(gpx_batch *b, int tile_cells, int xchunk);
-/* Autotuner: times `steps` iterations for each usable tile size and three
- * chunk lengths around the planned one, keeps the fastest shape, reports it
- * and its time per iteration, then resets every model to the zero state
- * (fields, Phi, T, receiver buffer).  Call before stepping. */
+/* Autotuner: times `steps` iterations of the passes and of the fused step for
+ * each usable tile size and three chunk lengths around the planned one, keeps
+ * the fastest (*tile_cells 0 for the passes), reports it and its time per
+ * iteration, then resets every model to the zero state (fields, Phi, T,
+ * receiver buffer).  Call before stepping. */
 int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xchunk, float *ms_per_step);
+/* The passes advance the models in groups of consecutive models whose state
+ * (fields, indices, Phi, T) adds up to at most `bytes` (0: 192 MiB, to stay in
+ * the 256 MB last-level cache; a larger model forms its own group); each group
+ * runs all its iterations before the next starts. */
+int gpx_batch_set_group_bytes(gpx_batch *b, double bytes);
 /* Iterations [it0, it0 + n) of every model; graph != 0 captures the n
  * launches in one HIP graph and launches that.  Waits; *ms (optional) is the
  * device time measured with HIP events around the launches. */

@@ -339,16 +339,19 @@ def _batch_models(iterations):
             for n, sh in enumerate(shapes)]
 
 
-def test_batch_matches_single_runs():
-    """Three models of different sizes advanced in one launch per iteration,
-    first eagerly and then as one captured HIP graph: every model ends in the
-    state and with the receiver traces of its own run."""
+@pytest.mark.parametrize('tile', [0, 1024, 512], ids=['passes', 'fused-1024', 'fused-512'])
+def test_batch_matches_single_runs(tile):
+    """Three models of different sizes advanced by shared launches (the passes
+    over all models, or the fused step over all models), first eagerly and
+    then as one captured HIP graph: every model ends in the state and with the
+    receiver traces of its own run."""
     from gprmax_amd.hip import HipBatch
     models = _batch_models(20)
     ref = _single_runs(models, 20)
     solvers = [_hip()(G) for G in models]
     try:
         with HipBatch(solvers) as b:
+            b.set_shape(tile, 0)
             b.run(0, 7, graph=False)
             b.run(7, 13, graph=True)
         for s, (F, rx) in zip(solvers, ref):
@@ -367,7 +370,7 @@ def test_batch_autotuner_picks_a_shape_and_resets_the_state():
     try:
         with HipBatch(solvers) as b:
             tile, xchunk, ms = b.tune(steps=2)
-            assert tile in (256, 512, 1024) and xchunk >= 2 and ms > 0
+            assert (tile == 0 or (tile in (256, 512, 1024) and xchunk >= 2)) and ms > 0
             for s in solvers:
                 assert not s.download_fields().any() and not s.download_rx().any()
             b.run(0, 16, graph=True)
@@ -377,3 +380,31 @@ def test_batch_autotuner_picks_a_shape_and_resets_the_state():
     finally:
         for s in solvers:
             s.close()
+
+
+def test_pass_batch_of_unlike_models():
+    """The passes batch needs only one kernel: 3-D and 2-D models, different
+    PML orders and slab sets, different receiver counts, advanced together."""
+    from gprmax_amd.hip import HipBatch
+    models = [rich_model(seed=40, shape=(24, 20, 18), iterations=18, snapshots=False, poles=1, order=2),
+              plane_model(seed=41, axis=2, iterations=18, snapshots=False, poles=1, order=1),
+              rich_model(seed=42, shape=(21, 26, 17), iterations=18, snapshots=False, poles=1, pml=0),
+              plane_model(seed=43, axis=0, iterations=18, snapshots=False, poles=1, order=2)]
+    ref = _single_runs(models, 18)
+    for groups in ('one', 'each', 'pairs'):
+        solvers = [_hip()(G) for G in models]
+        # a model's grouped state: its bytes less the fused step's second field copy
+        state = [s.memory_bytes() - 6 * 4 * (G.nx + 1) * (G.ny + 1) * (G.nz + 1) for s, G in zip(solvers, models)]
+        group_bytes = {'one': 0, 'each': 1, 'pairs': max(state[0] + state[1], state[2] + state[3]) + 4096}[groups]
+        try:
+            with HipBatch(solvers) as b:
+                b.set_group_bytes(group_bytes)
+                b.run(0, 9, graph=False)
+                b.run(9, 9, graph=True)
+            for s, (F, rx) in zip(solvers, ref):
+                assert np.array_equal(s.download_fields(), F), groups
+                assert np.array_equal(s.download_rx(), rx), groups
+        finally:
+            for s in solvers:
+                s.close()
+
