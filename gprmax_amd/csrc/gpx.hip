@@ -1078,7 +1078,8 @@ extern "C" int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block)
 // dispatcher balances them.  Chunks are short (6 planes for the interior, 4
 // for the boundary, unless fixed by gpx_set_launch): on the 300x300x200
 // half-space these beat both longer chunks and a persistent grid of resident
-// blocks by 11 %, since fewer planes are streamed at once.  Side-by-side
+// blocks by 11 %, since fewer planes are streamed at once.  Grids too small
+// to fill the GPU with such items take shorter chunks.  Side-by-side
 // launches keep a persistent grid of their share of the resident blocks.
 static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, StepArgs *a)
 {
@@ -1104,11 +1105,16 @@ static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, St
     return a1 + a2 + bb;
 }
 
-static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs &a, int &grid)
+static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs &a, int &grid, int fill = 0)
 {
     const int NX1 = ctx->desc.nx + 1;
     int xc = ctx->xchunk ? ctx->xchunk : (bnd ? 4 : 6);
     if (xc > NX1) xc = NX1;
+    // Small grids: shorter chunks, down to 2 planes, until the items fill the
+    // `fill` blocks the GPU holds at once (on the 100^3 grid 2 planes beat 1:
+    // each chunk loads one neighbour plane more).
+    if (!ctx->xchunk)
+        while (xc > 2 && count_items(ctx, bnd, block, xc, nullptr) < fill) --xc;
     a.xc = xc;
     const long long items = count_items(ctx, bnd, block, xc, &a);
     long long g = resident > 0 && resident < items ? resident : items;
@@ -1288,12 +1294,13 @@ static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd
     size_t lds;
     base_args(ctx, field, a, block, lds);
     if (count_items(ctx, bnd, block, 1, nullptr) <= 0) return hipSuccess;
+    const int fill = resident_blocks(ctx, (const void *)kernel, block, lds);
     int resident = 0;  // 0: one block per item
     if (share < 1.0) {
-        resident = (int)(resident_blocks(ctx, (const void *)kernel, block, lds) * share) / 8 * 8;
+        resident = (int)(fill * share) / 8 * 8;
         if (resident < 8) resident = 8;
     }
-    plan(ctx, bnd, resident, block, a, grid);
+    plan(ctx, bnd, resident, block, a, grid, fill);
     a.flags = flags;
     a.it = it;
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream ? stream : ctx->stream, a);
