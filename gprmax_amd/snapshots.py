@@ -8,6 +8,9 @@ CPU semantics (the reference's GPU kernel indexes differently when xs != 0 or
 the step is not 1).
 """
 
+import struct
+import sys
+
 import numpy as np
 
 from .constants import floattype
@@ -36,3 +39,41 @@ class Snapshot:
         self.fields = tuple(np.asarray(a, dtype=floattype) for a in (Ex, Ey, Ez, Hx, Hy, Hz))
         self.electric = np.stack(self.fields[:3]).reshape(-1, order='F')
         self.magnetic = np.stack(self.fields[3:]).reshape(-1, order='F')
+
+    @property
+    def ncells(self):
+        return self.nx * self.ny * self.nz
+
+    def write_vtk_imagedata(self, pbar=None, G=None):
+        """Write the stored snapshot to ``self.filename`` as VTK ImageData
+        (.vti): cell data "E-field" and "H-field", three components each,
+        appended raw, each block preceded by its byte count as UInt32
+        (gprMax/snapshots.py:132-167).  `pbar` (optional) is advanced by the
+        bytes written; G gives the cell size."""
+        if self.electric is None:
+            raise ValueError('snapshot {} holds no data'.format(self.basefilename))
+        e = np.ascontiguousarray(self.electric)
+        h = np.ascontiguousarray(self.magnetic)
+        floatname = {4: 'Float32', 8: 'Float64'}[e.dtype.itemsize]
+        byteorder = 'LittleEndian' if sys.byteorder == 'little' else 'BigEndian'
+        datasize = e.nbytes
+        hoffset = datasize + 4
+        ext = '{} {} {} {} {} {}'.format(self.xs, round_value(self.xf / self.dx), self.ys,
+                                         round_value(self.yf / self.dy), self.zs, round_value(self.zf / self.dz))
+        head = ('<?xml version="1.0"?>\n'
+                '<VTKFile type="ImageData" version="1.0" byte_order="{}">\n'
+                '<ImageData WholeExtent="{}" Origin="0 0 0" Spacing="{:.3} {:.3} {:.3}">\n'
+                '<Piece Extent="{}">\n'
+                '<CellData Vectors="E-field H-field">\n'
+                '<DataArray type="{}" Name="E-field" NumberOfComponents="3" format="appended" offset="0" />\n'
+                '<DataArray type="{}" Name="H-field" NumberOfComponents="3" format="appended" offset="{}" />\n'
+                '</CellData>\n</Piece>\n</ImageData>\n<AppendedData encoding="raw">\n_').format(
+            byteorder, ext, self.dx * G.dx, self.dy * G.dy, self.dz * G.dz, ext, floatname, floatname, hoffset)
+        with open(self.filename, 'wb') as fh:
+            fh.write(head.encode('utf-8'))
+            for block in (e, h):
+                fh.write(struct.pack('I', datasize))
+                fh.write(block.tobytes())
+                if pbar is not None:
+                    pbar.update(n=4 + datasize)
+            fh.write('\n</AppendedData>\n</VTKFile>'.encode('utf-8'))

@@ -160,3 +160,38 @@ def test_groups_with_many_members(tmp_path):
     d = hdf5.read_file(path)
     for n, name in enumerate(names):
         assert np.array_equal(d['/rxs/' + name]['data'], np.full(3, n))
+
+
+def test_snapshot_vti_file(tmp_path):
+    """A .vti snapshot: the reference's XML header, then per field a UInt32
+    byte count and the cell data, three components per cell, x fastest."""
+    import re
+    import struct
+    from gprmax_amd.grid import ModelBuilder
+    b = ModelBuilder(12, 10, 8, 1e-3, 2e-3, 1e-3, 10, pml_cells=0)
+    snap = b.add_snapshot(2, 0, 2, 10, 10, 8, 2, 2, 3, 5, filename='snap')
+    rng = np.random.default_rng(9)
+    comps = [rng.standard_normal((snap.nx, snap.ny, snap.nz)).astype(np.float32) for _ in range(6)]
+    snap.set_from_components(*comps)
+    snap.filename = str(tmp_path / 'snap.vti')
+    snap.write_vtk_imagedata(None, b.G)
+    raw = open(snap.filename, 'rb').read()
+    head, _, rest = raw.partition(b'<AppendedData encoding="raw">\n_')
+    text = head.decode()
+    assert '<VTKFile type="ImageData" version="1.0" byte_order="LittleEndian">' in text
+    # the extent mixes start cells with end cells divided by the step, as the
+    # reference writes it (gprMax/snapshots.py:145-146)
+    assert 'WholeExtent="2 5 0 5 2 3"' in text and 'Spacing="0.002 0.004 0.003"' in text
+    n = snap.nx * snap.ny * snap.nz
+    assert (snap.nx, snap.ny, snap.nz) == (4, 5, 2)
+    offsets = [int(v) for v in re.findall(r'offset="(\d+)"', text)]
+    assert offsets == [0, 4 + 12 * n]
+    for f, off in enumerate(offsets):
+        size, = struct.unpack('I', rest[off:off + 4])
+        assert size == 12 * n
+        data = np.frombuffer(rest, dtype=np.float32, count=3 * n, offset=off + 4)
+        for c in range(3):
+            for (i, j, k) in ((0, 0, 0), (3, 1, 1), (2, 4, 0)):
+                assert data[c + 3 * (i + snap.nx * (j + snap.ny * k))] == comps[3 * f + c][i, j, k]
+    assert rest.endswith(b'\n</AppendedData>\n</VTKFile>')
+    assert len(rest) == 2 * (4 + 12 * n) + len(b'\n</AppendedData>\n</VTKFile>')
