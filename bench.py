@@ -210,6 +210,8 @@ def main():
     ap.add_argument('--tile', type=int, default=0, help='cells per fused block (0 = auto)')
     ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
     ap.add_argument('--overlap', action='store_true', help='H and E passes with interior and boundary kernels side by side')
+    ap.add_argument('--split', action='store_true',
+                    help='split step: one kernel doing H and E for the interior, boundary passes around it')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
@@ -237,8 +239,9 @@ def main():
     info = device_info(0)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
-        s.set_step(1 if args.fused else (2 if args.overlap else 0), args.tile)
+        s.set_step(1 if args.fused else (2 if args.overlap else (3 if args.split else 0)), args.tile)
         fused = s.step_kind()
+        form = s.step_form()
         s.run(0, W)
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
@@ -254,6 +257,12 @@ def main():
     mcells = cells * K / (ms * 1e-3) / 1e6
     if fused:
         kernels = {'step': (algorithmic_bytes_per_step(G), ms4[0])}
+    elif form == 3:
+        # the fused box is the interior box less its first plane, row and column
+        box3 = (box[0] + 1, box[1], box[2] + 1, box[3], box[4] + 1, box[5])
+        bh, be = kernel_bytes(G, 1, box3), kernel_bytes(G, 0, box3)
+        kernels = {'interior': (bh[0] + be[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
+                   'e_boundary': (be[1], ms4[3])}
     else:
         bh, be = kernel_bytes(G, 1, box), kernel_bytes(G, 0, box)
         kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
@@ -285,7 +294,8 @@ def main():
                      'kernels_ms': {k: round(v[1], 5) for k, v in kernels.items()},
                      'kernels_bytes': {k: v[0] for k, v in kernels.items()},
                      'step_achieved': round(step_gbs, 1),
-                     'step_form': 'fused, {} cells per block'.format(fused) if fused else 'two-pass',
+                     'step_form': ('fused, {} cells per block'.format(fused) if fused else
+                                   {3: 'split', 2: 'two-pass side by side'}.get(form, 'two-pass')),
                      'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),

@@ -35,6 +35,7 @@
 
 #include "gpx.h"
 #include "gpx_step.h"
+#include "gpx_interior.h"
 
 namespace gpx {
 
@@ -43,6 +44,7 @@ struct StepArgs {
     int plane;           // (ny+1)*(nz+1)
     int nseg, nitems, per_xcd, xc;  // work items: nseg plane segments x x-chunks of xc planes
     float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    float *So[3];        // where the pass writes its own field (the same arrays, or the next state)
     const void *ID;      // [6][ntot] of the compact index type
     long long ntot;
     const float *coef;   // [nmat][5]
@@ -183,6 +185,7 @@ __device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int str
     const IDT *ID = static_cast<const IDT *>(a.ID);
     const rsrc_t rEx = rsrc(a.Ex), rEy = rsrc(a.Ey), rEz = rsrc(a.Ez);
     const rsrc_t rHx = rsrc(a.Hx), rHy = rsrc(a.Hy), rHz = rsrc(a.Hz);
+    const rsrc_t wHx = rsrc(a.So[0]), wHy = rsrc(a.So[1]), wHz = rsrc(a.So[2]);
     const rsrc_t rI3 = rsrc(ID + 3 * N), rI4 = rsrc(ID + 4 * N), rI5 = rsrc(ID + 5 * N);
     const uint32_t P4 = (uint32_t)a.plane * 4u, Pid = (uint32_t)a.plane * (uint32_t)sizeof(IDT);
 
@@ -295,9 +298,9 @@ __device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int str
             }
 
             if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
-                bst(rHx, to, so, hx);
-                bst(rHy, to, so, hy);
-                bst(rHz, to, so, hz);
+                bst(wHx, to, so, hx);
+                bst(wHy, to, so, hy);
+                bst(wHz, to, so, hz);
             }
 
             if ((flags & F_RX) && (pf & PF_RX)) {  // the state at the start of the iteration
@@ -389,6 +392,7 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
     const IDT *ID = static_cast<const IDT *>(a.ID);
     const rsrc_t rEx = rsrc(a.Ex), rEy = rsrc(a.Ey), rEz = rsrc(a.Ez);
     const rsrc_t rHx = rsrc(a.Hx), rHy = rsrc(a.Hy), rHz = rsrc(a.Hz);
+    const rsrc_t wEx = rsrc(a.So[0]), wEy = rsrc(a.So[1]), wEz = rsrc(a.So[2]);
     const rsrc_t rI0 = rsrc(ID), rI1 = rsrc(ID + N), rI2 = rsrc(ID + 2 * N);
     const uint32_t P4 = (uint32_t)Pl * 4u, Pid = (uint32_t)Pl * (uint32_t)sizeof(IDT);
 
@@ -549,9 +553,9 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
                 if (dz_) t_store<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
             }
             if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
-                bst(rEx, to, so, ex);
-                bst(rEy, to, so, ey);
-                bst(rEz, to, so, ez);
+                bst(wEx, to, so, ex);
+                bst(wEy, to, so, ey);
+                bst(wEz, to, so, ez);
             }
             hy_m = hy;
             hz_m = hz;
@@ -662,6 +666,9 @@ struct gpx_ctx {
     int box[6] = {0, 0, 0, 0, 0, 0}; // interior box (see StepArgs)
     int *frame = nullptr;            // plane offsets outside the interior rectangle
     int nframe = 0;
+    int box3[6] = {0, 0, 0, 0, 0, 0};  // the split step's fused box: the interior box less its
+    int *frame3 = nullptr;              // first plane, row and column on the minus sides, and
+    int nframe3 = 0;                    // the plane offsets outside its rectangle
     int xchunk = 0, block = 0;
     int fused = 0;                   // 1: one fused kernel per step when eligible; 0: two passes;
                                      // 2: two passes, interior and boundary kernels side by side
@@ -812,6 +819,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->planeflag) hipFree(ctx->planeflag);
     if (ctx->rare) hipFree(ctx->rare);
     if (ctx->frame) hipFree(ctx->frame);
+    if (ctx->frame3) hipFree(ctx->frame3);
     for (auto &s : ctx->snaps) if (s.out) hipFree(s.out);
     if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
@@ -1055,7 +1063,7 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
-    if (!ctx || fused < 0 || fused > 2 ||
+    if (!ctx || fused < 0 || fused > 3 ||
         !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
@@ -1081,9 +1089,10 @@ extern "C" int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block)
 // blocks by 11 %, since fewer planes are streamed at once.  Grids too small
 // to fill the GPU with such items take shorter chunks.  Side-by-side
 // launches keep a persistent grid of their share of the resident blocks.
-static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, StepArgs *a)
+static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, StepArgs *a, bool f3 = false)
 {
-    const int *b = ctx->box;
+    const int *b = f3 ? ctx->box3 : ctx->box;
+    const int nframe = f3 ? ctx->nframe3 : ctx->nframe;
     auto ch = [&](int len) { return len > 0 ? (len + xc - 1) / xc : 0; };
     if (!bnd) {
         const long long cells = (long long)(b[3] - b[2]) * (b[5] - b[4]);
@@ -1093,7 +1102,7 @@ static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, St
         return items;
     }
     const int nseg_a = (ctx->plane + block - 1) / block;
-    const int nseg_b = (ctx->nframe + block - 1) / block;
+    const int nseg_b = (nframe + block - 1) / block;
     const long long a1 = (long long)nseg_a * ch(b[0]);
     const long long a2 = (long long)nseg_a * ch(ctx->desc.nx + 1 - b[1]);
     const long long bb = (long long)nseg_b * ch(b[1] - b[0]);
@@ -1105,7 +1114,8 @@ static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, St
     return a1 + a2 + bb;
 }
 
-static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs &a, int &grid, int fill = 0)
+static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs &a, int &grid, int fill = 0,
+                 bool f3 = false)
 {
     const int NX1 = ctx->desc.nx + 1;
     int xc = ctx->xchunk ? ctx->xchunk : (bnd ? 4 : 6);
@@ -1114,9 +1124,9 @@ static void plan(const gpx_ctx *ctx, bool bnd, int resident, int block, StepArgs
     // `fill` blocks the GPU holds at once (on the 100^3 grid 2 planes beat 1:
     // each chunk loads one neighbour plane more).
     if (!ctx->xchunk)
-        while (xc > 2 && count_items(ctx, bnd, block, xc, nullptr) < fill) --xc;
+        while (xc > 2 && count_items(ctx, bnd, block, xc, nullptr, f3) < fill) --xc;
     a.xc = xc;
-    const long long items = count_items(ctx, bnd, block, xc, &a);
+    const long long items = count_items(ctx, bnd, block, xc, &a, f3);
     long long g = resident > 0 && resident < items ? resident : items;
     g = (g + 7) / 8 * 8;
     if (g < 8) g = 8;
@@ -1236,19 +1246,47 @@ static int update_planeflags(gpx_ctx *ctx)
     if (!fr.empty()) {
         GPX_TRY(dalloc(ctx, (void **)&ctx->frame, fr.size() * 4, false));
         rc = copy_in(ctx, ctx->frame, fr.data(), fr.size() * 4);
+        if (rc) return rc;
+    }
+    // The split step's fused box: one more plane, row and column off on the
+    // minus sides, so that the neighbours its kernel recomputes are interior
+    // cells.  Empty when the interior box is too thin.
+    {
+        const int *b = ctx->box;
+        const bool ok = !empty && b[0] + 1 < b[1] && b[2] + 1 < b[3] && b[4] + 1 < b[5];
+        const int b3[6] = {ok ? b[0] + 1 : g.nx + 1, ok ? b[1] : g.nx + 1, ok ? b[2] + 1 : 0, ok ? b[3] : 0,
+                           ok ? b[4] + 1 : 0, ok ? b[5] : 0};
+        memcpy(ctx->box3, b3, sizeof(b3));
+        std::vector<int> f3;
+        if (ok)
+            for (int j = 0; j <= g.ny; ++j)
+                for (int k = 0; k <= g.nz; ++k)
+                    if (!(j >= b3[2] && j < b3[3] && k >= b3[4] && k < b3[5])) f3.push_back(j * (g.nz + 1) + k);
+        if (ctx->frame3) { hipFree(ctx->frame3); ctx->frame3 = nullptr; }
+        ctx->nframe3 = (int)f3.size();
+        if (!f3.empty()) {
+            GPX_TRY(dalloc(ctx, (void **)&ctx->frame3, f3.size() * 4, false));
+            rc = copy_in(ctx, ctx->frame3, f3.data(), f3.size() * 4);
+        }
     }
     return rc;
 }
 
-static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, size_t &lds)
+// f3: the boundary passes of the split step, around the fused box, from the
+// current state into the next: the H pass reads E and H from F and writes H
+// to Fn; the E pass reads E from F and the new H from Fn and writes E to Fn.
+static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, size_t &lds, bool f3 = false)
 {
     block = ctx->block ? ctx->block : 256;
     memset(&a, 0, sizeof(a));
     const gpx_desc &g = ctx->desc;
     a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
     a.plane = ctx->plane;
+    float *const *hin = (f3 && field == 0) ? ctx->Fn : ctx->F;
     a.Ex = ctx->F[0]; a.Ey = ctx->F[1]; a.Ez = ctx->F[2];
-    a.Hx = ctx->F[3]; a.Hy = ctx->F[4]; a.Hz = ctx->F[5];
+    a.Hx = hin[3]; a.Hy = hin[4]; a.Hz = hin[5];
+    float *const *out = f3 ? ctx->Fn : ctx->F;
+    for (int c = 0; c < 3; ++c) a.So[c] = out[field == 0 ? c : 3 + c];
     a.ID = ctx->ID;
     a.ntot = ctx->ntot;
     a.nmat = g.nmat;
@@ -1257,9 +1295,9 @@ static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, si
     a.nR = (int)ctx->Rhost[0].size();
     a.form = g.pml_formulation;
     a.planeflag = ctx->planeflag;
-    memcpy(a.box, ctx->box, sizeof(a.box));
-    a.frame = ctx->frame;
-    a.nframe = ctx->nframe;
+    memcpy(a.box, f3 ? ctx->box3 : ctx->box, sizeof(a.box));
+    a.frame = f3 ? ctx->frame3 : ctx->frame;
+    a.nframe = f3 ? ctx->nframe3 : ctx->nframe;
     a.iterations = g.iterations;
     a.nslab = (int)ctx->slabs.size();
     for (int s = 0; s < a.nslab; ++s) fill_slab_args(ctx, field, s, a.slab[s]);
@@ -1287,20 +1325,20 @@ typedef void (*pass_kernel)(StepArgs);
 // `share` < 1 plans the kernel for that fraction of the resident blocks, so
 // that another kernel can run beside it.
 static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd, int field, int flags, int it,
-                                hipStream_t stream = nullptr, double share = 1.0)
+                                hipStream_t stream = nullptr, double share = 1.0, bool f3 = false)
 {
     StepArgs a;
     int block, grid;
     size_t lds;
-    base_args(ctx, field, a, block, lds);
-    if (count_items(ctx, bnd, block, 1, nullptr) <= 0) return hipSuccess;
+    base_args(ctx, field, a, block, lds, f3);
+    if (count_items(ctx, bnd, block, 1, nullptr, f3) <= 0) return hipSuccess;
     const int fill = resident_blocks(ctx, (const void *)kernel, block, lds);
     int resident = 0;  // 0: one block per item
     if (share < 1.0) {
         resident = (int)(fill * share) / 8 * 8;
         if (resident < 8) resident = 8;
     }
-    plan(ctx, bnd, resident, block, a, grid, fill);
+    plan(ctx, bnd, resident, block, a, grid, fill, f3);
     a.flags = flags;
     a.it = it;
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream ? stream : ctx->stream, a);
@@ -1580,6 +1618,65 @@ static hipError_t launch_step(gpx_ctx *ctx, int tc, int it)
     return hipSuccess;
 }
 
+// The split step (form 3): the fused interior kernel over the fused box and
+// the boundary H and E passes around it, from the current state into the
+// next.  It needs no dispersive poles (the fused kernel has no auxiliary
+// values), a nonempty fused box and 32-bit buffer offsets.
+static bool split_ok(const gpx_ctx *ctx)
+{
+    if (ctx->fused != 3 || ctx->desc.maxpoles > 0) return false;
+    const int *b = ctx->box3;
+    if (b[0] >= b[1] || b[2] >= b[3] || b[4] >= b[5]) return false;
+    return (double)ctx->fstride * 24.0 < 4294967295.0 && (double)ctx->ntot * 6 * ctx->id_bytes < 4294967295.0;
+}
+
+static hipError_t launch_interior(const gpx_ctx *ctx, int it)
+{
+    IntArgs a;
+    memset(&a, 0, sizeof(a));
+    const gpx_desc &g = ctx->desc;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+    a.plane = ctx->plane;
+    memcpy(a.box, ctx->box3, sizeof(a.box));
+    const int B = 256;
+    const long long cells = (long long)(a.box[3] - a.box[2]) * (a.box[5] - a.box[4]);
+    a.nseg = (int)((cells + B - 1) / B);
+    const int nxb = a.box[1] - a.box[0];
+    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior<uint16_t> : (const void *)k_interior<uint32_t>;
+    const size_t lds = (size_t)g.nmat * 40;
+    int xc = ctx->xchunk ? ctx->xchunk : 6;
+    if (!ctx->xchunk) {
+        const int fill = resident_blocks(ctx, k, B, lds);
+        while (xc > 2 && (long long)a.nseg * ((nxb + xc - 1) / xc) < fill) --xc;
+    }
+    if (xc > nxb) xc = nxb;
+    a.xc = xc;
+    a.nitems = a.nseg * ((nxb + xc - 1) / xc);
+    const int grid = (a.nitems + 7) / 8 * 8;
+    a.per_xcd = grid / 8;
+    a.Fc = ctx->F[0];
+    a.Fx = ctx->Fn[0];
+    a.cs = (uint32_t)(ctx->fstride * 4);
+    a.ID = ctx->ID;
+    a.ntot = ctx->ntot;
+    a.coefE = ctx->cE;
+    a.coefH = ctx->cH;
+    a.nmat = g.nmat;
+    a.it = it;
+    a.iterations = g.iterations;
+    const SrcSet &M = ctx->src[GPX_SRC_MAGNETIC], &V = ctx->src[GPX_SRC_VOLTAGE], &D = ctx->src[GPX_SRC_HERTZIAN];
+    a.srcM = {M.n, M.info, M.fac, M.waves};
+    a.srcV = {V.n, V.info, V.fac, V.waves};
+    a.srcD = {D.n, D.info, D.fac, D.waves};
+    a.nrx = ctx->nrx;
+    a.rxc = ctx->rxc;
+    a.rxbuf = ctx->rxbuf;
+    a.planeflag = ctx->planeflag;
+    if (ctx->id_bytes == 2) hipLaunchKernelGGL(k_interior<uint16_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
+    else hipLaunchKernelGGL(k_interior<uint32_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
+    return hipGetLastError();
+}
+
 static hipError_t launch_snapshot(const gpx_ctx *ctx, int s)
 {
     const Snap &S = ctx->snaps[s];
@@ -1624,6 +1721,7 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
     const int hf = h_flags(ctx), ef = e_flags(ctx);
     const int tc = step_tile(ctx);
+    const bool split = !tc && split_ok(ctx);
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double acc[4] = {0, 0, 0, 0};
     if (ms4)
@@ -1640,6 +1738,27 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
                 float t;
                 GPX_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
                 acc[0] += t;
+            }
+            continue;
+        }
+        if (split) {
+            if (ms4) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            GPX_TRY(launch_interior(ctx, it));
+            if (ms4) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, nullptr, 1.0, true));
+            if (ms4) GPX_TRY(hipEventRecord(ev[2], ctx->stream));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, 1.0, true));
+            for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
+            if (ms4) {
+                GPX_TRY(hipEventRecord(ev[3], ctx->stream));
+                GPX_TRY(hipEventSynchronize(ev[3]));
+                float t[3];
+                GPX_TRY(hipEventElapsedTime(&t[0], ev[0], ev[1]));
+                GPX_TRY(hipEventElapsedTime(&t[1], ev[1], ev[2]));
+                GPX_TRY(hipEventElapsedTime(&t[2], ev[2], ev[3]));
+                acc[0] += t[0];
+                acc[1] += t[1];
+                acc[3] += t[2];
             }
             continue;
         }
@@ -1670,6 +1789,14 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 extern "C" int gpx_step_kind(const gpx_ctx *ctx)
 {
     return ctx ? step_tile(ctx) : 0;
+}
+
+extern "C" int gpx_step_form(const gpx_ctx *ctx)
+{
+    if (!ctx) return GPX_EARG;
+    if (step_tile(ctx)) return 1;
+    if (split_ok(ctx)) return 3;
+    return ctx->fused == 2 ? 2 : 0;
 }
 
 extern "C" int gpx_run(gpx_ctx *ctx, int it0, int n)

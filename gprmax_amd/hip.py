@@ -50,6 +50,7 @@ SIGNATURES = {
     'gpx_set_launch': ([_P, _I, _I], _I),
     'gpx_set_step': ([_P, _I, _I], _I),
     'gpx_step_kind': ([_P], _I),
+    'gpx_step_form': ([_P], _I),
     'gpx_synchronize': ([_P], _I),
     'gpx_download_rx': ([_P, _P], _I),
     'gpx_download_fields': ([_P, _P], _I),
@@ -176,8 +177,12 @@ class HipSolver:
         self._check(self.lib.gpx_set_step(self.ctx, int(fused), int(tile_cells)))
 
     def step_kind(self):
-        """Cells per fused block, or 0 when the H and E passes run."""
+        """Cells per fused block, or 0 when another form runs."""
         return int(self.lib.gpx_step_kind(self.ctx))
+
+    def step_form(self):
+        """The form gpx_run uses: 0 passes, 1 fused, 2 side by side, 3 split."""
+        return int(self.lib.gpx_step_form(self.ctx))
 
     def run(self, it0=0, n=None):
         n = self.G.iterations - it0 if n is None else n

@@ -132,15 +132,23 @@ int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
  * H and the E update with the state double-buffered on the device; fused = 0
  * (default): an H pass and an E pass per iteration; fused = 2: the passes
  * with their interior and boundary kernels running side by side on two
- * streams (gpx_run_profiled always runs them one after the other).  tile_cells = cells per
- * fused block (512, 1024 or 2048; 0 = auto).  Both forms give identical
- * results; the fused form needs a tile of at least twice nz+1 cells within
- * the LDS budget, and the passes run when it does not fit. */
+ * streams (gpx_run_profiled always runs them one after the other); fused = 3:
+ * the split step, one kernel doing H and E for the interior and the H and E
+ * boundary passes around it, double-buffered.  tile_cells = cells per fused
+ * block (256, 512 or 1024; 0 = auto).  All forms give identical results; the
+ * fused form needs a tile of at least twice nz+1 cells within the LDS budget,
+ * the split step a model without dispersive poles, and the passes run when
+ * the chosen form does not apply. */
 int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells);
 
 /* Cells per block of the fused step that gpx_run will use, or 0 when it will
- * run the H and E passes. */
+ * run another form. */
 int gpx_step_kind(const gpx_ctx *ctx);
+
+/* The form gpx_run will use: 0 passes, 1 fused step, 2 passes side by side,
+ * 3 split step.  gpx_run_profiled reports the split step's interior kernel in
+ * ms4[0] and its boundary passes in ms4[1] and ms4[3]. */
+int gpx_step_form(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);
 

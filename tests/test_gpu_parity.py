@@ -158,7 +158,16 @@ def _compare_runs(F, rxb, snaps, phis, T, Gc):
             assert np.array_equal(T[n], getattr(Gc, name)), name
 
 
-@pytest.mark.parametrize('fused', [1, 0, 2], ids=['fused-step', 'two-pass', 'side-by-side'])
+FORMS = dict(argvalues=[1, 0, 2, 3], ids=['fused-step', 'two-pass', 'side-by-side', 'split'])
+
+
+def _expected_form(fused, G):
+    if fused == 3:
+        return 3 if G.maxpoles == 0 else 0
+    return fused
+
+
+@pytest.mark.parametrize('fused', **FORMS)
 @pytest.mark.parametrize('case', list(CASES))
 def test_fused_run_matches_oracle(case, fused):
     """All iterations, as one fused kernel per step or as an H and an E pass:
@@ -168,6 +177,7 @@ def test_fused_run_matches_oracle(case, fused):
     with _hip()(G) as s:
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
+        assert s.step_form() == _expected_form(fused, G)
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -178,6 +188,30 @@ def test_fused_run_matches_oracle(case, fused):
     _compare_runs(F, rxb, snaps, phis, T, Gc)
 
 
+@pytest.mark.parametrize('xchunk', [0, 1, 2, 5])
+@pytest.mark.parametrize('case', ['horipml1', 'mripml2'])
+def test_split_step_from_random_state(case, xchunk):
+    """The split step from a random state (fields and Phi) on a grid with
+    several blocks per plane and short chunks: state and receiver samples
+    equal to the oracle stepped the same number of iterations."""
+    G = rich_model(seed=22, shape=WIDE, iterations=12, snapshots=False, **CASES[case])
+    _random_state(G, 6)
+    lib = oracle.load(np.float32)
+    with _hip()(G) as s:
+        s.set_step(3)
+        s.set_launch(xchunk, 0)
+        assert s.step_form() == 3
+        _push_state(s, G)
+        s.run(2, 6)
+        for it in range(2, 8):
+            osolver.step(lib, G, it, 8)
+        _assert_state_equal(s, G, 'split xchunk {}'.format(xchunk))
+        buf = s.download_rx()
+    for r, rx in enumerate(G.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(buf[c, 2:8, r], rx.outputs[name][2:8]), (r, name)
+
+
 PLANE_CASES = {
     'horipml1': dict(order=1),
     'mripml2': dict(order=2, formulation='MRIPML'),
@@ -186,7 +220,7 @@ PLANE_CASES = {
 }
 
 
-@pytest.mark.parametrize('fused', [1, 0, 2], ids=['fused-step', 'two-pass', 'side-by-side'])
+@pytest.mark.parametrize('fused', **FORMS)
 @pytest.mark.parametrize('axis', [0, 1, 2], ids=['TMx', 'TMy', 'TMz'])
 @pytest.mark.parametrize('case', list(PLANE_CASES))
 def test_2d_run_matches_oracle(case, axis, fused):
@@ -198,6 +232,7 @@ def test_2d_run_matches_oracle(case, axis, fused):
     with _hip()(G) as s:
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
+        assert s.step_form() == (0 if fused == 3 else fused)   # a 2-D grid has no fused box
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -266,17 +301,20 @@ def test_fused_step_from_random_state(case, tile, xchunk):
             assert np.array_equal(buf[c, 2:8, r], rx.outputs[name][2:8]), (r, name)
 
 
-def test_fused_and_two_pass_agree_on_a_wide_grid():
-    """The two step forms give the same bits on a grid with several tiles."""
+@pytest.mark.parametrize('poles', [1, 0])
+def test_step_forms_agree_on_a_wide_grid(poles):
+    """All step forms give the same bits on a grid with several tiles and
+    several blocks per plane."""
     out = []
-    for fused in (1, 0):
-        G = rich_model(seed=8, shape=WIDE, iterations=30, poles=1, order=2)
+    for fused in (1, 0, 2, 3):
+        G = rich_model(seed=8, shape=WIDE, iterations=30, poles=poles, order=2)
         with _hip()(G) as s:
             s.set_step(fused)
             s.run(0, G.iterations)
             out.append((s.download_fields(), s.download_rx()))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for F, rx in out[1:]:
+        assert np.array_equal(out[0][0], F)
+        assert np.array_equal(out[0][1], rx)
 
 
 @pytest.mark.parametrize('name', ['hertzian_dipole_fs', 'hertzian_dipole_dispersive', 'magnetic_dipole_fs'])
