@@ -127,6 +127,41 @@ def bench_bscan(args):
     return 0 if same else 1
 
 
+def bench_edges(args):
+    """Edge material indices of a 300x300x200 grid whose lower half is an
+    averaged random mixture of 6 soils (every soil edge needs an averaged
+    material): the GPU build against the host build, same result required."""
+    import numpy as np
+    from gprmax_amd.edges import build_components_on_device
+    from gprmax_amd.grid import ModelBuilder, build_electric_components, build_magnetic_components
+
+    def model():
+        rng = np.random.default_rng(1006)
+        nx, ny, nz = 300, 300, 200
+        b = ModelBuilder(nx, ny, nz, 2e-3, 2e-3, 2e-3, 10, pml_cells=0)
+        for n in range(6):
+            b.add_material(2 + 0.5 * n, 1e-3 * n, 1, 0, 'soil{}'.format(n))
+        b.G.solid[:, :, :nz // 2] = rng.integers(2, 8, (nx, ny, nz // 2))
+        return b.G
+    G = model()
+    t0 = time.perf_counter()
+    build_components_on_device(G)
+    t_dev = time.perf_counter() - t0
+    G2 = model()
+    t0 = time.perf_counter()
+    build_electric_components(G2)
+    build_magnetic_components(G2)
+    t_host = time.perf_counter() - t0
+    same = bool(np.array_equal(G.ID, G2.ID)) and [m.ID for m in G.materials] == [m.ID for m in G2.materials]
+    line = {'metric': 'edge material build, host time / GPU time', 'value': round(t_host / t_dev, 2),
+            'unit': 'x', 'n_gpus': 1, 'steps': 1, 'warmup': 0, 'higher_is_better': True, 'scaling': 'strong',
+            'vs_baseline': None, 'dtype': 'u32', 'data': 'synthetic',
+            'config': {'workload': 'averaged random 6-soil half-space 300x300x200', 'materials': len(G.materials)},
+            'gpu_s': round(t_dev, 3), 'host_s': round(t_host, 3), 'identical': same}
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
 def bench_adjoint(args):
     """Config of the differentiable solve: 160x160x120 cells at 2 mm, 10-cell
     PML, smooth true model (er in [4, 10], sigma in [1e-3, 2e-2]),
@@ -227,6 +262,8 @@ def main():
         return bench_bscan(args)
     if args.workload == 'adjoint':
         return bench_adjoint(args)
+    if args.workload == 'edges':
+        return bench_edges(args)
     from gprmax_amd.hip import HipSolver, device_info
     from gprmax_amd.workloads import WORKLOADS, algorithmic_bytes_per_step, kernel_bytes
 

@@ -305,9 +305,10 @@ class ModelBuilder:
                     np.copyto(view, numids, where=mask)
 
     # finalise ---------------------------------------------------------------
-    def finalise(self):
+    def finalise(self, edge_device=None):
         """PMLs, edge build, voltage-source materials, coefficients, receivers
-        (gprMax/model_build_run.py:187-281)."""
+        (gprMax/model_build_run.py:187-281).  With `edge_device` (a GPU
+        index) the edge indices are built on that GPU (gprmax_amd.edges)."""
         G = self.G
         if any(v > 0 for v in G.pmlthickness.values()):
             if not G.cfs:
@@ -315,8 +316,12 @@ class ModelBuilder:
             build_pmls(G)
         else:
             G.pmls = []
-        build_electric_components(G)
-        build_magnetic_components(G)
+        if edge_device is None:
+            build_electric_components(G)
+            build_magnetic_components(G)
+        else:
+            from .edges import build_components_on_device
+            build_components_on_device(G, edge_device)
         pec_invariant_axis(G)
         for vs in G.voltagesources:
             vs.create_material(G)
@@ -381,17 +386,41 @@ def _edge_build(G, comp, ranges, cell_offsets, rigidcells, electric):
     target = G.ID[comp, i0:i1, j0:j1, k0:k1]
     uniform = same & ~rig
     target[uniform] = sl[0][uniform]
-    mixed = np.argwhere(~same & ~rig)  # C order = the reference's loop order
-    if mixed.size == 0:
+    mask = ~same & ~rig
+    if not mask.any():
         return
+    # The cell materials of each mixed edge in the reference's numID1.. order,
+    # edges in C order (the reference's loop order).
+    ids = np.stack([s[mask] for s in sl], axis=1).astype(np.int64)
     cache = {}
-    names_of = [m.ID for m in G.materials]
-    for (a, b, c) in mixed:
-        ids = [int(s[a, b, c]) for s in sl]
-        if len(names_of) != len(G.materials):
-            names_of = [m.ID for m in G.materials]
-        names = [names_of[n] for n in ids]
-        target[a, b, c] = _average_material(G, names, ids, cache, electric)
+    if any('+' in m.ID for m in G.materials[:int(ids.max()) + 1]):
+        # a name holding '+' makes the match depend on the constituents'
+        # order: resolve edge by edge, as the reference does
+        out = np.empty(len(ids), dtype=target.dtype)
+        for e, row in enumerate(ids.tolist()):
+            names = [G.materials[n].ID for n in row]
+            out[e] = _average_material(G, names, row, cache, electric)
+        target[mask] = out
+        return
+    # Edges holding the same materials in any order resolve to the same
+    # material: the match counts each name in the '+'-joined names, which for
+    # names without '+' does not depend on their order.  Each distinct set is
+    # resolved once, in the order of its first edge (so materials are created
+    # in the reference's order, named and averaged from that edge's order).
+    key = np.sort(ids, axis=1)
+    if key.max() < 1 << 16:
+        packed = np.zeros(len(key), dtype=np.uint64)
+        for col in range(key.shape[1]):
+            packed = (packed << np.uint64(16)) | key[:, col].astype(np.uint64)
+        _, first, inverse = np.unique(packed, return_index=True, return_inverse=True)
+    else:
+        _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    resolved = np.empty(len(first), dtype=target.dtype)
+    for u in np.argsort(first, kind='stable'):
+        row = ids[first[u]].tolist()
+        names = [G.materials[n].ID for n in row]
+        resolved[u] = _average_material(G, names, row, cache, electric)
+    target[mask] = resolved[inverse.ravel()]
 
 
 def pec_invariant_axis(G):

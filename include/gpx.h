@@ -250,6 +250,34 @@ int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, float *gCB, flo
  * included), ms. */
 int gpx_inv_timings(const gpx_inv *v, float *ms_forward, float *ms_backward);
 
+/* ---- Edge material indices on the device.
+ *
+ * Replaces the edge loops of build_electric_components and
+ * build_magnetic_components (gprMax/yee_cell_build_ext.pyx:110-257; called at
+ * gprMax/model_build_run.py:215-219) for grids whose cell materials are below
+ * 65535.  solid uint32 [nx][ny][nz] (cell materials), rigidE / rigidH uint8
+ * [nx][ny][nz] (cells whose edges the geometry fixed), ID uint32
+ * [6][nx+1][ny+1][nz+1] (the indices the geometry wrote).  Per field (0: E,
+ * then 1: H, the reference's order): gpx_edges_classify writes the edges whose
+ * cells agree and returns the distinct sets of cell materials of the others
+ * (materials sorted and packed 16 bits each, with the first edge of each set
+ * as component * (nx+1)(ny+1)(nz+1) + point index); the caller finds or
+ * creates each set's averaged material in first-edge order
+ * (create_electric_average / create_magnetic_average,
+ * gprMax/yee_cell_build_ext.pyx:31-107) and passes them, in the order the sets
+ * came, to gpx_edges_resolve, which writes them to every edge of the set.
+ * `capacity` bounds the distinct sets per field (GPX_ELIMIT beyond). */
+typedef struct gpx_edges gpx_edges;
+
+int gpx_edges_create(int device, int nx, int ny, int nz, const uint32_t *solid, const uint8_t *rigidE,
+                     const uint8_t *rigidH, const uint32_t *ID, int capacity, gpx_edges **out);
+int gpx_edges_destroy(gpx_edges *b);
+const char *gpx_edges_last_error(const gpx_edges *b);
+int gpx_edges_classify(gpx_edges *b, int field, unsigned long long *keys, unsigned long long *first, int max,
+                       int *count);
+int gpx_edges_resolve(gpx_edges *b, int field, const uint32_t *numids, int count);
+int gpx_edges_download(gpx_edges *b, uint32_t *ID);
+
 #ifdef __cplusplus
 }
 #endif

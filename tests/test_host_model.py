@@ -239,3 +239,59 @@ def test_axis_aligned_cylinder_matches_brute_force():
         assert np.array_equal(got, want), axis
         # a rigid cylinder writes every edge of its cells
         assert (b.G.ID[axis][tuple(np.argwhere(want)[0])] == m.numID)
+
+
+def _loop_edge_build(G):
+    """The reference's edge loops (gprMax/yee_cell_build_ext.pyx:110-257),
+    one edge at a time in loop order, for comparison with the vectorised
+    build."""
+    from gprmax_amd.grid import _average_material
+    nx, ny, nz = G.nx, G.ny, G.nz
+    specs = [(0, ((0, nx), (1, ny), (1, nz)), ((0, 0, 0), (0, -1, 0), (0, -1, -1), (0, 0, -1)), True),
+             (1, ((1, nx), (0, ny), (1, nz)), ((0, 0, 0), (-1, 0, 0), (-1, 0, -1), (0, 0, -1)), True),
+             (2, ((1, nx), (1, ny), (0, nz)), ((0, 0, 0), (-1, 0, 0), (-1, -1, 0), (0, -1, 0)), True),
+             (3, ((1, nx), (0, ny), (0, nz)), ((0, 0, 0), (-1, 0, 0)), False),
+             (4, ((0, nx), (1, ny), (0, nz)), ((0, 0, 0), (0, -1, 0)), False),
+             (5, ((0, nx), (0, ny), (1, nz)), ((0, 0, 0), (0, 0, -1)), False)]
+    for comp, ((i0, i1), (j0, j1), (k0, k1)), offs, electric in specs:
+        rigid = G.rigidcellE if electric else G.rigidcellH
+        cache = {}
+        for i in range(i0, i1):
+            for j in range(j0, j1):
+                for k in range(k0, k1):
+                    cells = [(i + a, j + b, k + c) for a, b, c in offs]
+                    if any(rigid[c] for c in cells):
+                        continue
+                    ids = [int(G.solid[c]) for c in cells]
+                    if all(n == ids[0] for n in ids):
+                        G.ID[comp, i, j, k] = ids[0]
+                    else:
+                        names = [G.materials[n].ID for n in ids]
+                        G.ID[comp, i, j, k] = _average_material(G, names, ids, cache, electric)
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+def test_vectorised_edge_build_matches_edge_loop(seed):
+    """Averaged random media with rigid voxels and an averaged box: the same
+    edge indices and the same averaged materials, created in the same order
+    with the same names and properties, as the edge-by-edge loop."""
+    from gprmax_amd.grid import build_electric_components, build_magnetic_components
+
+    def model():
+        rng = np.random.default_rng(seed)
+        b = ModelBuilder(14, 12, 11, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+        mats = [b.add_material(2 + 0.7 * n, 0.002 * n, 1, 0, 'm{}'.format(n)) for n in range(5)]
+        b.G.solid[:, :, :7] = rng.integers(2, 7, (14, 12, 7))
+        b.box(3, 9, 2, 8, 5, 9, mats[1], averaging=True)
+        mask = rng.random((14, 12, 11)) < 0.08
+        b.paint_voxels(mask, rng.integers(0, 7, (14, 12, 11)).astype(np.uint32))
+        return b.G
+    G1, G2 = model(), model()
+    build_electric_components(G1)
+    build_magnetic_components(G1)
+    _loop_edge_build(G2)
+    assert np.array_equal(G1.ID, G2.ID)
+    assert [m.ID for m in G1.materials] == [m.ID for m in G2.materials]
+    assert len(G1.materials) > 30
+    for a, b in zip(G1.materials, G2.materials):
+        assert (a.er, a.se, a.mr, a.sm) == (b.er, b.se, b.mr, b.sm)

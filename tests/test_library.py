@@ -18,7 +18,8 @@ def declared_functions():
 def test_header_and_binding_agree():
     from gprmax_amd.hip import SIGNATURES
     from gprmax_amd.inversion import SIGNATURES as INV
-    assert declared_functions() == sorted(set(SIGNATURES) | set(INV))
+    from gprmax_amd.edges import SIGNATURES as EDGES
+    assert declared_functions() == sorted(set(SIGNATURES) | set(INV) | set(EDGES))
 
 
 def test_library_exports_every_declared_symbol():
