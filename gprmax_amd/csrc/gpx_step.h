@@ -94,11 +94,19 @@ __device__ __forceinline__ void st(T *base, uint32_t boff, T v)
 // [order][t] in LDS; i1/off index Phi order 0 and the order-1 offset.  Phi is
 // read from `pin` and, when `wr`, the advanced values are written to `pout`
 // (the same array for an in-place update).
+// Phi pointers come from slab descriptions staged in LDS, so the compiler
+// cannot tell their address space; Phi is always device memory, and global
+// loads and stores avoid the flat path (which waits on both LDS and memory
+// counters).
+typedef __attribute__((address_space(1))) float gfloat_t;
+
 template <int FORM, int ORDER>
 __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p, float F,
-                                       bool plus_sign, float c, float dd, const float *pin,
-                                       float *pout, bool wr, uint32_t i1, uint32_t off)
+                                       bool plus_sign, float c, float dd, const float *pin_,
+                                       float *pout_, bool wr, uint32_t i1, uint32_t off)
 {
+    const gfloat_t *pin = (const gfloat_t *)pin_;
+    gfloat_t *pout = (gfloat_t *)pout_;
     const int ot = ORDER * s.t;
     float corr;
     if (FORM == 0) {
