@@ -229,7 +229,7 @@ __device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int str
         if (!BND) load(c.xa, cur);
         for (int i = c.xa; i < c.xb; ++i) {
             const uint32_t so = (uint32_t)i * P4;
-            const uint32_t pf = a.planeflag[i];
+            const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
             if (BND) load(i, cur);
             else load(i + 1 < c.xb ? i + 1 : i, nxt);
             const float ex = cur.ex, ey1 = cur.ey1, ez1 = cur.ez1;
@@ -431,7 +431,7 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
         for (int i = c.xa; i < c.xb; ++i) {
             const uint32_t so = (uint32_t)i * P4;
             const size_t po = (size_t)i * Pl;
-            const uint32_t pf = a.planeflag[i];
+            const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
             if (BND || P > 0) load(i, cur);
             else load(i + 1 < c.xb ? i + 1 : i, nxt);
             const float hx = cur.hx, hy = cur.hy, hz = cur.hz;

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
         const float exk = bld(rF, tk, so + C(EX)), exj = bld(rF, tj, so + C(EX));
         const float ey1 = bld(rF, to, sn + C(EY)), ez1 = bld(rF, to, sn + C(EZ));
         const uint32_t m4 = bld_id<IDT>(rI, tid, si + 4 * ci), m5 = bld_id<IDT>(rI, tid, si + 5 * ci);
-        const uint32_t pf = a.planeflag[ih];
+        const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)ih);
         float4 qy = cH[m4], qz = cH[m5];
         hym = qy.x * hy0 - qy.w * (exk - exh) + qy.y * (ez1 - ezh);
         hzm = qz.x * hz0 - qz.y * (ey1 - eyh) + qz.z * (exj - exh);
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
     PL cur;
     for (int i = xa; i < xb; ++i) {
         const uint32_t so = (uint32_t)i * P4;
-        const uint32_t pf = a.planeflag[i];
+        const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
         load(i, cur);
 
         // new H of the cell

@@ -10,6 +10,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gpx {
 
 constexpr int MAX_SLABS = 6;
@@ -80,12 +82,26 @@ __device__ __forceinline__ uint32_t bld_id(rsrc_t r, uint32_t voff, uint32_t sof
 template <class T>
 __device__ __forceinline__ T ld(const T *base, uint32_t boff)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+    // Device memory: global, not flat, accesses (the pointer may have been
+    // read from memory, which hides its address space from the compiler).
+    // The value moves as a 4- or 8-byte word.
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+    typedef typename std::conditional<sizeof(T) == 4, uint32_t, unsigned long long>::type W;
+    typedef __attribute__((address_space(1))) const W gW;
+    const W w = *(gW *)(reinterpret_cast<const char *>(base) + boff);
+    T v;
+    __builtin_memcpy(&v, &w, sizeof(T));
+    return v;
 }
 template <class T>
 __device__ __forceinline__ void st(T *base, uint32_t boff, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+    typedef typename std::conditional<sizeof(T) == 4, uint32_t, unsigned long long>::type W;
+    typedef __attribute__((address_space(1))) W gW;
+    W w;
+    __builtin_memcpy(&w, &v, sizeof(T));
+    *(gW *)(reinterpret_cast<char *>(base) + boff) = w;
 }
 
 // ---------------------------------------------------------------- PML
@@ -513,7 +529,7 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
 
         for (int i = xs; i < xb; ++i) {
             const bool hp = i < xa;  // halo plane: H only, nothing written
-            const uint32_t pf = a.planeflag[i];
+            const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
             const uint32_t so = (uint32_t)i * P4;
 
             float ex1[CPT], ey1[CPT], ez1[CPT], hx[CPT], hy[CPT], hz[CPT];
