@@ -87,7 +87,10 @@ def bench_bscan(args):
         cells += G.nx * G.ny * G.nz
         t0 = time.perf_counter()
         with HipSolver(G) as s:
-            s.set_step(1 if args.fused else 0)
+            if args.fused:
+                s.set_step(1)
+            elif args.passes:
+                s.set_step(0)
             ms, _, _ = s.run_timed(0, G.iterations)
             plain_rx.append(s.download_rx())
         plain_wall += time.perf_counter() - t0
@@ -120,7 +123,8 @@ def bench_bscan(args):
         'kernel_ratio': round(plain_dev / (ms * 1e-3), 3),
         'batched_mcells_per_s': round(cells * args.bscan_its / (ms * 1e-3) / 1e6, 1),
         'tuned_shape': {'tile_cells': tile, 'xchunk': xchunk, 'ms_per_iteration': round(ms_tune, 4)},
-        'plain_step_form': 'fused' if args.fused else 'two-pass',
+        'plain_step_form': 'fused' if args.fused else ('two-pass' if args.passes else
+                                                       ('split' if args.split else 'default (timed choice)')),
         'receivers_identical': same,
     }
     print(json.dumps(line), flush=True)
@@ -245,8 +249,10 @@ def main():
     ap.add_argument('--tile', type=int, default=0, help='cells per fused block (0 = auto)')
     ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
     ap.add_argument('--overlap', action='store_true', help='H and E passes with interior and boundary kernels side by side')
+    ap.add_argument('--passes', action='store_true', help='the H and E passes (step form 0)')
     ap.add_argument('--split', action='store_true',
-                    help='split step: one kernel doing H and E for the interior, boundary passes around it')
+                    help='the split step (form 3): one kernel doing H and E for the interior, boundary passes '
+                         'around it; by default the library times it against the passes and keeps the faster')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
@@ -276,13 +282,14 @@ def main():
     info = device_info(0)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
-        s.set_step(1 if args.fused else (2 if args.overlap else (3 if args.split else 0)), args.tile)
+        s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (3 if args.split else 4))),
+                   args.tile)
         fused = s.step_kind()
-        form = s.step_form()
         s.run(0, W)
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
         s.synchronize()
+        form = s.step_form()   # after the runs: the default form times two forms on its first long run
         # Per-kernel durations (HIP events around every launch on the solver's
         # stream), in a separate run after the timed region.
         n_prof = min(50, G.iterations - W - K)
@@ -332,7 +339,10 @@ def main():
                      'kernels_bytes': {k: v[0] for k, v in kernels.items()},
                      'step_achieved': round(step_gbs, 1),
                      'step_form': ('fused, {} cells per block'.format(fused) if fused else
-                                   {3: 'split', 2: 'two-pass side by side'}.get(form, 'two-pass')),
+                                   {3: 'split', 2: 'two-pass side by side'}.get(form, 'two-pass')) +
+                                  (' (timed against the other and kept)' if not (args.fused or args.overlap or
+                                                                                   args.passes or args.split)
+                                   else ''),
                      'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),

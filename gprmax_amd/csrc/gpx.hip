@@ -670,8 +670,9 @@ struct gpx_ctx {
     int *frame3 = nullptr;              // first plane, row and column on the minus sides, and
     int nframe3 = 0;                    // the plane offsets outside its rectangle
     int xchunk = 0, block = 0;
-    int fused = 0;                   // 1: one fused kernel per step when eligible; 0: two passes;
-                                     // 2: two passes, interior and boundary kernels side by side
+    int fused = 4;                   // step form (gpx_set_step): 0 passes, 1 fused step, 2 passes side
+                                     // by side, 3 split step, 4 (default) the faster of 3 and 0
+    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step
     hipStream_t stream2 = nullptr;   // second stream of the side-by-side form
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int tile_cells = 0;              // cells per fused block (0 = auto)
@@ -1063,11 +1064,12 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
-    if (!ctx || fused < 0 || fused > 3 ||
+    if (!ctx || fused < 0 || fused > 4 ||
         !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
     ctx->tile_cells = tile_cells;
+    ctx->split_choice = -1;
     return 0;
 }
 
@@ -1622,12 +1624,18 @@ static hipError_t launch_step(gpx_ctx *ctx, int tc, int it)
 // the boundary H and E passes around it, from the current state into the
 // next.  It needs no dispersive poles (the fused kernel has no auxiliary
 // values), a nonempty fused box and 32-bit buffer offsets.
-static bool split_ok(const gpx_ctx *ctx)
+static bool split_eligible(const gpx_ctx *ctx)
 {
-    if (ctx->fused != 3 || ctx->desc.maxpoles > 0) return false;
+    if (ctx->desc.maxpoles > 0) return false;
     const int *b = ctx->box3;
     if (b[0] >= b[1] || b[2] >= b[3] || b[4] >= b[5]) return false;
     return (double)ctx->fstride * 24.0 < 4294967295.0 && (double)ctx->ntot * 6 * ctx->id_bytes < 4294967295.0;
+}
+
+static bool split_ok(const gpx_ctx *ctx)
+{
+    if (ctx->fused == 3) return split_eligible(ctx);
+    return ctx->fused == 4 && ctx->split_choice != 0 && split_eligible(ctx);
 }
 
 static hipError_t launch_interior(const gpx_ctx *ctx, int it)
@@ -1638,13 +1646,13 @@ static hipError_t launch_interior(const gpx_ctx *ctx, int it)
     a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
     a.plane = ctx->plane;
     memcpy(a.box, ctx->box3, sizeof(a.box));
-    const int B = 256;
+    const int B = INT_BLOCK;
     const long long cells = (long long)(a.box[3] - a.box[2]) * (a.box[5] - a.box[4]);
     a.nseg = (int)((cells + B - 1) / B);
     const int nxb = a.box[1] - a.box[0];
     const void *k = ctx->id_bytes == 2 ? (const void *)k_interior<uint16_t> : (const void *)k_interior<uint32_t>;
     const size_t lds = (size_t)g.nmat * 40;
-    int xc = ctx->xchunk ? ctx->xchunk : 6;
+    int xc = ctx->xchunk ? ctx->xchunk : 4;
     if (!ctx->xchunk) {
         const int fill = resident_blocks(ctx, k, B, lds);
         while (xc > 2 && (long long)a.nseg * ((nxb + xc - 1) / xc) < fill) --xc;
@@ -1712,54 +1720,91 @@ static int e_flags(const gpx_ctx *ctx)
     return f;
 }
 
-// Runs iterations [it0, it0+n): one fused kernel per iteration when the grid
-// allows it, else the H and E passes.  With `ms4`, events around every kernel
-// give the mean duration of each kernel of an iteration over the run: the
-// fused kernel in ms4[0] (the rest 0), or H interior, H boundary, E interior
-// and E boundary (the host waits after each iteration to read them).
+// Runs iterations [it0, it0+n) in the step form set by gpx_set_step.  With
+// `ms4`, events around every kernel give the mean duration of each kernel of
+// an iteration over the run: the fused kernel in ms4[0] (the rest 0); the
+// split step's interior, H boundary and E boundary kernels in ms4[0], [1]
+// and [3]; or H interior, H boundary, E interior and E boundary.
+static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4);
+
+// Form 4 picks between the split step and the passes on the first run long
+// enough to time both: after two warm-up iterations, TUNE_ITS iterations of
+// each, timed with events; the faster serves the rest of this run and later
+// runs.  The forms give the same bits, so the choice does not change results.
+static const int TUNE_ITS = 6;
+
 static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
+{
+    if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || !split_eligible(ctx) ||
+        n < 2 + 4 * TUNE_ITS)
+        return run_steps_form(ctx, it0, n, ms4);
+    hipEvent_t e[3];
+    for (auto &x : e) GPX_TRY(hipEventCreate(&x));
+    ctx->split_choice = 1;
+    int rc = run_steps_form(ctx, it0, 2, nullptr);
+    if (!rc) { GPX_TRY(hipEventRecord(e[0], ctx->stream)); rc = run_steps_form(ctx, it0 + 2, TUNE_ITS, nullptr); }
+    ctx->split_choice = 0;
+    if (!rc) { GPX_TRY(hipEventRecord(e[1], ctx->stream)); rc = run_steps_form(ctx, it0 + 2 + TUNE_ITS, TUNE_ITS, nullptr); }
+    if (!rc) {
+        GPX_TRY(hipEventRecord(e[2], ctx->stream));
+        GPX_TRY(hipEventSynchronize(e[2]));
+        float t_split = 0, t_pass = 0;
+        GPX_TRY(hipEventElapsedTime(&t_split, e[0], e[1]));
+        GPX_TRY(hipEventElapsedTime(&t_pass, e[1], e[2]));
+        ctx->split_choice = t_split <= t_pass ? 1 : 0;
+    }
+    for (auto &x : e) hipEventDestroy(x);
+    if (rc) {
+        ctx->split_choice = -1;
+        return rc;
+    }
+    const int done = 2 + 2 * TUNE_ITS;
+    return run_steps_form(ctx, it0 + done, n - done, nullptr);
+}
+
+// Timing events of a profiled run, destroyed with the run.
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    ~EventSet()
+    {
+        for (hipEvent_t x : e)
+            if (x) hipEventDestroy(x);
+    }
+};
+
+static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
     const int hf = h_flags(ctx), ef = e_flags(ctx);
     const int tc = step_tile(ctx);
     const bool split = !tc && split_ok(ctx);
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    double acc[4] = {0, 0, 0, 0};
-    if (ms4)
-        for (auto &e : ev) GPX_TRY(hipEventCreate(&e));
+    // With ms4, six events per iteration are recorded around the kernels and
+    // read once at the end, so the host never waits between iterations and
+    // the gaps between events are the kernels' own.
+    const int per = 6;
+    EventSet evs;
+    if (ms4) {
+        evs.e.assign((size_t)(n > 0 ? n : 0) * per, nullptr);
+        for (auto &e : evs.e) GPX_TRY(hipEventCreate(&e));
+    }
     for (int it = it0; it < it0 + n; ++it) {
+        hipEvent_t *ev = ms4 ? &evs.e[(size_t)(it - it0) * per] : nullptr;
         for (size_t s = 0; s < ctx->snaps.size(); ++s)
             if (ctx->snaps[s].time == it + 1) GPX_TRY(launch_snapshot(ctx, (int)s));
         if (tc) {
-            if (ms4) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
             GPX_TRY(launch_step(ctx, tc, it));
-            if (ms4) {
-                GPX_TRY(hipEventRecord(ev[1], ctx->stream));
-                GPX_TRY(hipEventSynchronize(ev[1]));
-                float t;
-                GPX_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
-                acc[0] += t;
-            }
+            if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
             continue;
         }
         if (split) {
-            if (ms4) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
             GPX_TRY(launch_interior(ctx, it));
-            if (ms4) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+            if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
             GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, nullptr, 1.0, true));
-            if (ms4) GPX_TRY(hipEventRecord(ev[2], ctx->stream));
+            if (ev) GPX_TRY(hipEventRecord(ev[2], ctx->stream));
             GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, 1.0, true));
+            if (ev) GPX_TRY(hipEventRecord(ev[3], ctx->stream));
             for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
-            if (ms4) {
-                GPX_TRY(hipEventRecord(ev[3], ctx->stream));
-                GPX_TRY(hipEventSynchronize(ev[3]));
-                float t[3];
-                GPX_TRY(hipEventElapsedTime(&t[0], ev[0], ev[1]));
-                GPX_TRY(hipEventElapsedTime(&t[1], ev[1], ev[2]));
-                GPX_TRY(hipEventElapsedTime(&t[2], ev[2], ev[3]));
-                acc[0] += t[0];
-                acc[1] += t[1];
-                acc[3] += t[2];
-            }
             continue;
         }
         if (ctx->fused == 2 && !ms4) {
@@ -1767,20 +1812,29 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
             GPX_TRY(launch_pass_overlap(ctx, 0, ef, it));
             continue;
         }
-        GPX_TRY(launch_pass(ctx, 1, hf, it, ms4 ? ev : nullptr));
-        GPX_TRY(launch_pass(ctx, 0, ef, it, ms4 ? ev + 3 : nullptr));
-        if (ms4) {
-            GPX_TRY(hipEventSynchronize(ev[5]));
-            float t[4];
-            GPX_TRY(hipEventElapsedTime(&t[0], ev[0], ev[1]));
-            GPX_TRY(hipEventElapsedTime(&t[1], ev[1], ev[2]));
-            GPX_TRY(hipEventElapsedTime(&t[2], ev[3], ev[4]));
-            GPX_TRY(hipEventElapsedTime(&t[3], ev[4], ev[5]));
-            for (int q = 0; q < 4; ++q) acc[q] += t[q];
-        }
+        GPX_TRY(launch_pass(ctx, 1, hf, it, ev));
+        GPX_TRY(launch_pass(ctx, 0, ef, it, ev ? ev + 3 : nullptr));
     }
     if (ms4) {
-        for (auto &e : ev) hipEventDestroy(e);
+        double acc[4] = {0, 0, 0, 0};
+        GPX_TRY(hipStreamSynchronize(ctx->stream));
+        // (begin, end) event pairs of each kernel slot: fused step; split
+        // step (interior, H boundary, -, E boundary); passes (H interior, H
+        // boundary, E interior, E boundary)
+        static const int pairs[3][4][2] = {{{0, 1}, {-1, -1}, {-1, -1}, {-1, -1}},
+                                           {{0, 1}, {1, 2}, {-1, -1}, {2, 3}},
+                                           {{0, 1}, {1, 2}, {3, 4}, {4, 5}}};
+        const int form = tc ? 0 : (split ? 1 : 2);
+        for (int i = 0; i < n; ++i) {
+            hipEvent_t *ev = &evs.e[(size_t)i * per];
+            for (int q = 0; q < 4; ++q) {
+                const int a = pairs[form][q][0], b = pairs[form][q][1];
+                if (a < 0) continue;
+                float t;
+                GPX_TRY(hipEventElapsedTime(&t, ev[a], ev[b]));
+                acc[q] += t;
+            }
+        }
         for (int q = 0; q < 4; ++q) ms4[q] = (float)(acc[q] / (n > 0 ? n : 1));
     }
     return 0;
@@ -1797,6 +1851,12 @@ extern "C" int gpx_step_form(const gpx_ctx *ctx)
     if (step_tile(ctx)) return 1;
     if (split_ok(ctx)) return 3;
     return ctx->fused == 2 ? 2 : 0;
+}
+
+extern "C" int gpx_step_choice(const gpx_ctx *ctx)
+{
+    if (!ctx) return GPX_EARG;
+    return ctx->split_choice;
 }
 
 extern "C" int gpx_run(gpx_ctx *ctx, int it0, int n)

@@ -6,14 +6,16 @@
 // The E update of cell (i, j, k) needs new H at (i, j, k) and at its i-1, j-1
 // and k-1 neighbours.  A lane owns one (j, k) column of the box and marches x
 // through a chunk of planes, so new H at i-1 is its own from the previous
-// plane (the chunk starts one plane early to make it).  New Hz and Hx at j-1
-// are recomputed by the lane from the current state: the same expression on
-// the same inputs gives the same bits as the owning lane's result.  New Hx and
-// Hy at k-1 come from lane-1 of the wave by a cross-lane shift, or are
-// recomputed where lane-1 does not own that cell (lane 0, and the first cell
-// of a box row).  The box leaves one plain cell on its minus sides, so every
-// recomputed neighbour is a cell of the plain update (no PML, all neighbours
-// inside the grid) and no neighbour needs a term this kernel lacks.
+// plane (the chunk starts one plane early to make it).  A block holds
+// INT_BLOCK consecutive cells of the box's (j, k) rectangle; each plane's new
+// H goes to LDS (two buffers, planes alternating, one barrier per plane), and
+// new H at j-1 and k-1 is read there from the lanes Kin and 1 back.  Where
+// that lane is not in the block (the block's first box row for j-1; its first
+// lane and the first cell of a box row for k-1) the lane recomputes the value
+// from the current state: the same expression on the same inputs gives the
+// same bits as the owning lane's result.  The box leaves one plain cell on its
+// minus sides, so every recomputed neighbour is a cell of the plain update (no
+// PML, all neighbours inside the grid) and needs no term this kernel lacks.
 //
 // The formulas and their association are those of update_magnetic and
 // update_electric (gprMax/fields_updates_ext.pyx:81-107, 404-412), then
@@ -23,6 +25,8 @@
 #pragma once
 
 namespace gpx {
+
+constexpr int INT_BLOCK = 512;  // 256: 281 us, 512: 249 us, 1024: 254 us on the 300x300x200 half-space
 
 struct IntArgs {
     int nx, ny, nz, plane;
@@ -57,7 +61,7 @@ __device__ __forceinline__ float int_msrc(const IntArgs &a, const float *sH, int
 }
 
 template <class IDT>
-__global__ void __launch_bounds__(256) k_interior(IntArgs a)
+__global__ void __launch_bounds__(INT_BLOCK) k_interior(IntArgs a)
 {
     extern __shared__ float4 lds4[];
     const int nm = a.nmat;
@@ -70,7 +74,7 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
     const int item = logical_block(a.per_xcd);
     if (item >= a.nitems) return;
     const int it = a.it;
-    const int B = blockDim.x, lane = threadIdx.x & 63;
+    const int B = blockDim.x;
     const int NZ1 = a.nz + 1;
     const int Kin = a.box[5] - a.box[4];
     const int ncell = (a.box[3] - a.box[2]) * Kin;
@@ -80,8 +84,6 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
     const int q = live ? t : 0;
     const int j = a.box[2] + q / Kin, k = a.box[4] + q % Kin;
     const int xa = a.box[0] + chunk * a.xc, xb = min(xa + a.xc, a.box[1]);
-    // k-1 from lane-1 only where lane-1 holds cell (j, k-1)
-    const bool kself = lane == 0 || k == a.box[4];
 
     const rsrc_t rF = rsrc(a.Fc), wF = rsrc(a.Fx), rI = rsrc(a.ID);
     const uint32_t cs = a.cs, ci = (uint32_t)a.ntot * (uint32_t)sizeof(IDT);
@@ -95,10 +97,13 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
 
     struct PL {
         float hx, hy, hz, ezj, exj, eyk, exk, ex1, ey1, ez1;
-        float hxjm, hzjm, ezjm, exjm, eyjm, eyjmk, eyjm1;
-        uint32_t m0, m1, m2, m3, m4, m5, m3jm, m5jm;
+        uint32_t m0, m1, m2, m3, m4, m5;
     };
-    // Loads of plane ip: the cell's own values and those of its j-1 neighbour.
+    __shared__ float lh[2][3][INT_BLOCK];  // new H of the block, planes alternating
+    const int tl = threadIdx.x;
+    const bool jrec = tl < Kin;
+    // Loads of plane ip for the cell: its old H and E at i, i+1, j+1 and k+1,
+    // and its six material indices.
     auto load = [&](int ip, PL &L) {
         const uint32_t so = (uint32_t)ip * P4, sn = so + P4, si = (uint32_t)ip * Pid;
         L.hx = bld(rF, to, so + C(HX));
@@ -111,21 +116,12 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
         L.ex1 = bld(rF, to, sn + C(EX));
         L.ey1 = bld(rF, to, sn + C(EY));
         L.ez1 = bld(rF, to, sn + C(EZ));
-        L.hxjm = bld(rF, tjm, so + C(HX));
-        L.hzjm = bld(rF, tjm, so + C(HZ));
-        L.ezjm = bld(rF, tjm, so + C(EZ));
-        L.exjm = bld(rF, tjm, so + C(EX));
-        L.eyjm = bld(rF, tjm, so + C(EY));
-        L.eyjmk = bld(rF, tjm + 4u, so + C(EY));
-        L.eyjm1 = bld(rF, tjm, sn + C(EY));
         L.m0 = bld_id<IDT>(rI, tid, si);
         L.m1 = bld_id<IDT>(rI, tid, si + ci);
         L.m2 = bld_id<IDT>(rI, tid, si + 2 * ci);
         L.m3 = bld_id<IDT>(rI, tid, si + 3 * ci);
         L.m4 = bld_id<IDT>(rI, tid, si + 4 * ci);
         L.m5 = bld_id<IDT>(rI, tid, si + 5 * ci);
-        L.m3jm = bld_id<IDT>(rI, tidjm, si + 3 * ci);
-        L.m5jm = bld_id<IDT>(rI, tidjm, si + 5 * ci);
     };
 
     // the halo plane xa-1: new Hy and Hz there, for the E update of plane xa
@@ -151,9 +147,8 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
         ez = ez1;
     }
 
-    // No software prefetch: loading the next plane ahead raised the kernel to
-    // 91 VGPRs (5 waves per SIMD) for no gain; at 58 VGPRs 7 waves hide the
-    // latency as well.
+    // No software prefetch: loading the next plane ahead raised the VGPR count
+    // from under 64 to over 90 (5 waves per SIMD) for no gain.
     PL cur;
     for (int i = xa; i < xb; ++i) {
         const uint32_t so = (uint32_t)i * P4;
@@ -168,23 +163,42 @@ __global__ void __launch_bounds__(256) k_interior(IntArgs a)
             hy = qy.x * cur.hy - qy.w * (cur.exk - ex) + qy.y * (cur.ez1 - ez);
             hz = qz.x * cur.hz - qz.y * (cur.ey1 - ey) + qz.z * (cur.exj - ex);
         }
-        // new Hx and Hz at j-1
-        float hxj, hzj;
-        {
-            const float4 qx = cH[cur.m3jm], qz = cH[cur.m5jm];
-            hxj = qx.x * cur.hxjm - qx.z * (ez - cur.ezjm) + qx.w * (cur.eyjmk - cur.eyjm);
-            hzj = qz.x * cur.hzjm - qz.y * (cur.eyjm1 - cur.eyjm) + qz.z * (ex - cur.exjm);
-        }
         if (pf & PF_SRCH) {
             hx = int_msrc(a, sH, i, j, k, 0, cur.m3, hx, it);
             hy = int_msrc(a, sH, i, j, k, 1, cur.m4, hy, it);
             hz = int_msrc(a, sH, i, j, k, 2, cur.m5, hz, it);
-            hxj = int_msrc(a, sH, i, j - 1, k, 0, cur.m3jm, hxj, it);
-            hzj = int_msrc(a, sH, i, j - 1, k, 2, cur.m5jm, hzj, it);
         }
-        // new Hx and Hy at k-1
-        float hxk = __shfl_up(hx, 1, 64), hyk = __shfl_up(hy, 1, 64);
-        if (kself) {
+        const int pb = (i - xa) & 1;
+        lh[pb][0][tl] = hx;
+        lh[pb][1][tl] = hy;
+        lh[pb][2][tl] = hz;
+        __syncthreads();
+        // new Hx and Hz at j-1: from the lane Kin back, or recomputed
+        float hxj, hzj;
+        if (!jrec) {
+            hxj = lh[pb][0][tl - Kin];
+            hzj = lh[pb][2][tl - Kin];
+        } else {
+            const uint32_t sn = so + P4, si = (uint32_t)i * Pid;
+            const float hxjm = bld(rF, tjm, so + C(HX)), hzjm = bld(rF, tjm, so + C(HZ));
+            const float ezjm = bld(rF, tjm, so + C(EZ)), exjm = bld(rF, tjm, so + C(EX));
+            const float eyjm = bld(rF, tjm, so + C(EY)), eyjmk = bld(rF, tjm + 4u, so + C(EY));
+            const float eyjm1 = bld(rF, tjm, sn + C(EY));
+            const uint32_t m3 = bld_id<IDT>(rI, tidjm, si + 3 * ci), m5 = bld_id<IDT>(rI, tidjm, si + 5 * ci);
+            const float4 qx = cH[m3], qz = cH[m5];
+            hxj = qx.x * hxjm - qx.z * (ez - ezjm) + qx.w * (eyjmk - eyjm);
+            hzj = qz.x * hzjm - qz.y * (eyjm1 - eyjm) + qz.z * (ex - exjm);
+            if (pf & PF_SRCH) {
+                hxj = int_msrc(a, sH, i, j - 1, k, 0, m3, hxj, it);
+                hzj = int_msrc(a, sH, i, j - 1, k, 2, m5, hzj, it);
+            }
+        }
+        // new Hx and Hy at k-1: from the previous lane, or recomputed
+        float hxk, hyk;
+        if (!(tl == 0 || k == a.box[4])) {
+            hxk = lh[pb][0][tl - 1];
+            hyk = lh[pb][1][tl - 1];
+        } else {
             const float hx0 = bld(rF, tkm, so + C(HX)), hy0 = bld(rF, tkm, so + C(HY));
             const float ezk = bld(rF, tkm, so + C(EZ)), ezjk = bld(rF, tkm + 4u * NZ1, so + C(EZ));
             const float eyk = bld(rF, tkm, so + C(EY)), exk = bld(rF, tkm, so + C(EX));

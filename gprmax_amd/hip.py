@@ -51,6 +51,7 @@ SIGNATURES = {
     'gpx_set_step': ([_P, _I, _I], _I),
     'gpx_step_kind': ([_P], _I),
     'gpx_step_form': ([_P], _I),
+    'gpx_step_choice': ([_P], _I),
     'gpx_synchronize': ([_P], _I),
     'gpx_download_rx': ([_P, _P], _I),
     'gpx_download_fields': ([_P, _P], _I),
@@ -183,6 +184,10 @@ class HipSolver:
     def step_form(self):
         """The form gpx_run uses: 0 passes, 1 fused, 2 side by side, 3 split."""
         return int(self.lib.gpx_step_form(self.ctx))
+
+    def step_choice(self):
+        """The default form's choice: -1 not timed yet, 0 passes, 1 split."""
+        return int(self.lib.gpx_step_choice(self.ctx))
 
     def run(self, it0=0, n=None):
         n = self.G.iterations - it0 if n is None else n

@@ -111,9 +111,12 @@ int gpx_run_timed(gpx_ctx *ctx, int it0, int n, int per_kernel, float *ms_total,
 
 /* As gpx_run, with HIP events around every kernel; ms4 receives the mean
  * duration in ms of each kernel of an iteration: the fused step kernel in
- * ms4[0] (ms4[1..3] = 0), or with the two-pass form H interior, H boundary,
- * E interior, E boundary (interior = cells away from the faces and PML slabs,
- * boundary = the rest).  Waits after each iteration. */
+ * ms4[0] (ms4[1..3] = 0); the split step's interior kernel, H boundary and E
+ * boundary passes in ms4[0], ms4[1], ms4[3] (ms4[2] = 0); or with the
+ * two-pass form H interior, H boundary, E interior, E boundary (interior =
+ * cells away from the faces and PML slabs, boundary = the rest).  The form 4
+ * choice is not timed here: a profiled run uses the split step until a
+ * gpx_run has made the choice.  Waits once, at the end. */
 int gpx_run_profiled(gpx_ctx *ctx, int it0, int n, float *ms4);
 
 /* Cells of the interior box (the rest are boundary cells), and the box
@@ -129,12 +132,14 @@ int gpx_part(gpx_ctx *ctx, int part, int arg, int it);
 int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
 
 /* Step form of gpx_run.  fused = 1: one kernel per iteration that does the
- * H and the E update with the state double-buffered on the device; fused = 0
- * (default): an H pass and an E pass per iteration; fused = 2: the passes
- * with their interior and boundary kernels running side by side on two
- * streams (gpx_run_profiled always runs them one after the other); fused = 3:
- * the split step, one kernel doing H and E for the interior and the H and E
- * boundary passes around it, double-buffered.  tile_cells = cells per fused
+ * H and the E update with the state double-buffered on the device; fused = 0:
+ * an H pass and an E pass per iteration; fused = 2: the passes with their
+ * interior and boundary kernels running side by side on two streams
+ * (gpx_run_profiled always runs them one after the other); fused = 3: the
+ * split step, one kernel doing H and E for the interior and the H and E
+ * boundary passes around it, double-buffered; fused = 4 (default): the first
+ * gpx_run of at least 26 iterations times the split step and the passes on
+ * six iterations each and keeps the faster (gpx_step_choice).  tile_cells = cells per fused
  * block (256, 512 or 1024; 0 = auto).  All forms give identical results; the
  * fused form needs a tile of at least twice nz+1 cells within the LDS budget,
  * the split step a model without dispersive poles, and the passes run when
@@ -149,6 +154,9 @@ int gpx_step_kind(const gpx_ctx *ctx);
  * 3 split step.  gpx_run_profiled reports the split step's interior kernel in
  * ms4[0] and its boundary passes in ms4[1] and ms4[3]. */
 int gpx_step_form(const gpx_ctx *ctx);
+
+/* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step. */
+int gpx_step_choice(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);
 

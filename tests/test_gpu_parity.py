@@ -158,11 +158,12 @@ def _compare_runs(F, rxb, snaps, phis, T, Gc):
             assert np.array_equal(T[n], getattr(Gc, name)), name
 
 
-FORMS = dict(argvalues=[1, 0, 2, 3], ids=['fused-step', 'two-pass', 'side-by-side', 'split'])
+FORMS = dict(argvalues=[1, 0, 2, 3, 4], ids=['fused-step', 'two-pass', 'side-by-side', 'split', 'timed'])
 
 
 def _expected_form(fused, G):
-    if fused == 3:
+    # form 4 runs the split step until its first long run has timed it
+    if fused in (3, 4):
         return 3 if G.maxpoles == 0 else 0
     return fused
 
@@ -178,7 +179,15 @@ def test_fused_run_matches_oracle(case, fused):
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
         assert s.step_form() == _expected_form(fused, G)
+        assert s.step_choice() == -1
         s.run(0, G.iterations)
+        # 40 iterations: form 4 timed both forms on iterations 2-13 and went on
+        # with the faster, switching forms in the middle of the run
+        if fused == 4 and not G.maxpoles:
+            assert s.step_choice() in (0, 1)
+            assert s.step_form() == (3 if s.step_choice() else 0)
+        else:
+            assert s.step_choice() == -1
         F = s.download_fields()
         rxb = s.download_rx()
         snaps = [s.download_snapshot(n) for n in range(len(G.snapshots))]
@@ -232,7 +241,7 @@ def test_2d_run_matches_oracle(case, axis, fused):
     with _hip()(G) as s:
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
-        assert s.step_form() == (0 if fused == 3 else fused)   # a 2-D grid has no fused box
+        assert s.step_form() == (0 if fused in (3, 4) else fused)   # a 2-D grid has no fused box
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -306,7 +315,7 @@ def test_step_forms_agree_on_a_wide_grid(poles):
     """All step forms give the same bits on a grid with several tiles and
     several blocks per plane."""
     out = []
-    for fused in (1, 0, 2, 3):
+    for fused in (1, 0, 2, 3, 4):
         G = rich_model(seed=8, shape=WIDE, iterations=30, poles=poles, order=2)
         with _hip()(G) as s:
             s.set_step(fused)
