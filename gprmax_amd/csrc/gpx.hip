@@ -608,6 +608,82 @@ __global__ void k_snapshot(int ny, int nz, int xs, int ys, int zs, int sdx, int 
     out[5 * n + o] = (Hz[at(i, j, k)] + Hz[at(i, j, k + 1)]) / 2.0f;
 }
 
+
+// ---------------------------------------------------------------- transmission lines
+// A one-dimensional line attached to the E edge at (i, j, k)
+// (gprMax/sources.py:286-451).  Line arithmetic follows NumPy 2 promotion
+// of the reference's expressions: the line coefficients are float64
+// scalars, so each update is a float64 product on float32 differences,
+// rounded to float32 when stored; the cell sizes of the current loop and of
+// the E assignment are Python floats, so those are float32 operations.
+struct TLDev {
+    int i, j, k, pol, first, last, nl, srcpos, antpos;
+    double cV, cI, cW, h;   // R (c dt / dl), (1/R)(c dt / dl), c dt / dl, ABC factor
+    float dpol, da, db;     // cell size along pol; the two sizes of the current loop
+    float *V, *I, *abc;     // line voltage [nl], current [nl], ABC memory (abcv0, abcv1)
+    const float *whole, *half;  // waveform on whole and half steps [iterations]
+    float *vt, *itot;       // voltage and current at the antenna end per iteration
+};
+
+// After the H pass: the line state at the start of the iteration goes to the
+// outputs (gprMax/fields_outputs.py:65-67); in the active window the line
+// current is updated and its antenna end takes the current of the H loop
+// around the edge (update_magnetic, gprMax/sources.py:428-451, and
+// Ix/Iy/Iz, gprMax/grid.py:413-461).  One thread per line.
+__global__ void k_tl_h(const TLDev *tl, int n, int it, int ny, int nz, const float *Hx, const float *Hy,
+                       const float *Hz)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const TLDev &L = tl[s];
+    L.vt[it] = L.V[L.antpos];
+    L.itot[it] = L.I[L.antpos];
+    if (it < L.first || it > L.last) return;
+    for (int m = 0; m + 1 < L.nl; ++m) {
+        const float d = L.V[m + 1] - L.V[m];
+        L.I[m] = (float)((double)L.I[m] - L.cI * (double)d);
+    }
+    L.I[L.srcpos - 1] = (float)((double)L.I[L.srcpos - 1] + L.cI * (double)L.half[it]);
+    const long long NY1 = ny + 1, NZ1 = nz + 1;
+    auto at = [&](int a, int b, int c) { return ((long long)a * NY1 + b) * NZ1 + c; };
+    const int x = L.i, y = L.j, z = L.k;
+    float cur = 0.f;
+    if (L.pol == 0) {
+        if (y != 0 && z != 0)
+            cur = L.da * (Hy[at(x, y, z - 1)] - Hy[at(x, y, z)]) + L.db * (Hz[at(x, y, z)] - Hz[at(x, y - 1, z)]);
+    } else if (L.pol == 1) {
+        if (x != 0 && z != 0)
+            cur = L.da * (Hx[at(x, y, z)] - Hx[at(x, y, z - 1)]) + L.db * (Hz[at(x - 1, y, z)] - Hz[at(x, y, z)]);
+    } else {
+        if (x != 0 && y != 0)
+            cur = L.da * (Hx[at(x, y - 1, z)] - Hx[at(x, y, z)]) + L.db * (Hy[at(x, y, z)] - Hy[at(x - 1, y, z)]);
+    }
+    L.I[L.antpos] = cur;
+}
+
+// After the E pass, in the active window: line voltage, the one-way
+// injector, the absorbing end, then the edge's E from the antenna voltage
+// (update_electric, gprMax/sources.py:361-426).
+__global__ void k_tl_e(const TLDev *tl, int n, int it, int ny, int nz, float *Ex, float *Ey, float *Ez)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const TLDev &L = tl[s];
+    if (it < L.first || it > L.last) return;
+    for (int m = L.nl - 1; m >= 1; --m) {
+        const float d = L.I[m] - L.I[m - 1];
+        L.V[m] = (float)((double)L.V[m] - L.cV * (double)d);
+    }
+    L.V[L.srcpos] = (float)((double)L.V[L.srcpos] + L.cW * (double)L.whole[it]);
+    const float a0 = L.abc[0], a1 = L.abc[1];
+    L.V[0] = (float)(L.h * (double)(L.V[1] - a0) + (double)a1);
+    L.abc[0] = L.V[0];
+    L.abc[1] = L.V[1];
+    const long long at = ((long long)L.i * (ny + 1) + L.j) * (nz + 1) + L.k;
+    float *E = L.pol == 0 ? Ex : (L.pol == 1 ? Ey : Ez);
+    E[at] = -L.V[L.antpos] / L.dpol;
+}
+
 }  // namespace gpx
 
 using namespace gpx;
@@ -625,6 +701,11 @@ struct Slab {
 struct Snap {
     int xs, ys, zs, dx, dy, dz, snx, sny, snz, time;
     float *out = nullptr;  // [6][snx][sny][snz]
+};
+
+struct TLine {
+    TLDev d;                // device pointers into `mem`
+    float *mem = nullptr;   // V, I, abc, whole, half, vt, itot
 };
 
 struct SrcSet {
@@ -660,6 +741,8 @@ struct gpx_ctx {
     int *rxc = nullptr;
     float *rxbuf = nullptr;
     std::vector<Snap> snaps;
+    std::vector<TLine> tls;          // transmission lines
+    TLDev *tldev = nullptr;          // device copy of their descriptions
     uint32_t *planeflag = nullptr;  // [nx+1]
     StepRare *rare = nullptr;        // device copy of the source/receiver tables for the fused step
     bool rare_dirty = true;
@@ -822,6 +905,8 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->frame) hipFree(ctx->frame);
     if (ctx->frame3) hipFree(ctx->frame3);
     for (auto &s : ctx->snaps) if (s.out) hipFree(s.out);
+    for (auto &t : ctx->tls) if (t.mem) hipFree(t.mem);
+    if (ctx->tldev) hipFree(ctx->tldev);
     if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
@@ -1021,6 +1106,72 @@ extern "C" int gpx_set_sources(gpx_ctx *ctx, int kind, int n, const int32_t *inf
     if (!rc) rc = copy_in(ctx, S.waves, waves, (size_t)n * g.iterations * 4);
     if (!rc) rc = update_planeflags(ctx);
     return rc;
+}
+
+extern "C" int gpx_add_transmission_line(gpx_ctx *ctx, const int32_t *info, const double *coef,
+                                         const float *sizes, const float *whole, const float *half,
+                                         const float *voltage, const float *current, const float *abc,
+                                         int *index)
+{
+    if (!ctx || !info || !coef || !sizes || !whole || !half || !voltage || !current || !abc)
+        return fail(ctx, GPX_EARG, "gpx_add_transmission_line: null argument");
+    const gpx_desc &g = ctx->desc;
+    const int i = info[0], j = info[1], k = info[2], pol = info[3], nl = info[6], srcpos = info[7],
+              antpos = info[8];
+    if (i < 0 || i > g.nx || j < 0 || j > g.ny || k < 0 || k > g.nz || pol < 0 || pol > 2)
+        return fail(ctx, GPX_EARG, "gpx_add_transmission_line: bad cell or polarisation");
+    if (nl < 2 || srcpos < 1 || srcpos >= nl || antpos < 0 || antpos >= nl)
+        return fail(ctx, GPX_EARG, "gpx_add_transmission_line: bad line length or positions");
+    hipSetDevice(g.device);
+    TLine t;
+    TLDev &d = t.d;
+    d.i = i; d.j = j; d.k = k; d.pol = pol; d.first = info[4]; d.last = info[5];
+    d.nl = nl; d.srcpos = srcpos; d.antpos = antpos;
+    d.cV = coef[0]; d.cI = coef[1]; d.cW = coef[2]; d.h = coef[3];
+    d.dpol = sizes[0]; d.da = sizes[1]; d.db = sizes[2];
+    const size_t its = (size_t)g.iterations;
+    const size_t nf = 2 * (size_t)nl + 2 + 4 * its;
+    GPX_TRY(dalloc(ctx, (void **)&t.mem, nf * 4));
+    d.V = t.mem;
+    d.I = d.V + nl;
+    d.abc = d.I + nl;
+    d.whole = d.abc + 2;
+    d.half = d.whole + its;
+    d.vt = const_cast<float *>(d.half) + its;
+    d.itot = d.vt + its;
+    int rc = copy_in(ctx, d.V, voltage, (size_t)nl * 4);
+    if (!rc) rc = copy_in(ctx, d.I, current, (size_t)nl * 4);
+    if (!rc) rc = copy_in(ctx, d.abc, abc, 8);
+    if (!rc) rc = copy_in(ctx, const_cast<float *>(d.whole), whole, its * 4);
+    if (!rc) rc = copy_in(ctx, const_cast<float *>(d.half), half, its * 4);
+    if (rc) {
+        hipFree(t.mem);
+        return rc;
+    }
+    ctx->tls.push_back(t);
+    if (ctx->tldev) { hipFree(ctx->tldev); ctx->tldev = nullptr; }
+    GPX_TRY(dalloc(ctx, (void **)&ctx->tldev, ctx->tls.size() * sizeof(TLDev), false));
+    std::vector<TLDev> all;
+    for (const TLine &x : ctx->tls) all.push_back(x.d);
+    rc = copy_in(ctx, ctx->tldev, all.data(), all.size() * sizeof(TLDev));
+    if (index) *index = (int)ctx->tls.size() - 1;
+    return rc;
+}
+
+extern "C" int gpx_download_transmission_line(gpx_ctx *ctx, int index, float *vtotal, float *itotal,
+                                              float *voltage, float *current)
+{
+    if (!ctx || index < 0 || index >= (int)ctx->tls.size())
+        return fail(ctx, GPX_EARG, "gpx_download_transmission_line: bad index");
+    hipSetDevice(ctx->desc.device);
+    const TLDev &d = ctx->tls[index].d;
+    const size_t its = (size_t)ctx->desc.iterations;
+    GPX_TRY(hipStreamSynchronize(ctx->stream));
+    if (vtotal) GPX_TRY(hipMemcpy(vtotal, d.vt, its * 4, hipMemcpyDeviceToHost));
+    if (itotal) GPX_TRY(hipMemcpy(itotal, d.itot, its * 4, hipMemcpyDeviceToHost));
+    if (voltage) GPX_TRY(hipMemcpy(voltage, d.V, (size_t)d.nl * 4, hipMemcpyDeviceToHost));
+    if (current) GPX_TRY(hipMemcpy(current, d.I, (size_t)d.nl * 4, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 extern "C" int gpx_set_receivers(gpx_ctx *ctx, int n, const int32_t *coords)
@@ -1489,7 +1640,7 @@ static size_t step_lds(const gpx_ctx *ctx, int tc) { return step_table_lds(ctx) 
 // then the two-pass kernels run instead.
 static int step_tile(const gpx_ctx *ctx)
 {
-    if (ctx->fused != 1 || slab_order(ctx) < 0) return 0;
+    if (ctx->fused != 1 || slab_order(ctx) < 0 || !ctx->tls.empty()) return 0;
     const int NZ1 = ctx->desc.nz + 1;
     // Streamed offsets are 32-bit: a state of six components and the six
     // index arrays must each stay below 4 GiB.
@@ -1626,7 +1777,7 @@ static hipError_t launch_step(gpx_ctx *ctx, int tc, int it)
 // values), a nonempty fused box and 32-bit buffer offsets.
 static bool split_eligible(const gpx_ctx *ctx)
 {
-    if (ctx->desc.maxpoles > 0) return false;
+    if (ctx->desc.maxpoles > 0 || !ctx->tls.empty()) return false;
     const int *b = ctx->box3;
     if (b[0] >= b[1] || b[2] >= b[3] || b[4] >= b[5]) return false;
     return (double)ctx->fstride * 24.0 < 4294967295.0 && (double)ctx->ntot * 6 * ctx->id_bytes < 4294967295.0;
@@ -1697,13 +1848,67 @@ static hipError_t launch_snapshot(const gpx_ctx *ctx, int s)
     return hipGetLastError();
 }
 
+// The transmission-line kernel after the H pass (field 1) or the E pass.
+static hipError_t launch_tl(const gpx_ctx *ctx, int field, int it)
+{
+    const int n = (int)ctx->tls.size();
+    if (!n) return hipSuccess;
+    const int B = 64, grid = (n + B - 1) / B;
+    const gpx_desc &g = ctx->desc;
+    if (field == 1)
+        hipLaunchKernelGGL(k_tl_h, dim3(grid), dim3(B), 0, ctx->stream, ctx->tldev, n, it, g.ny, g.nz, ctx->F[3],
+                           ctx->F[4], ctx->F[5]);
+    else
+        hipLaunchKernelGGL(k_tl_e, dim3(grid), dim3(B), 0, ctx->stream, ctx->tldev, n, it, g.ny, g.nz, ctx->F[0],
+                           ctx->F[1], ctx->F[2]);
+    return hipGetLastError();
+}
+
+// Cases the line kernels cannot order as the reference does: the line
+// reads H after the PML and before magnetic dipoles, and sets E after
+// voltage sources, before Hertzian dipoles and dispersive part B.
+static int check_tls(gpx_ctx *ctx)
+{
+    if (ctx->tls.empty()) return 0;
+    if (ctx->desc.maxpoles > 0)
+        return fail(ctx, GPX_EARG, "transmission lines in a model with dispersive materials are not supported");
+    for (const TLine &t : ctx->tls) {
+        const TLDev &L = t.d;
+        // (component 0..2 = Hx..Hz, cell) pairs the line's current reads
+        int reads[4][4];
+        if (L.pol == 0) {
+            const int r[4][4] = {{1, L.i, L.j, L.k - 1}, {1, L.i, L.j, L.k}, {2, L.i, L.j, L.k}, {2, L.i, L.j - 1, L.k}};
+            memcpy(reads, r, sizeof(r));
+        } else if (L.pol == 1) {
+            const int r[4][4] = {{0, L.i, L.j, L.k}, {0, L.i, L.j, L.k - 1}, {2, L.i - 1, L.j, L.k}, {2, L.i, L.j, L.k}};
+            memcpy(reads, r, sizeof(r));
+        } else {
+            const int r[4][4] = {{0, L.i, L.j - 1, L.k}, {0, L.i, L.j, L.k}, {1, L.i, L.j, L.k}, {1, L.i - 1, L.j, L.k}};
+            memcpy(reads, r, sizeof(r));
+        }
+        const SrcSet &M = ctx->src[GPX_SRC_MAGNETIC], &D = ctx->src[GPX_SRC_HERTZIAN];
+        for (int s = 0; s < M.n; ++s) {
+            const int *in = &M.host[6 * s];
+            for (auto &r : reads)
+                if (in[3] == r[0] && in[0] == r[1] && in[1] == r[2] && in[2] == r[3])
+                    return fail(ctx, GPX_EARG, "a magnetic dipole on an H component a transmission line reads");
+        }
+        for (int s = 0; s < D.n; ++s) {
+            const int *in = &D.host[6 * s];
+            if (in[3] == L.pol && in[0] == L.i && in[1] == L.j && in[2] == L.k)
+                return fail(ctx, GPX_EARG, "a Hertzian dipole on the edge of a transmission line");
+        }
+    }
+    return 0;
+}
+
 static int check_ready(gpx_ctx *ctx)
 {
     if (!ctx->have_id || !ctx->have_coeffs)
         return fail(ctx, GPX_ESTATE, "ID and coefficient tables must be uploaded before stepping");
     if (ctx->desc.maxpoles > 0 && !ctx->have_disp)
         return fail(ctx, GPX_ESTATE, "dispersive coefficients must be uploaded before stepping");
-    return 0;
+    return check_tls(ctx);
 }
 
 static int h_flags(const gpx_ctx *ctx)
@@ -1809,11 +2014,15 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
         }
         if (ctx->fused == 2 && !ms4) {
             GPX_TRY(launch_pass_overlap(ctx, 1, hf, it));
+            GPX_TRY(launch_tl(ctx, 1, it));
             GPX_TRY(launch_pass_overlap(ctx, 0, ef, it));
+            GPX_TRY(launch_tl(ctx, 0, it));
             continue;
         }
         GPX_TRY(launch_pass(ctx, 1, hf, it, ev));
+        GPX_TRY(launch_tl(ctx, 1, it));
         GPX_TRY(launch_pass(ctx, 0, ef, it, ev ? ev + 3 : nullptr));
+        GPX_TRY(launch_tl(ctx, 0, it));
     }
     if (ms4) {
         double acc[4] = {0, 0, 0, 0};
@@ -2159,6 +2368,8 @@ extern "C" int gpx_batch_create(gpx_ctx *const *ctxs, int n, gpx_batch **out)
         gpx_ctx *c = ctxs[t];
         if (!c) return bfail(nullptr, GPX_EARG, "gpx_batch_create: null context");
         if (int rc = check_ready(c)) return bfail(nullptr, rc, "gpx_batch_create: model " + std::to_string(t) + ": " + c->err);
+        if (!c->tls.empty())
+            return bfail(nullptr, GPX_EARG, "gpx_batch_create: model " + std::to_string(t) + " has transmission lines");
         if (c->desc.device != c0->desc.device)
             return bfail(nullptr, GPX_EARG, "gpx_batch_create: all models must be on one device");
         if (c->desc.iterations != c0->desc.iterations)

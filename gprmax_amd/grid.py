@@ -25,7 +25,7 @@ from .materials import Material, default_materials, process_materials
 from .pml import CFS, build_pmls, default_pml_thickness
 from .receivers import Rx
 from .snapshots import Snapshot
-from .sources import HertzianDipole, MagneticDipole, VoltageSource
+from .sources import HertzianDipole, MagneticDipole, TransmissionLine, VoltageSource
 from .utilities import GeneralError, courant_dt, iterations_for, round_value
 from .waveforms import Waveform
 
@@ -180,6 +180,22 @@ class ModelBuilder:
         v.resistance = float(resistance)
         self.G.voltagesources.append(self._place(v, pol, cell, waveform, start, stop))
         return v
+
+    def add_transmission_line(self, pol, cell, resistance, waveform, start=None, stop=None):
+        """#transmission_line (gprMax/input_cmds_multiuse.py:330-383): a line of
+        impedance 0 < resistance < z0 on the E edge at `cell`; its incident
+        voltage and current are computed here, as the reference does when it
+        reads the command."""
+        from .constants import z0
+        if resistance <= 0 or resistance >= z0:
+            raise GeneralError('a transmission line needs a resistance greater than zero and less than '
+                               'the impedance of free space ({:.2f} Ohms)'.format(float(z0)))
+        t = TransmissionLine(self.G)
+        t.resistance = float(resistance)
+        self._place(t, pol, cell, waveform, start, stop)
+        t.calculate_incident_V_I(self.G)
+        self.G.transmissionlines.append(t)
+        return t
 
     def add_rx(self, cell, name=None):
         rx = Rx(ID=name or 'Rx({},{},{})'.format(*cell), xcoord=int(cell[0]),

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from .pml import DIRECTION_CODE
-from .sources import device_source_table
+from .sources import device_source_table, device_tl_args
 from .receivers import rx_coords
 from .utilities import GeneralError
 
@@ -40,6 +40,8 @@ SIGNATURES = {
     'gpx_add_pml': ([_P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _P, _P, ctypes.POINTER(_I)], _I),
     'gpx_set_sources': ([_P, _I, _I, _P, _P, _P], _I),
     'gpx_set_receivers': ([_P, _I, _P], _I),
+    'gpx_add_transmission_line': ([_P] * 9 + [ctypes.POINTER(_I)], _I),
+    'gpx_download_transmission_line': ([_P, _I, _P, _P, _P, _P], _I),
     'gpx_add_snapshot': ([_P] + [_I] * 10 + [ctypes.POINTER(_I)], _I),
     'gpx_run': ([_P, _I, _I], _I),
     'gpx_run_timed': ([_P, _I, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
@@ -143,6 +145,11 @@ class HipSolver:
                                                      _arr(waves, np.float32)[1]))
         if G.rxs:
             self._check(self.lib.gpx_set_receivers(ctx, len(G.rxs), _arr(rx_coords(G), np.int32)[1]))
+        for tl in getattr(G, 'transmissionlines', []):
+            args = device_tl_args(tl, G)
+            idx = ctypes.c_int()
+            self._check(self.lib.gpx_add_transmission_line(ctx, *(a.ctypes.data_as(ctypes.c_void_p) for a in args),
+                                                           ctypes.byref(idx)))
         for snap in G.snapshots:
             idx = ctypes.c_int()
             self._check(self.lib.gpx_add_snapshot(ctx, snap.xs, snap.ys, snap.zs, snap.dx, snap.dy, snap.dz,
@@ -232,6 +239,16 @@ class HipSolver:
         if a.size != 6 * self.ntot:
             raise GeneralError('upload_fields expects 6 x (nx+1)(ny+1)(nz+1) values')
         self._check(self.lib.gpx_upload_fields(self.ctx, p))
+
+    def download_transmission_line(self, n):
+        """(Vtotal, Itotal) per iteration and the line's (voltage, current) now."""
+        tl = self.G.transmissionlines[n]
+        its = self.G.iterations
+        vt, it_ = np.zeros(its, np.float32), np.zeros(its, np.float32)
+        v, i = np.zeros(tl.nl, np.float32), np.zeros(tl.nl, np.float32)
+        self._check(self.lib.gpx_download_transmission_line(self.ctx, n, *(a.ctypes.data_as(ctypes.c_void_p)
+                                                                            for a in (vt, it_, v, i))))
+        return vt, it_, v, i
 
     def download_rx(self):
         out = np.zeros((6, self.G.iterations, max(len(self.G.rxs), 1)), dtype=np.float32)

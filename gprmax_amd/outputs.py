@@ -6,8 +6,10 @@ gprmax_amd.hdf5 (no h5py needed).
 The file layout is the reference's: root attributes gprMax, Title,
 Iterations, nx_ny_nz, dx_dy_dz, dt, nsrc, nrx, srcsteps, rxsteps; a group
 /srcs/src<n> (attributes Type, Position) per voltage source, Hertzian dipole
-and magnetic dipole, in that order; a group /rxs/rx<n> (attributes Name,
-Position) per receiver holding one dataset per output component.
+and magnetic dipole, in that order; a group /tls/tl<n> (attributes Position,
+Resistance, dl; datasets Vinc, Iinc, Vtotal, Itotal) per transmission line;
+a group /rxs/rx<n> (attributes Name, Position) per receiver holding one
+dataset per output component.
 """
 
 import glob
@@ -24,9 +26,8 @@ GPRMAX_VERSION = '3.1.7'
 
 
 def write_hdf5_outputfile(outputfile, G, version=GPRMAX_VERSION):
-    """Write the receiver outputs of grid G (gprMax/fields_outputs.py:111-161).
-
-    Transmission lines are not part of this solver and are not written."""
+    """Write the receiver and transmission-line outputs of grid G
+    (gprMax/fields_outputs.py:111-161)."""
     f = Group()
     f.attrs['gprMax'] = version
     f.attrs['Title'] = G.title
@@ -43,6 +44,14 @@ def write_hdf5_outputfile(outputfile, G, version=GPRMAX_VERSION):
         grp = f.group('/srcs/src{}'.format(n + 1))
         grp.attrs['Type'] = type(src).__name__
         grp.attrs['Position'] = (src.xcoord * G.dx, src.ycoord * G.dy, src.zcoord * G.dz)
+    for n, tl in enumerate(getattr(G, 'transmissionlines', [])):
+        path = '/tls/tl{}'.format(n + 1)
+        grp = f.group(path)
+        grp.attrs['Position'] = (tl.xcoord * G.dx, tl.ycoord * G.dy, tl.zcoord * G.dz)
+        grp.attrs['Resistance'] = float(tl.resistance)
+        grp.attrs['dl'] = float(tl.dl)
+        for name in ('Vinc', 'Iinc', 'Vtotal', 'Itotal'):
+            f.dataset(path + '/' + name, getattr(tl, name))
     for n, rx in enumerate(G.rxs):
         path = '/rxs/rx{}'.format(n + 1)
         grp = f.group(path)

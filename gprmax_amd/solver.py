@@ -2,8 +2,9 @@
 
 ``solve_hip(currentmodelrun, modelend, G)`` has the signature and side effects
 of ``solve_gpu`` (gprMax/model_build_run.py:477-716): it runs all
-``G.iterations`` on the device, fills ``rx.outputs`` of every receiver and
-``snap.electric/magnetic`` of every snapshot, and returns
+``G.iterations`` on the device, fills ``rx.outputs`` of every receiver,
+``snap.electric/magnetic`` of every snapshot and ``Vtotal``/``Itotal`` of
+every transmission line, and returns
 ``(tsolve_seconds, memsolve_bytes)``.  ``GPU``/``detect_check_gpus`` stand in
 for gprMax/utilities.py:341-413 so the ``-gpu`` selection works unchanged.
 """
@@ -62,4 +63,10 @@ def solve_hip(currentmodelrun, modelend, G):
             scatter_rx_buffer(s.download_rx(), G)
         for n, snap in enumerate(G.snapshots):
             snap.set_from_components(*s.download_snapshot(n))
+        for n, tl in enumerate(getattr(G, 'transmissionlines', [])):
+            vt, it_, v, i = s.download_transmission_line(n)
+            tl.Vtotal[:] = vt
+            tl.Itotal[:] = it_
+            tl.voltage[:tl.nl] = v
+            tl.current[:tl.nl] = i
     return ms * 1e-3, memsolve

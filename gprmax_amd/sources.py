@@ -1,5 +1,6 @@
-"""Sources of the solve path: Hertzian dipoles, magnetic dipoles and voltage
-sources (gprMax/sources.py:33-283).
+"""Sources of the solve path: Hertzian dipoles, magnetic dipoles, voltage
+sources (gprMax/sources.py:33-283) and transmission lines
+(gprMax/sources.py:286-451).
 
 Waveforms are sampled on the host once per iteration, as the reference does
 (gprMax/sources.py:47-68).  For the device each source becomes one row of
@@ -11,7 +12,8 @@ from copy import deepcopy
 
 import numpy as np
 
-from .constants import IDlookup, floattype
+from .constants import IDlookup, c, floattype
+from .utilities import round_value
 
 POLARISATION = {'x': 0, 'y': 1, 'z': 2}
 
@@ -96,6 +98,100 @@ class VoltageSource(Source):
             new.se += G.dz / (self.resistance * G.dx * G.dy)
         G.ID[comp, i, j, k] = new.numID
         G.materials.append(new)
+
+
+class TransmissionLine(Source):
+    """A one-dimensional line of impedance ``resistance`` attached to an E
+    edge (gprMax/sources.py:286-451): a one-way injector at cell ``srcpos``
+    drives the line, whose cell ``antpos`` exchanges voltage and current with
+    the grid; an absorbing condition terminates the other end.
+
+    The line arithmetic keeps the reference's operand types: its cell length
+    ``dl`` comes from ``np.sqrt`` (a NumPy float64), so under NumPy 2
+    promotion every line coefficient is a float64 scalar applied to float32
+    differences, rounded to float32 when stored.
+    """
+
+    def __init__(self, G):
+        super().__init__()
+        self.resistance = None
+        self.abcv0 = 0
+        self.abcv1 = 0
+        self.dl = np.sqrt(3) * c * G.dt
+        # a long line first, to record the incident voltage and current
+        self.nl = round_value(0.667 * G.iterations)
+        self.srcpos = 5
+        self.antpos = 10
+        ftype = getattr(G, 'floattype', floattype)
+        self.voltage = np.zeros(self.nl, dtype=ftype)
+        self.current = np.zeros(self.nl, dtype=ftype)
+        self.Vtotal = np.zeros(G.iterations, dtype=ftype)
+        self.Itotal = np.zeros(G.iterations, dtype=ftype)
+
+    def coefficients(self, G):
+        """(R c dt/dl, (1/R) c dt/dl, c dt/dl, ABC factor), float64."""
+        cdt_dl = c * G.dt / self.dl
+        return (self.resistance * cdt_dl, (1 / self.resistance) * cdt_dl, cdt_dl,
+                (c * G.dt - self.dl) / (c * G.dt + self.dl))
+
+    def calculate_incident_V_I(self, G):
+        """Run the long line alone for every iteration, recording the voltage
+        and current at the antenna cell, then keep only the cells up to it
+        (gprMax/sources.py:335-358).  The kept cells carry the long run's
+        final state into the solve, as in the reference."""
+        ftype = self.voltage.dtype
+        self.Vinc = np.zeros(G.iterations, dtype=ftype)
+        self.Iinc = np.zeros(G.iterations, dtype=ftype)
+        for it in range(G.iterations):
+            self.Iinc[it] = self.current[self.antpos]
+            self.Vinc[it] = self.voltage[self.antpos]
+            self.update_current(it, G)
+            self.update_voltage(it, G)
+        self.nl = self.antpos + 1
+
+    def update_abc(self, G):
+        """First-order absorbing end at cell 0 (gprMax/sources.py:360-372)."""
+        h = self.coefficients(G)[3]
+        self.voltage[0] = h * (self.voltage[1] - self.abcv0) + self.abcv1
+        self.abcv0 = self.voltage[0]
+        self.abcv1 = self.voltage[1]
+
+    def update_voltage(self, it, G):
+        """Voltages from the current differences, the injector, the absorbing
+        end (gprMax/sources.py:374-388)."""
+        cV, _, cW, _ = self.coefficients(G)
+        n = self.nl
+        self.voltage[1:n] -= cV * (self.current[1:n] - self.current[:n - 1])
+        self.voltage[self.srcpos] += cW * self.waveformvalues_wholestep[it]
+        self.update_abc(G)
+
+    def update_current(self, it, G):
+        """Currents from the voltage differences and the injector
+        (gprMax/sources.py:390-402)."""
+        _, cI, _, _ = self.coefficients(G)
+        n = self.nl
+        self.current[:n - 1] -= cI * (self.voltage[1:n] - self.voltage[:n - 1])
+        self.current[self.srcpos - 1] += cI * self.waveformvalues_halfstep[it]
+
+
+def device_tl_args(tl, G):
+    """Arguments of gpx_add_transmission_line for line ``tl``: info int32[9],
+    coefficients float64[4], cell sizes float32[3], the whole- and half-step
+    waveforms, and the line state (voltage, current, ABC memory)."""
+    pol = POLARISATION[tl.polarisation]
+    first, last = tl.active_window(G)
+    info = np.array([tl.xcoord, tl.ycoord, tl.zcoord, pol, first, last, tl.nl, tl.srcpos, tl.antpos],
+                    dtype=np.int32)
+    coef = np.array(tl.coefficients(G), dtype=np.float64)
+    d = (G.dx, G.dy, G.dz)
+    loop = [d[a] for a in range(3) if a != pol]
+    sizes = np.array([d[pol]] + loop, dtype=np.float32)
+    n = tl.nl
+    abc = np.array([tl.abcv0, tl.abcv1], dtype=np.float32)
+    return (info, coef, sizes, np.ascontiguousarray(tl.waveformvalues_wholestep, dtype=np.float32),
+            np.ascontiguousarray(tl.waveformvalues_halfstep, dtype=np.float32),
+            np.ascontiguousarray(tl.voltage[:n], dtype=np.float32),
+            np.ascontiguousarray(tl.current[:n], dtype=np.float32), abc)
 
 
 def device_source_table(sources, G):

@@ -90,6 +90,31 @@ int gpx_add_pml(gpx_ctx *ctx, int direction, int xs, int xf, int ys, int yf, int
 int gpx_set_sources(gpx_ctx *ctx, int kind, int n, const int32_t *info, const float *factors,
                     const float *waves);
 
+/* A transmission line (gprMax/sources.py:286-451) attached to the E edge
+ * (i, j, k) of polarisation pol; the reference runs these on the CPU only.
+ * info int32 [9] = (i, j, k, pol, first_it, last_it, nl, srcpos, antpos);
+ * coef float64 [4] = (R c dt/dl, (1/R) c dt/dl, c dt/dl, ABC factor
+ * (c dt - dl)/(c dt + dl)); sizes float32 [3] = (cell size along pol, and the
+ * two sizes of the current loop: dy, dz for x; dx, dz for y; dx, dy for z);
+ * whole, half float32 [iterations] the waveform; voltage, current float32
+ * [nl] and abc float32 [2] the line state after the incident run
+ * (calculate_incident_V_I).  After the H pass the line records its antenna
+ * voltage and current, then updates its current and takes the current of
+ * the H loop around the edge; after the E pass it updates its voltage and
+ * sets the edge's E.  Models with transmission lines step in the two-pass
+ * form; dispersive models, a magnetic dipole on an H component the line
+ * reads, or a Hertzian dipole on its edge are refused at the first step
+ * (GPX_EARG).  Returns the line's index through *index. */
+int gpx_add_transmission_line(gpx_ctx *ctx, const int32_t *info, const double *coef, const float *sizes,
+                              const float *whole, const float *half, const float *voltage,
+                              const float *current, const float *abc, int *index);
+
+/* The antenna-end voltage and current of every iteration (Vtotal, Itotal,
+ * float32 [iterations]) and the line's voltage and current now (float32
+ * [nl]); any pointer may be NULL.  Waits for the ctx stream. */
+int gpx_download_transmission_line(gpx_ctx *ctx, int index, float *vtotal, float *itotal, float *voltage,
+                                   float *current);
+
 /* Receivers (gprMax/receivers.py:45-68): coords int32 [n][3]. */
 int gpx_set_receivers(gpx_ctx *ctx, int n, const int32_t *coords);
 

@@ -195,3 +195,23 @@ def test_snapshot_vti_file(tmp_path):
                 assert data[c + 3 * (i + snap.nx * (j + snap.ny * k))] == comps[3 * f + c][i, j, k]
     assert rest.endswith(b'\n</AppendedData>\n</VTKFile>')
     assert len(rest) == 2 * (4 + 12 * n) + len(b'\n</AppendedData>\n</VTKFile>')
+
+
+def test_transmission_line_groups(tmp_path):
+    """/tls/tl<n> per line (gprMax/fields_outputs.py:139-151): Position,
+    Resistance and dl attributes, Vinc, Iinc, Vtotal, Itotal datasets."""
+    from oracle import solver as osolver
+    from tl_models import tl_model
+    G = tl_model(iterations=40)
+    osolver.solve_cpu(1, 1, G, nthreads=4)
+    path = str(tmp_path / 'tl.out')
+    write_hdf5_outputfile(path, G)
+    d = hdf5.read_file(path)
+    assert d['/']['attrs']['nsrc'] == 3 + 3
+    for n, tl in enumerate(G.transmissionlines):
+        g = d['/tls/tl{}'.format(n + 1)]
+        assert np.allclose(g['attrs']['Position'], (tl.xcoord * 1e-3, tl.ycoord * 1e-3, tl.zcoord * 1e-3))
+        assert g['attrs']['Resistance'] == tl.resistance and g['attrs']['dl'] == tl.dl
+        for name in ('Vinc', 'Iinc', 'Vtotal', 'Itotal'):
+            got = d['/tls/tl{}/{}'.format(n + 1, name)]['data']
+            assert got.dtype == np.float32 and np.array_equal(got, getattr(tl, name))

@@ -1,0 +1,114 @@
+"""Transmission lines (gprMax/sources.py:286-451) on the host and in the
+oracle.
+
+The host computes the incident voltage and current with NumPy, vectorised
+as the reference writes it; the oracle restates every line update element
+by element with explicit float64/float32 casts.  Agreement of the two pins
+the promotion both rely on (a planted float32 coefficient breaks it).  The
+reference holds no transmission-line output files, so the coupled solve is
+pinned by the travelling-wave relation V = R I of the matched incident line
+and by the GPU comparison against the oracle; against the reference's own
+outputs the line is parity unpinned.
+"""
+
+import numpy as np
+import pytest
+
+from gprmax_amd.grid import ModelBuilder
+from gprmax_amd.sources import TransmissionLine
+from gprmax_amd.utilities import GeneralError
+from oracle import solver as osolver
+from tl_models import tl_model
+
+
+def _builder(its=150):
+    b = ModelBuilder(24, 22, 20, 1e-3, 1e-3, 1e-3, its, pml_cells=4)
+    b.add_waveform('gaussian', 1, 3e9, 'g')
+    return b
+
+
+def _oracle_incident(b, pol, cell, R, start=None, stop=None):
+    tl = TransmissionLine(b.G)
+    tl.resistance = float(R)
+    b._place(tl, pol, cell, 'g', start, stop)
+    V = np.zeros(b.G.iterations, np.float32)
+    I = np.zeros_like(V)
+    for it in range(b.G.iterations):
+        V[it] = tl.voltage[tl.antpos]
+        I[it] = tl.current[tl.antpos]
+        osolver.tl_update_current(tl, it, b.G)
+        osolver.tl_update_voltage(tl, it, b.G)
+    return tl, V, I
+
+
+@pytest.mark.parametrize('R', [20.0, 73.0, 300.0])
+def test_incident_run_matches_oracle(R):
+    b = _builder()
+    t = b.add_transmission_line('z', (12, 11, 10), R, 'g')
+    ref, V, I = _oracle_incident(b, 'z', (12, 11, 10), R)
+    assert np.array_equal(t.Vinc, V) and np.array_equal(t.Iinc, I)
+    assert t.nl == t.antpos + 1 == 11
+    assert np.array_equal(t.voltage[:11], ref.voltage[:11])
+    assert np.array_equal(t.current[:11], ref.current[:11])
+    assert (t.abcv0, t.abcv1) == (ref.abcv0, ref.abcv1)
+
+
+def test_float32_coefficients_break_the_match(monkeypatch):
+    """A planted mistake in the oracle (coefficients rounded to float32, the
+    promotion of NumPy 1 for array operations) changes the incident run."""
+    b = _builder()
+    t = b.add_transmission_line('z', (12, 11, 10), 73.0, 'g')
+    exact = osolver._tl_coefficients
+    monkeypatch.setattr(osolver, '_tl_coefficients',
+                        lambda tl, G: tuple(float(np.float32(x)) for x in exact(tl, G)))
+    _, V, _ = _oracle_incident(b, 'z', (12, 11, 10), 73.0)
+    assert not np.array_equal(t.Vinc, V)
+
+
+@pytest.mark.parametrize('R', [30.0, 73.0, 200.0])
+def test_incident_wave_travels_on_a_matched_line(R):
+    """The one-way injector launches a wave toward the antenna cell, whose
+    voltage and current then satisfy V = R I (to the half-step offset
+    between them)."""
+    b = _builder(400)
+    t = b.add_transmission_line('z', (12, 11, 10), R, 'g')
+    V, I = t.Vinc.astype(np.float64), t.Iinc.astype(np.float64)
+    peak = np.argmax(np.abs(V))
+    assert abs(V[peak]) > 0
+    assert abs(V[peak] / I[peak] - R) < 0.1 * R
+    assert np.corrcoef(V, I)[0, 1] > 0.999
+
+
+def test_line_geometry_and_coefficients():
+    b = _builder(300)
+    t = b.add_transmission_line('x', (5, 6, 7), 50.0, 'g')
+    G = b.G
+    assert t.dl == np.sqrt(3) * 299792458.0 * G.dt
+    assert isinstance(t.dl, np.float64)
+    cV, cI, cW, h = t.coefficients(G)
+    assert abs(cW - 1 / np.sqrt(3)) < 1e-15
+    assert abs(cV - 50.0 * cW) <= 1e-12 * cV and abs(cI - cW / 50.0) <= 1e-12 * cI
+    assert h < 0                       # dl > c dt: the ABC factor is negative
+    assert t.ID == 'TransmissionLine(5,6,7)'
+
+
+@pytest.mark.parametrize('R', [0.0, -5.0, 376.74, 1000.0])
+def test_resistance_out_of_range_is_refused(R):
+    b = _builder()
+    with pytest.raises(GeneralError):
+        b.add_transmission_line('z', (12, 11, 10), R, 'g')
+
+
+def test_coupled_oracle_solve_records_line_outputs():
+    """Vtotal/Itotal hold the antenna cell at the start of each iteration, and
+    the edge's E is -V/d after the last active iteration."""
+    G = tl_model(iterations=100)
+    osolver.solve_cpu(1, 1, G, nthreads=4)
+    z = G.transmissionlines[0]
+    assert np.any(z.Vtotal != 0) and np.any(z.Itotal != 0)
+    assert G.Ez[12, 11, 10] == -np.float32(z.voltage[z.antpos]) / np.float32(G.dz)
+    # the windowed line stops at its stop time: its state freezes there
+    x = G.transmissionlines[1]
+    last = max(it for it in range(G.iterations) if x.start <= it * G.dt <= x.stop)
+    assert last < G.iterations - 1
+    assert x.Vtotal[last + 1] == x.Vtotal[-1]
