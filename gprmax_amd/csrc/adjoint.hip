@@ -88,6 +88,7 @@ struct InvArgs {
     // and no PML term: the adjoint kernels take a short path there.
     int fast[6];
     int shell[6][6];                     // [lo x, y, z, hi x, y, z] of the shell boxes
+    int mix_nf, mix_ns, mix_gf[2], mix_gs;  // block counts and grid sizes of the mixed launches
 };
 
 // Grid offsets are 32-bit: gpx_inv_create requires fewer than 2^31 points.
@@ -325,22 +326,22 @@ __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k
 // FAST = true runs over the points of the fast box only (every lane of a wave
 // takes the short path); FAST = false over all other points.
 template <bool FAST>
-__global__ void k_adj_e(InvArgs a)
+__device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, int bz)
 {
     {
         // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
         int i, j, k;
         if (FAST) {
-            k = a.fast[4] + blockIdx.x * blockDim.x + threadIdx.x;
-            j = a.fast[2] + blockIdx.y;
-            i = a.fast[0] + blockIdx.z;
+            k = a.fast[4] + bx * blockDim.x + threadIdx.x;
+            j = a.fast[2] + by;
+            i = a.fast[0] + bz;
             if (k >= a.fast[5]) return;
         } else {
             // the shell around the fast box, as up to six disjoint boxes
             // (blockIdx.y), each enumerated flat so every lane has a point
-            const int *lo = a.shell[blockIdx.y], *hi = a.shell[blockIdx.y] + 3;
+            const int *lo = a.shell[by], *hi = a.shell[by] + 3;
             const int sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
-            const int t = blockIdx.x * blockDim.x + threadIdx.x;
+            const int t = bx * blockDim.x + threadIdx.x;
             if (sx <= 0 || sy <= 0 || sz <= 0 || t >= sx * sy * sz) return;
             k = lo[2] + t % sz;
             j = lo[1] + (t / sz) % sy;
@@ -449,6 +450,12 @@ __global__ void k_adj_e(InvArgs a)
     }
 }
 
+template <bool FAST>
+__global__ void k_adj_e(InvArgs a)
+{
+    adj_e_point<FAST>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // DB times lH_c at H cell (i, j, k), when that cell is updated.
 __device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int k)
 {
@@ -463,22 +470,22 @@ __device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int 
 // FAST = true runs over the points of the fast box only (every lane of a wave
 // takes the short path); FAST = false over all other points.
 template <bool FAST>
-__global__ void k_adj_h(InvArgs a)
+__device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, int bz)
 {
     {
         // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
         int i, j, k;
         if (FAST) {
-            k = a.fast[4] + blockIdx.x * blockDim.x + threadIdx.x;
-            j = a.fast[2] + blockIdx.y;
-            i = a.fast[0] + blockIdx.z;
+            k = a.fast[4] + bx * blockDim.x + threadIdx.x;
+            j = a.fast[2] + by;
+            i = a.fast[0] + bz;
             if (k >= a.fast[5]) return;
         } else {
             // the shell around the fast box, as up to six disjoint boxes
             // (blockIdx.y), each enumerated flat so every lane has a point
-            const int *lo = a.shell[blockIdx.y], *hi = a.shell[blockIdx.y] + 3;
+            const int *lo = a.shell[by], *hi = a.shell[by] + 3;
             const int sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
-            const int t = blockIdx.x * blockDim.x + threadIdx.x;
+            const int t = bx * blockDim.x + threadIdx.x;
             if (sx <= 0 || sy <= 0 || sz <= 0 || t >= sx * sy * sz) return;
             k = lo[2] + t % sz;
             j = lo[1] + (t / sz) % sy;
@@ -535,6 +542,38 @@ __global__ void k_adj_h(InvArgs a)
             }
         }
     }
+}
+
+template <bool FAST>
+__global__ void k_adj_h(InvArgs a)
+{
+    adj_h_point<FAST>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// The fast-box and shell blocks of one half-step in one launch: the shell
+// blocks (latency-bound gathers) are spread evenly among the fast ones
+// (bandwidth-bound) so that the two kinds run side by side.  Block b is a
+// shell block when floor((b+1) ns / (nf+ns)) > floor(b ns / (nf+ns)).
+__device__ __forceinline__ bool mix_shell(const InvArgs &a, int b, int &idx)
+{
+    const long long tot = (long long)a.mix_nf + a.mix_ns;
+    const long long c0 = (long long)b * a.mix_ns / tot, c1 = (long long)(b + 1) * a.mix_ns / tot;
+    idx = c1 > c0 ? (int)c0 : (int)(b - c1);
+    return c1 > c0;
+}
+
+__global__ void k_adj_e_mix(InvArgs a)
+{
+    int q;
+    if (mix_shell(a, blockIdx.x, q)) adj_e_point<false>(a, q % a.mix_gs, q / a.mix_gs, 0);
+    else adj_e_point<true>(a, q % a.mix_gf[0], (q / a.mix_gf[0]) % a.mix_gf[1], q / (a.mix_gf[0] * a.mix_gf[1]));
+}
+
+__global__ void k_adj_h_mix(InvArgs a)
+{
+    int q;
+    if (mix_shell(a, blockIdx.x, q)) adj_h_point<false>(a, q % a.mix_gs, q / a.mix_gs, 0);
+    else adj_h_point<true>(a, q % a.mix_gf[0], (q / a.mix_gf[0]) % a.mix_gf[1], q / (a.mix_gf[0] * a.mix_gf[1]));
 }
 
 // lEz at the receivers += dL/d(trace sample of iteration it).
@@ -988,11 +1027,21 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
                 a.slab[s].PE1 = P[0]; a.slab[s].PE2 = P[1]; a.slab[s].PH1 = P[2]; a.slab[s].PH2 = P[3];
                 a.slab[s].QE1 = Q[0]; a.slab[s].QE2 = Q[1]; a.slab[s].QH1 = Q[2]; a.slab[s].QH2 = Q[3];
             }
-            if (!nofast) hipLaunchKernelGGL(k_adj_e<true>, GF, dim3(128), 0, v->stream, a);
-            hipLaunchKernelGGL(k_adj_e<false>, GS, dim3(128), 0, v->stream, a);
-            for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
-            if (!nofast) hipLaunchKernelGGL(k_adj_h<true>, GF, dim3(128), 0, v->stream, a);
-            hipLaunchKernelGGL(k_adj_h<false>, GS, dim3(128), 0, v->stream, a);
+            if (nofast) {
+                hipLaunchKernelGGL(k_adj_e<false>, GS, dim3(128), 0, v->stream, a);
+                for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
+                hipLaunchKernelGGL(k_adj_h<false>, GS, dim3(128), 0, v->stream, a);
+            } else {
+                a.mix_nf = (int)(GF.x * GF.y * GF.z);
+                a.mix_ns = (int)(GS.x * GS.y);
+                a.mix_gf[0] = (int)GF.x;
+                a.mix_gf[1] = (int)GF.y;
+                a.mix_gs = (int)GS.x;
+                const dim3 GM(a.mix_nf + a.mix_ns);
+                hipLaunchKernelGGL(k_adj_e_mix, GM, dim3(128), 0, v->stream, a);
+                for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
+                hipLaunchKernelGGL(k_adj_h_mix, GM, dim3(128), 0, v->stream, a);
+            }
             if (v->nrx) hipLaunchKernelGGL(k_adj_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
             cur ^= 1;
         }
