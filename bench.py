@@ -260,7 +260,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this solver runs on one GPU; --gpus {} is not supported '
-              '(one-GPU boxes only; multi-GPU sharding is a design, not a build)'.format(args.gpus),
+              '(sharding over several GPUs is designed, not built)'.format(args.gpus),
               file=sys.stderr)
         return 2
 
