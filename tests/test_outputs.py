@@ -207,7 +207,7 @@ def test_transmission_line_groups(tmp_path):
     path = str(tmp_path / 'tl.out')
     write_hdf5_outputfile(path, G)
     d = hdf5.read_file(path)
-    assert d['/']['attrs']['nsrc'] == 3 + 3
+    assert d['/']['attrs']['nsrc'] == 3 + 4
     for n, tl in enumerate(G.transmissionlines):
         g = d['/tls/tl{}'.format(n + 1)]
         assert np.allclose(g['attrs']['Position'], (tl.xcoord * 1e-3, tl.ycoord * 1e-3, tl.zcoord * 1e-3))

@@ -112,3 +112,6 @@ def test_coupled_oracle_solve_records_line_outputs():
     last = max(it for it in range(G.iterations) if x.start <= it * G.dt <= x.stop)
     assert last < G.iterations - 1
     assert x.Vtotal[last + 1] == x.Vtotal[-1]
+    # on the y = 0 face the loop current is 0 (gprMax/grid.py:422-423)
+    face = G.transmissionlines[3]
+    assert face.ycoord == 0 and np.all(face.Itotal[1:] == 0)

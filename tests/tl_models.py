@@ -6,8 +6,9 @@ from gprmax_amd.pml import CFS
 
 def tl_model(iterations=120, order=1, formulation='HORIPML', extra_sources=True):
     """A 24x22x20 grid with 4-cell PML, a soil layer, a line of each
-    polarisation (one with a start/stop window), point sources away from the
-    lines' edges and receivers, one on a line's edge."""
+    polarisation (one with a start/stop window) and one on a grid face,
+    point sources away from the lines' edges and receivers, one on a line's
+    edge."""
     cfs = None
     if order == 2:
         a, b = CFS(), CFS()
@@ -23,6 +24,8 @@ def tl_model(iterations=120, order=1, formulation='HORIPML', extra_sources=True)
     bld.add_transmission_line('z', (12, 11, 10), 73, 'g')
     bld.add_transmission_line('x', (8, 9, 12), 50, 'r', start=1e-11, stop=1.5e-10)
     bld.add_transmission_line('y', (15, 6, 14), 120, 'g')
+    # on the y = 0 face, inside the PML: its loop current is 0 there
+    bld.add_transmission_line('x', (5, 0, 7), 60, 'g')
     if extra_sources:
         bld.add_hertzian_dipole('z', (6, 15, 9), 'r')
         bld.add_magnetic_dipole('x', (17, 14, 6), 'g')
