@@ -123,9 +123,11 @@ int gpx_set_receivers(gpx_ctx *ctx, int n, const int32_t *coords);
 int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, int dy, int dz,
                      int snx, int sny, int snz, int time, int *index);
 
-/* The time loop: iterations [it0, it0 + n) with all sub-steps fused into one
- * H pass and one E pass per iteration (replaces model_build_run.py:590-696).
- * Enqueued on the ctx stream; returns without waiting. */
+/* The time loop: iterations [it0, it0 + n), every sub-step of an iteration
+ * inside the kernels of the step form set by gpx_set_step (replaces
+ * model_build_run.py:590-696).  Enqueued on the ctx stream; returns without
+ * waiting, except that under the default form the first run of at least 26
+ * iterations waits once, after its 14th iteration, to read its timing. */
 int gpx_run(gpx_ctx *ctx, int it0, int n);
 
 /* As gpx_run, and reports the elapsed device time measured with HIP events
