@@ -455,3 +455,31 @@ def test_pass_batch_of_unlike_models():
             for s in solvers:
                 s.close()
 
+
+
+def test_default_form_times_once_and_resets():
+    """The default form times the split step and the passes on the first
+    run of at least 26 iterations only: short runs and profiled runs leave
+    the choice open, gpx_set_step clears it, and the results stay the
+    oracle's."""
+    G = rich_model(seed=31, shape=WIDE, iterations=60, snapshots=False)
+    Gc = rich_model(seed=31, shape=WIDE, iterations=60, snapshots=False)
+    with _hip()(G) as s:
+        assert s.step_choice() == -1
+        s.run(0, 10)                       # too short to time
+        assert s.step_choice() == -1
+        s.run_profiled(10, 5)              # profiled runs never time
+        assert s.step_choice() == -1
+        s.run(15, 30)                      # times iterations 17-28, then keeps the faster
+        assert s.step_choice() in (0, 1)
+        s.set_step(4)
+        assert s.step_choice() == -1
+        s.run(45, 15)
+        F = s.download_fields()
+        rxb = s.download_rx()
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    for n, c in enumerate(COMPS):
+        assert np.array_equal(F[n], getattr(Gc, c)), c
+    for r, rx in enumerate(Gc.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
