@@ -787,6 +787,11 @@ static int fail_hip(gpx_ctx *ctx, hipError_t e, const char *what)
     return fail(ctx, (int)e, msg);
 }
 
+static int check_hip(gpx_ctx *ctx, hipError_t e)
+{
+    return e == hipSuccess ? 0 : fail_hip(ctx, e, "HIP call of the step-form timing");
+}
+
 static hipError_t dalloc(gpx_ctx *ctx, void **p, size_t bytes, bool zero = true)
 {
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
@@ -1925,6 +1930,16 @@ static int e_flags(const gpx_ctx *ctx)
     return f;
 }
 
+// Timing events, destroyed when the set goes out of scope.
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    ~EventSet()
+    {
+        for (hipEvent_t x : e)
+            if (x) hipEventDestroy(x);
+    }
+};
+
 // Runs iterations [it0, it0+n) in the step form set by gpx_set_step.  With
 // `ms4`, events around every kernel give the mean duration of each kernel of
 // an iteration over the run: the fused kernel in ms4[0] (the rest 0); the
@@ -1943,39 +1958,30 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
     if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || !split_eligible(ctx) ||
         n < 2 + 4 * TUNE_ITS)
         return run_steps_form(ctx, it0, n, ms4);
-    hipEvent_t e[3];
-    for (auto &x : e) GPX_TRY(hipEventCreate(&x));
+    EventSet ev;
+    ev.e.assign(3, nullptr);
+    for (auto &x : ev.e) GPX_TRY(hipEventCreate(&x));
     ctx->split_choice = 1;
     int rc = run_steps_form(ctx, it0, 2, nullptr);
-    if (!rc) { GPX_TRY(hipEventRecord(e[0], ctx->stream)); rc = run_steps_form(ctx, it0 + 2, TUNE_ITS, nullptr); }
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[0], ctx->stream));
+    if (!rc) rc = run_steps_form(ctx, it0 + 2, TUNE_ITS, nullptr);
     ctx->split_choice = 0;
-    if (!rc) { GPX_TRY(hipEventRecord(e[1], ctx->stream)); rc = run_steps_form(ctx, it0 + 2 + TUNE_ITS, TUNE_ITS, nullptr); }
-    if (!rc) {
-        GPX_TRY(hipEventRecord(e[2], ctx->stream));
-        GPX_TRY(hipEventSynchronize(e[2]));
-        float t_split = 0, t_pass = 0;
-        GPX_TRY(hipEventElapsedTime(&t_split, e[0], e[1]));
-        GPX_TRY(hipEventElapsedTime(&t_pass, e[1], e[2]));
-        ctx->split_choice = t_split <= t_pass ? 1 : 0;
-    }
-    for (auto &x : e) hipEventDestroy(x);
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[1], ctx->stream));
+    if (!rc) rc = run_steps_form(ctx, it0 + 2 + TUNE_ITS, TUNE_ITS, nullptr);
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[2], ctx->stream));
+    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[2]));
+    float t_split = 0, t_pass = 0;
+    if (!rc) rc = check_hip(ctx, hipEventElapsedTime(&t_split, ev.e[0], ev.e[1]));
+    if (!rc) rc = check_hip(ctx, hipEventElapsedTime(&t_pass, ev.e[1], ev.e[2]));
     if (rc) {
         ctx->split_choice = -1;
         return rc;
     }
+    ctx->split_choice = t_split <= t_pass ? 1 : 0;
     const int done = 2 + 2 * TUNE_ITS;
     return run_steps_form(ctx, it0 + done, n - done, nullptr);
 }
 
-// Timing events of a profiled run, destroyed with the run.
-struct EventSet {
-    std::vector<hipEvent_t> e;
-    ~EventSet()
-    {
-        for (hipEvent_t x : e)
-            if (x) hipEventDestroy(x);
-    }
-};
 
 static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
