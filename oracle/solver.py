@@ -9,7 +9,10 @@ gprMax/sources.py:81-232 writes it, so their rounding is the reference's.
 Transmission lines (gprMax/sources.py:286-451) are restated element by
 element with explicit casts: float64 products of float32 differences,
 rounded to float32 when stored (the NumPy 2 promotion of the reference's
-expressions, whose line coefficients are NumPy float64 scalars).
+expressions, whose line coefficients are NumPy float64 scalars).  Against the
+reference's outputs these line functions are parity unpinned (no output file
+of the reference holds a line); they are pinned against the host's
+vectorised incident run and the matched-line relation V = R I.
 """
 
 import os
