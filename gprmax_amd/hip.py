@@ -95,6 +95,16 @@ def library():
     return lib
 
 
+def grid_maxpoles(G):
+    """Dispersive poles of grid G: ``G.maxpoles`` on this package's grid; the
+    reference keeps it on its Material class and sizes
+    ``updatecoeffsdispersive`` [nmat][3*maxpoles] by it."""
+    if hasattr(G, 'maxpoles'):
+        return int(G.maxpoles)
+    d = getattr(G, 'updatecoeffsdispersive', None)
+    return int(d.shape[1]) // 3 if d is not None and d.ndim == 2 else 0
+
+
 def _arr(a, dtype):
     a = np.ascontiguousarray(a, dtype=dtype)
     return a, a.ctypes.data_as(ctypes.c_void_p)
@@ -110,9 +120,10 @@ class HipSolver:
     def __init__(self, G, device=0):
         self.lib = library()
         self.G = G
-        if G.floattype != np.float32:
+        if getattr(G, 'floattype', np.float32) != np.float32:
             raise GeneralError('the HIP solver computes in float32')
-        desc = GpxDesc(G.nx, G.ny, G.nz, G.iterations, len(G.updatecoeffsE), G.maxpoles,
+        self.maxpoles = grid_maxpoles(G)
+        desc = GpxDesc(G.nx, G.ny, G.nz, G.iterations, len(G.updatecoeffsE), self.maxpoles,
                        0 if G.pmlformulation == 'HORIPML' else 1, device)
         ctx = ctypes.c_void_p()
         rc = self.lib.gpx_create(ctypes.byref(desc), ctypes.byref(ctx))
@@ -125,7 +136,7 @@ class HipSolver:
         e, pe = _arr(G.updatecoeffsE, np.float32)
         h, ph = _arr(G.updatecoeffsH, np.float32)
         self._check(self.lib.gpx_upload_coeffs(ctx, pe, ph))
-        if G.maxpoles > 0:
+        if self.maxpoles > 0:
             d = np.ascontiguousarray(G.updatecoeffsdispersive, dtype=np.complex64)
             self._check(self.lib.gpx_upload_dispersive(ctx, d.ctypes.data_as(ctypes.c_void_p)))
         for pml in G.pmls:
@@ -139,7 +150,7 @@ class HipSolver:
         for kind, sources in (('hertzian', G.hertziandipoles), ('magnetic', G.magneticdipoles),
                               ('voltage', G.voltagesources)):
             if sources:
-                info, fac, waves = device_source_table(sources, G)
+                info, fac, waves = device_source_table(sources, G, kind)
                 self._check(self.lib.gpx_set_sources(ctx, SRC_KIND[kind], len(sources),
                                                      _arr(info, np.int32)[1], _arr(fac, np.float32)[1],
                                                      _arr(waves, np.float32)[1]))
@@ -277,7 +288,7 @@ class HipSolver:
 
     def download_T(self):
         G = self.G
-        out = np.empty((3, G.maxpoles, G.nx + 1, G.ny + 1, G.nz + 1), dtype=np.complex64)
+        out = np.empty((3, self.maxpoles, G.nx + 1, G.ny + 1, G.nz + 1), dtype=np.complex64)
         self._check(self.lib.gpx_download_dispersive_T(self.ctx, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 

@@ -77,3 +77,15 @@ class Snapshot:
                 if pbar is not None:
                     pbar.update(n=4 + datasize)
             fh.write('\n</AppendedData>\n</VTKFile>'.encode('utf-8'))
+
+
+def store_components(snap, Ex, Ey, Ez, Hx, Hy, Hz):
+    """Store a snapshot's six [nx, ny, nz] components on `snap`: this
+    package's Snapshot keeps them and builds the Paraview order; on the
+    reference's Snapshot the ``electric`` and ``magnetic`` arrays are set as
+    gprMax/snapshots.py:214-228 sets them."""
+    if hasattr(snap, 'set_from_components'):
+        snap.set_from_components(Ex, Ey, Ez, Hx, Hy, Hz)
+        return
+    snap.electric = np.stack([np.asarray(a, dtype=floattype) for a in (Ex, Ey, Ez)]).reshape(-1, order='F')
+    snap.magnetic = np.stack([np.asarray(a, dtype=floattype) for a in (Hx, Hy, Hz)]).reshape(-1, order='F')

@@ -13,6 +13,7 @@ import os
 
 from .hip import HipSolver, device_info
 from .receivers import scatter_rx_buffer
+from .snapshots import store_components
 from .utilities import GeneralError
 
 
@@ -62,7 +63,7 @@ def solve_hip(currentmodelrun, modelend, G):
         if G.rxs:
             scatter_rx_buffer(s.download_rx(), G)
         for n, snap in enumerate(G.snapshots):
-            snap.set_from_components(*s.download_snapshot(n))
+            store_components(snap, *s.download_snapshot(n))
         for n, tl in enumerate(getattr(G, 'transmissionlines', [])):
             vt, it_, v, i = s.download_transmission_line(n)
             tl.Vtotal[:] = vt

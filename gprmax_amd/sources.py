@@ -45,16 +45,21 @@ class Source:
                 self.waveformvalues_halfstep[it] = waveform.calculate_value(time + 0.5 * G.dt, G.dt)
 
     def active_window(self, G):
-        """First and last iteration at which the CPU path applies the source.
+        return active_window(self, G)
 
-        The CPU update runs only when ``start <= it*dt <= stop``
-        (gprMax/sources.py:92,174,213); the window is evaluated with the same
-        float64 products so the device gating matches it exactly.
-        """
-        active = [it for it in range(G.iterations) if self.start <= it * G.dt <= self.stop]
-        if not active:
-            return 1, 0
-        return active[0], active[-1]
+
+def active_window(src, G):
+    """First and last iteration at which the CPU path applies source `src`
+    (any object with ``start`` and ``stop``, the reference's included).
+
+    The CPU update runs only when ``start <= it*dt <= stop``
+    (gprMax/sources.py:92,174,213); the window is evaluated with the same
+    float64 products so the device gating matches it exactly.
+    """
+    active = [it for it in range(G.iterations) if src.start <= it * G.dt <= src.stop]
+    if not active:
+        return 1, 0
+    return active[0], active[-1]
 
 
 class HertzianDipole(Source):
@@ -175,14 +180,18 @@ class TransmissionLine(Source):
 
 
 def device_tl_args(tl, G):
-    """Arguments of gpx_add_transmission_line for line ``tl``: info int32[9],
-    coefficients float64[4], cell sizes float32[3], the whole- and half-step
-    waveforms, and the line state (voltage, current, ABC memory)."""
+    """Arguments of gpx_add_transmission_line for line ``tl`` (this module's
+    TransmissionLine or the reference's: only their attributes are read):
+    info int32[9], coefficients float64[4], cell sizes float32[3], the whole-
+    and half-step waveforms, and the line state (voltage, current, ABC
+    memory)."""
     pol = POLARISATION[tl.polarisation]
-    first, last = tl.active_window(G)
+    first, last = active_window(tl, G)
     info = np.array([tl.xcoord, tl.ycoord, tl.zcoord, pol, first, last, tl.nl, tl.srcpos, tl.antpos],
                     dtype=np.int32)
-    coef = np.array(tl.coefficients(G), dtype=np.float64)
+    cdt_dl = c * G.dt / tl.dl            # as TransmissionLine.coefficients
+    coef = np.array([tl.resistance * cdt_dl, (1 / tl.resistance) * cdt_dl, cdt_dl,
+                     (c * G.dt - tl.dl) / (c * G.dt + tl.dl)], dtype=np.float64)
     d = (G.dx, G.dy, G.dz)
     loop = [d[a] for a in range(3) if a != pol]
     sizes = np.array([d[pol]] + loop, dtype=np.float32)
@@ -194,7 +203,10 @@ def device_tl_args(tl, G):
             np.ascontiguousarray(tl.current[:n], dtype=np.float32), abc)
 
 
-def device_source_table(sources, G):
+SOURCE_KINDS = ('hertzian', 'magnetic', 'voltage')
+
+
+def device_source_table(sources, G, kind=None):
     """Pack one kind of sources for the device.
 
     Returns ``(info1 int32[n,6], factors float32[n,2], waves float32[n,iterations])``.
@@ -207,6 +219,10 @@ def device_source_table(sources, G):
     * resistive voltage source: ``(1/(R*d_a*d_b), 0)`` on the whole-step wave;
     * hard voltage source: ``(d_pol, 0)`` on the half-step wave, ``R == 0``
       marked by ``pol + 4`` in column 3.
+
+    ``kind`` ('hertzian', 'magnetic' or 'voltage') names the list the sources
+    come from, so the reference's own source objects pack too; without it the
+    kind follows this module's classes.
     """
     n = len(sources)
     info1 = np.zeros((n, 6), dtype=np.int32)
@@ -216,15 +232,18 @@ def device_source_table(sources, G):
     steps = {'x': G.dx, 'y': G.dy, 'z': G.dz}
     for s, src in enumerate(sources):
         pol = POLARISATION[src.polarisation]
-        first, last = src.active_window(G)
+        first, last = active_window(src, G)
         info1[s] = (src.xcoord, src.ycoord, src.zcoord, pol, first, last)
-        if isinstance(src, HertzianDipole):
+        k = kind or ('hertzian' if isinstance(src, HertzianDipole) else
+                     'magnetic' if isinstance(src, MagneticDipole) else
+                     'voltage' if isinstance(src, VoltageSource) else None)
+        if k == 'hertzian':
             factors[s] = (src.dl, inv_vol)
             waves[s] = src.waveformvalues_wholestep
-        elif isinstance(src, MagneticDipole):
+        elif k == 'magnetic':
             factors[s] = (inv_vol, 0)
             waves[s] = src.waveformvalues_halfstep
-        elif isinstance(src, VoltageSource):
+        elif k == 'voltage':
             if src.resistance != 0:
                 other = [steps[a] for a in 'xyz' if a != src.polarisation]
                 factors[s] = (1 / (src.resistance * other[0] * other[1]), 0)

@@ -295,3 +295,33 @@ def test_vectorised_edge_build_matches_edge_loop(seed):
     assert len(G1.materials) > 30
     for a, b in zip(G1.materials, G2.materials):
         assert (a.er, a.se, a.mr, a.sm) == (b.er, b.se, b.mr, b.sm)
+
+
+def test_device_tables_from_reference_like_objects():
+    """solve_hip reads attributes only: the reference's own source, line,
+    snapshot and grid objects (which lack this package's helper methods)
+    pack into the same device tables."""
+    from types import SimpleNamespace
+    from gprmax_amd.hip import grid_maxpoles
+    from gprmax_amd.snapshots import store_components
+    from gprmax_amd.sources import device_source_table, device_tl_args
+    from tl_models import tl_model
+    G = tl_model(iterations=40)
+    plain = lambda o: SimpleNamespace(**{k: v for k, v in vars(o).items()})
+    for kind, srcs in (('hertzian', G.hertziandipoles), ('magnetic', G.magneticdipoles),
+                       ('voltage', G.voltagesources)):
+        ours = device_source_table(srcs, G)
+        theirs = device_source_table([plain(s) for s in srcs], G, kind)
+        for a, b in zip(ours, theirs):
+            assert np.array_equal(a, b), kind
+    for tl in G.transmissionlines:
+        for a, b in zip(device_tl_args(tl, G), device_tl_args(plain(tl), G)):
+            assert np.array_equal(a, b)
+        assert np.array_equal(device_tl_args(tl, G)[1], np.array(tl.coefficients(G)))
+    g = SimpleNamespace(updatecoeffsdispersive=np.zeros((7, 9), np.complex64))
+    assert grid_maxpoles(g) == 3 and grid_maxpoles(SimpleNamespace()) == 0 and grid_maxpoles(G) == G.maxpoles
+    snap = SimpleNamespace()
+    comps = [np.full((2, 3, 4), v, np.float32) for v in range(6)]
+    store_components(snap, *comps)
+    assert np.array_equal(snap.electric, np.stack(comps[:3]).reshape(-1, order='F'))
+    assert np.array_equal(snap.magnetic, np.stack(comps[3:]).reshape(-1, order='F'))
