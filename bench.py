@@ -29,31 +29,44 @@ METRIC = 'Mcells/s + % HBM roofline; batched B-scan and adjoint-grad time vs pla
 
 
 def cpu_baseline(workload, seconds=12.0):
-    """CPU oracle (C restatement of the reference's OpenMP kernels) on the host
-    cores, on the first iterations of the same workload, until ~`seconds` of
-    CPU work have run."""
+    """The reference's CPU path on the host cores, on the first iterations of
+    the same workload, until ~`seconds` of CPU work have run: its own Cython
+    kernels compiled from its sources into oracle/_ref (kind "reference"),
+    or, where those are not built, the oracle's C restatement of them (kind
+    "port").  Both give the same bits; the source updates are NumPy in both."""
     import numpy as np  # noqa: F401
     from gprmax_amd.workloads import WORKLOADS
     from oracle import load
-    from oracle.solver import default_threads, initialise_state, step
+    from oracle.solver import default_threads, initialise_state, load_reference, step, step_reference
 
     G = WORKLOADS[workload]()
-    lib = load(G.floattype)
     threads = default_threads()
+    mods = load_reference('exact')
+    if mods is not None:
+        kind, what = 'reference', "the reference's Cython kernels built from its sources (-O3 -fopenmp -ffp-contract=off)"
+
+        def one(it):
+            step_reference(mods, G, it, threads)
+    else:
+        lib = load(G.floattype)
+        kind, what = 'port', 'the C restatement of the reference kernels'
+
+        def one(it):
+            step(lib, G, it, threads)
     initialise_state(G)
-    step(lib, G, 0, threads)  # first touch of every array
+    one(0)  # first touch of every array
     n, t0 = 0, time.perf_counter()
     while True:
-        step(lib, G, n + 1, threads)
+        one(n + 1)
         n += 1
         el = time.perf_counter() - t0
         if el >= seconds or n + 2 >= G.iterations:
             break
     cells = G.nx * G.ny * G.nz
     return {'value': round(cells * n / el / 1e6, 2), 'unit': 'Mcells/s', 'cores': threads,
-            'kind': 'port',
-            'sample': '{} iterations (1..{}) of {} on {} OpenMP threads, {:.1f} s'.format(
-                n, n, G.title, threads, el)}
+            'kind': kind,
+            'sample': '{} iterations (1..{}) of {}, {}, {} OpenMP threads, {:.1f} s'.format(
+                n, n, G.title, what, threads, el)}
 
 
 def measured_traffic(workload, kernel):

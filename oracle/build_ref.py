@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE.  Used only in a container where the reference tree is
 mounted, to pin the C restatement in ``oracle/fdtd_oracle.c`` bit for bit and
 to time the reference's CPU kernels.  Nothing here is copied: the .pyx sources
 are compiled where they lie; the generated C and the built modules go to
-``oracle/_ref/`` (git-ignored, and not shipped to GPU runs).
+``oracle/_ref/`` (git-ignored; it travels with GPU runs, where bench.py times
+the exact build as the CPU baseline).
 
 Two builds are made:
   exact/   -O3 -fopenmp -ffp-contract=off   (every product rounded, no FMA)

@@ -224,3 +224,85 @@ def solve_cpu(currentmodelrun, modelend, G, nthreads=None, iterations=None):
     for it in range(n):
         step(lib, G, it, nthreads)
     return time.perf_counter() - t0
+
+
+# ---------------------------------------------------------------------------
+# The reference's own compiled kernels (oracle/_ref, built from its .pyx
+# sources by oracle/build_ref.py), driven in the same step order.  Used to
+# time the reference's CPU path beside the GPU; never as the thing measured.
+
+REF_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), '_ref')
+
+
+def load_reference(flavour='exact'):
+    """The compiled reference modules of one build flavour, or None when they
+    are not built here."""
+    import importlib
+    import sys
+    tree = os.path.join(REF_DIR, flavour)
+    if not os.path.isdir(tree):
+        return None
+    sys.path.insert(0, tree)
+    try:
+        mods = {'fields': importlib.import_module('gprMax.fields_updates_ext')}
+        for field in ('electric', 'magnetic'):
+            for form in ('HORIPML', 'MRIPML'):
+                mods[(field, form)] = importlib.import_module(
+                    'gprMax.pml_updates.pml_updates_{}_{}_ext'.format(field, form))
+    except ImportError:
+        return None
+    finally:
+        sys.path.remove(tree)
+    return mods
+
+
+def _ref_pml(mods, G, pml, field, nthreads):
+    """gprMax/pml.py:283-305: the slab's order<n>_<direction> kernel."""
+    name = 'electric' if field == 0 else 'magnetic'
+    fn = getattr(mods[(name, G.pmlformulation)], 'order{}_{}'.format(len(pml.CFS), pml.direction))
+    if field == 0:
+        fn(pml.xs, pml.xf, pml.ys, pml.yf, pml.zs, pml.zf, nthreads, G.updatecoeffsE, G.ID,
+           G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz, pml.EPhi1, pml.EPhi2, pml.ERA, pml.ERB, pml.ERE, pml.ERF, pml.d)
+    else:
+        fn(pml.xs, pml.xf, pml.ys, pml.yf, pml.zs, pml.zf, nthreads, G.updatecoeffsH, G.ID,
+           G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz, pml.HPhi1, pml.HPhi2, pml.HRA, pml.HRB, pml.HRE, pml.HRF, pml.d)
+
+
+def step_reference(mods, G, it, nthreads):
+    """One iteration of gprMax/model_build_run.py:424-470 with the reference's
+    compiled kernels and the NumPy source updates of this module."""
+    fu = mods['fields']
+    P = G.maxpoles
+    store_outputs(it, G)
+    fu.update_magnetic(G.nx, G.ny, G.nz, nthreads, G.updatecoeffsH, G.ID, G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz)
+    for pml in G.pmls:
+        _ref_pml(mods, G, pml, 1, nthreads)
+    for tl in G.transmissionlines:
+        tl_update_magnetic(tl, it, G)
+    for src in G.magneticdipoles:
+        magnetic_dipole_update(src, it, G)
+    T = (G.Tx, G.Ty, G.Tz) if P else ()
+    if P == 0:
+        fu.update_electric(G.nx, G.ny, G.nz, nthreads, G.updatecoeffsE, G.ID, G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz)
+    elif P == 1:
+        fu.update_electric_dispersive_1pole_A(G.nx, G.ny, G.nz, nthreads, G.updatecoeffsE,
+                                              G.updatecoeffsdispersive, G.ID, *T,
+                                              G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz)
+    else:
+        fu.update_electric_dispersive_multipole_A(G.nx, G.ny, G.nz, nthreads, P, G.updatecoeffsE,
+                                                  G.updatecoeffsdispersive, G.ID, *T,
+                                                  G.Ex, G.Ey, G.Ez, G.Hx, G.Hy, G.Hz)
+    for pml in G.pmls:
+        _ref_pml(mods, G, pml, 0, nthreads)
+    for src in G.voltagesources:
+        voltage_source_update(src, it, G)
+    for tl in G.transmissionlines:
+        tl_update_electric(tl, it, G)
+    for src in G.hertziandipoles:
+        hertzian_dipole_update(src, it, G)
+    if P == 1:
+        fu.update_electric_dispersive_1pole_B(G.nx, G.ny, G.nz, nthreads, G.updatecoeffsdispersive, G.ID,
+                                              *T, G.Ex, G.Ey, G.Ez)
+    elif P > 1:
+        fu.update_electric_dispersive_multipole_B(G.nx, G.ny, G.nz, nthreads, P, G.updatecoeffsdispersive,
+                                                  G.ID, *T, G.Ex, G.Ey, G.Ez)

@@ -419,3 +419,23 @@ def test_float32_error_against_float64_on_the_100_cube():
         peak = np.abs(double[c]).max()
         assert np.abs(single[c] - double[c]).max() < 1e-4 * peak, c
     assert np.abs(single[0] - double[0]).max() < 1e-3 * np.abs(double[0]).max()  # Ex
+
+
+@pytest.mark.parametrize('poles', [0, 2])
+def test_reference_kernel_step_matches_the_port(ref_modules, poles):
+    """The step that drives the reference's compiled kernels (the CPU
+    baseline's) equals the oracle's C restatement bit for bit."""
+    from gpu_models import rich_model
+    from oracle import solver as osolver
+    mods = osolver.load_reference('exact')
+    G = rich_model(seed=5, iterations=20, poles=poles, snapshots=False)
+    Gc = rich_model(seed=5, iterations=20, poles=poles, snapshots=False)
+    osolver.initialise_state(G)
+    for it in range(G.iterations):
+        osolver.step_reference(mods, G, it, 4)
+    osolver.solve_cpu(1, 1, Gc, nthreads=4)
+    for c in ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz'):
+        assert np.array_equal(getattr(G, c), getattr(Gc, c)), c
+    for rx, rxc in zip(G.rxs, Gc.rxs):
+        for name in rx.outputs:
+            assert np.array_equal(rx.outputs[name], rxc.outputs[name])
