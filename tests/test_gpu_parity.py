@@ -483,3 +483,50 @@ def test_default_form_times_once_and_resets():
     for r, rx in enumerate(Gc.rxs):
         for c, name in enumerate(COMPS):
             assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
+
+
+@pytest.mark.parametrize('xchunk', [0, 3])
+def test_split_step_sources_on_recomputed_cells(xchunk):
+    """Magnetic dipoles on the cells whose new H the split step recomputes
+    in-lane (the row before every block, the column before the box, the plane
+    before every chunk) and on the box's own first row, plane and column: the
+    recomputed values carry the dipole, as the owner's do."""
+    from gprmax_amd.grid import ModelBuilder
+    def model():
+        nx, ny, nz = WIDE
+        bld = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, 14, pml_cells=4)
+        soil = bld.add_material(4.0, 0.005, 1, 0, 'soil')
+        bld.box(0, nx, 0, ny, 0, nz // 2, soil)
+        bld.add_waveform('ricker', 1, 20e9, 'w')
+        # box of the passes: cells 5..n-5; the split step's box starts one later
+        i0, j0, k0 = 5, 5, 5
+        for j in range(j0, ny - 5):                      # every row: block starts fall on some
+            bld.add_magnetic_dipole('x', (i0 + 3, j, k0 + 2), 'w')
+            bld.add_magnetic_dipole('z', (i0 + 6, j, k0 + 7), 'w')
+        for i in range(i0, nx - 5):                      # every plane: chunk starts fall on some
+            bld.add_magnetic_dipole('y', (i, j0 + 4, k0 + 3), 'w')
+            bld.add_magnetic_dipole('z', (i, j0 + 9, k0 + 1), 'w')
+        for k in range(k0, nz - 5):                      # the column before the box and the box's first
+            bld.add_magnetic_dipole('x', (i0 + 2, j0, k), 'w')
+            bld.add_magnetic_dipole('y', (i0 + 4, j0 + 1, k0), 'w')
+        bld.add_magnetic_dipole('y', (i0 + 5, j0 + 6, k0), 'w')
+        bld.add_hertzian_dipole('z', (nx // 2, ny // 2, nz // 2), 'w')
+        for r in range(6):
+            bld.add_rx((i0 + 1 + r, j0 + 2 * r, k0 + 1 + r))
+        return bld.finalise()
+    G, Gc = model(), model()
+    with _hip()(G) as s:
+        s.set_step(3)
+        s.set_launch(xchunk, 0)
+        assert s.step_form() == 3
+        box = s.interior_box()
+        assert box[0] == 5 and box[2] == 5 and box[4] == 5
+        s.run(0, G.iterations)
+        F = s.download_fields()
+        rxb = s.download_rx()
+    osolver.solve_cpu(1, 1, Gc, nthreads=8)
+    for n, c in enumerate(COMPS):
+        assert np.array_equal(F[n], getattr(Gc, c)), c
+    for r, rx in enumerate(Gc.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
