@@ -438,25 +438,25 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
             float ex = cur.ex, ey = cur.ey, ez = cur.ez;
             const float hzj = cur.hzj, hxj = cur.hxj, hyk = cur.hyk, hxk = cur.hxk;
             const uint32_t m0 = cur.m0, m1 = cur.m1, m2 = cur.m2;
+            const bool rx_ = !BND || (i < nx && j >= 1 && j < ny && k >= 1 && k < nz);
+            const bool ry_ = !BND || (i >= 1 && i < nx && j < ny && k >= 1 && k < nz);
+            const bool rz_ = !BND || (i >= 1 && i < nx && j >= 1 && j < ny && k < nz);
+            // Auxiliary values change only on edges of materials with poles,
+            // so only those edges read and write them.
+            const bool dx_ = P > 0 && rx_ && lflag[m0];
+            const bool dy_ = P > 0 && ry_ && lflag[m1];
+            const bool dz_ = P > 0 && rz_ && lflag[m2];
             float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
             float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
             float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
             if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-                t_load<P, CPLX>(a.T, 0, N, po, to, txr, txi);
-                t_load<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
-                t_load<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
+                if (dx_) t_load<P, CPLX>(a.T, 0, N, po, to, txr, txi);
+                if (dy_) t_load<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
+                if (dz_) t_load<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
             }
             // Differences of H used by the E update and the E-PML (backward).
             const float dHz_y = hz - hzj, dHy_z = hy - hyk, dHx_z = hx - hxk;
             const float dHz_x = hz - hz_m, dHy_x = hy - hy_m, dHx_y = hx - hxj;
-
-            const bool rx_ = !BND || (i < nx && j >= 1 && j < ny && k >= 1 && k < nz);
-            const bool ry_ = !BND || (i >= 1 && i < nx && j < ny && k >= 1 && k < nz);
-            const bool rz_ = !BND || (i >= 1 && i < nx && j >= 1 && j < ny && k < nz);
-            // Auxiliary values change only on edges of materials with poles.
-            const bool dx_ = P > 0 && rx_ && lflag[m0];
-            const bool dy_ = P > 0 && ry_ && lflag[m1];
-            const bool dz_ = P > 0 && rz_ && lflag[m2];
 
             if (flags & F_MAIN) {
                 if (rx_) {
