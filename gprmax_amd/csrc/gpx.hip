@@ -31,6 +31,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 #include "gpx.h"
@@ -52,7 +53,7 @@ struct StepArgs {
     int nR;
     const float *disp;   // [nmat][3P] (real) or [nmat][3P][2] (complex)
     const uint8_t *dispflag;
-    float *T[3];         // [P][ntot] or [P][ntot][2]
+    TArgs T;             // compacted auxiliary values
     int nmat;
     int form;            // 0 HORIPML, 1 MRIPML
     int nslab;
@@ -449,10 +450,11 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
             float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
             float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
             float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
+            uint32_t sx = 0, sy = 0, sz = 0;
             if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-                if (dx_) t_load<P, CPLX>(a.T, 0, N, po, to, txr, txi);
-                if (dy_) t_load<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
-                if (dz_) t_load<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
+                if (dx_) sx = t_load<P, CPLX>(a.T, 0, po, to, txr, txi);
+                if (dy_) sy = t_load<P, CPLX>(a.T, 1, po, to, tyr, tyi);
+                if (dz_) sz = t_load<P, CPLX>(a.T, 2, po, to, tzr, tzi);
             }
             // Differences of H used by the E update and the E-PML (backward).
             const float dHz_y = hz - hzj, dHy_z = hy - hyk, dHx_z = hx - hxk;
@@ -548,9 +550,9 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
                 if (dz_) disp_b<P, CPLX>(ldisp, m2, tzr, tzi, ez);
             }
             if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-                if (dx_) t_store<P, CPLX>(a.T, 0, N, po, to, txr, txi);
-                if (dy_) t_store<P, CPLX>(a.T, 1, N, po, to, tyr, tyi);
-                if (dz_) t_store<P, CPLX>(a.T, 2, N, po, to, tzr, tzi);
+                if (dx_) t_store<P, CPLX>(a.T, 0, sx, txr, txi);
+                if (dy_) t_store<P, CPLX>(a.T, 1, sy, tyr, tyi);
+                if (dz_) t_store<P, CPLX>(a.T, 2, sz, tzr, tzi);
             }
             if (flags & (F_MAIN | F_PML_ALL | F_SRC)) {
                 bst(wEx, to, so, ex);
@@ -732,7 +734,10 @@ struct gpx_ctx {
     int disp_complex = 0;
     float *disp = nullptr;    // real [nmat][3P] or complex [nmat][3P][2]
     uint8_t *dispflag = nullptr;
-    float *T[3] = {nullptr, nullptr, nullptr};
+    float *T[3] = {nullptr, nullptr, nullptr};  // compacted: [P][tn[c]] (x2 complex)
+    uint32_t *tslot[3] = {nullptr, nullptr, nullptr};  // [ntot]: place of an edge with poles
+    long long tn[3] = {0, 0, 0};     // edges with poles per component
+    bool tslot_dirty = true;         // ID or dispersive tables changed since the slots were built
     bool have_id = false, have_coeffs = false, have_disp = false;
     std::vector<Slab> slabs;
     SrcSet src[3];
@@ -886,6 +891,7 @@ extern "C" int gpx_destroy(gpx_ctx *ctx)
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->Fmem) if (p) hipFree(p);
     for (auto &p : ctx->T) if (p) hipFree(p);
+    for (auto &p : ctx->tslot) if (p) hipFree(p);
     if (ctx->ID) hipFree(ctx->ID);
     if (ctx->cE) hipFree(ctx->cE);
     if (ctx->cH) hipFree(ctx->cH);
@@ -935,6 +941,111 @@ static int copy_out(gpx_ctx *ctx, void *dst, const void *src, size_t bytes)
     return 0;
 }
 
+// ---------------------------------------------------------------- auxiliary-value slots
+// Edges of one component whose material has poles, per block of 1024 cells.
+template <class IDT>
+__global__ void k_tslot_count(const IDT *ID, const uint8_t *flag, long long ntot, uint32_t *bsum)
+{
+    __shared__ int tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * 1024;
+    int cnt = 0;
+    for (int q = threadIdx.x; q < 1024; q += blockDim.x) {
+        const long long e = base + q;
+        if (e < ntot && flag[ID[e]]) ++cnt;
+    }
+    atomicAdd(&tot, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (uint32_t)tot;
+}
+
+// The slot of every such edge: the block's offset plus the edges before it
+// in the block (wave ballots, then the waves before it).  1024 threads.
+template <class IDT>
+__global__ void __launch_bounds__(1024) k_tslot_fill(const IDT *ID, const uint8_t *flag, long long ntot,
+                                                     const uint32_t *boff, uint32_t *slot)
+{
+    __shared__ int wsum[16];
+    const long long e = (long long)blockIdx.x * 1024 + threadIdx.x;
+    const bool f = e < ntot && flag[ID[e]];
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < w; ++q) off += wsum[q];
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (e < ntot) slot[e] = f ? boff[blockIdx.x] + (uint32_t)(off + before) : 0xffffffffu;
+}
+
+static int ensure_tslots(gpx_ctx *ctx);
+
+static void fill_targs(const gpx_ctx *ctx, TArgs &t)
+{
+    for (int c = 0; c < 3; ++c) {
+        t.T[c] = ctx->T[c];
+        t.slot[c] = ctx->tslot[c];
+        t.n[c] = ctx->tn[c];
+    }
+}
+
+// Builds the slots and the compacted auxiliary arrays (zeroed) once the
+// indices and the dispersive tables are on the device.  A new upload of
+// either starts the auxiliary values again from zero.
+static int ensure_tslots(gpx_ctx *ctx)
+{
+    const int P = ctx->desc.maxpoles;
+    if (P == 0 || !ctx->tslot_dirty || !ctx->have_id || !ctx->have_disp) return 0;
+    const long long N = ctx->ntot;
+    if (N >= 0xffffffffLL) return fail(ctx, GPX_ELIMIT, "dispersive models need fewer than 2^32 points");
+    hipSetDevice(ctx->desc.device);
+    const int nb = (int)((N + 1023) / 1024);
+    uint32_t *boff = nullptr;
+    GPX_TRY(dalloc(ctx, (void **)&boff, (size_t)nb * 4, false));
+    std::vector<uint32_t> h(nb);
+    int rc = 0;
+    for (int c = 0; c < 3 && !rc; ++c) {
+        if (!ctx->tslot[c]) {
+            hipError_t e = dalloc(ctx, (void **)&ctx->tslot[c], (size_t)N * 4, false);
+            if (e != hipSuccess) { rc = fail_hip(ctx, e, "slot array"); break; }
+        }
+        if (ctx->id_bytes == 2)
+            hipLaunchKernelGGL(k_tslot_count<uint16_t>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff);
+        else
+            hipLaunchKernelGGL(k_tslot_count<uint32_t>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const uint32_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff);
+        if ((rc = copy_out(ctx, h.data(), boff, (size_t)nb * 4))) break;
+        unsigned long long acc = 0;
+        for (int b = 0; b < nb; ++b) {
+            const uint32_t v = h[b];
+            h[b] = (uint32_t)acc;
+            acc += v;
+        }
+        if ((rc = copy_in(ctx, boff, h.data(), (size_t)nb * 4))) break;
+        if (ctx->id_bytes == 2)
+            hipLaunchKernelGGL(k_tslot_fill<uint16_t>, dim3(nb), dim3(1024), 0, ctx->stream,
+                               (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff, ctx->tslot[c]);
+        else
+            hipLaunchKernelGGL(k_tslot_fill<uint32_t>, dim3(nb), dim3(1024), 0, ctx->stream,
+                               (const uint32_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff, ctx->tslot[c]);
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) { rc = fail_hip(ctx, e, "slot build"); break; }
+        ctx->tn[c] = (long long)acc;
+        if (ctx->T[c]) { hipFree(ctx->T[c]); ctx->T[c] = nullptr; }
+        const size_t tb = (size_t)P * (size_t)(acc > 0 ? acc : 1) * 4 * (ctx->disp_complex ? 2 : 1);
+        e = dalloc(ctx, (void **)&ctx->T[c], tb);
+        if (e != hipSuccess) { rc = fail_hip(ctx, e, "auxiliary values"); break; }
+    }
+    hipFree(boff);
+    if (!rc) {
+        GPX_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->tslot_dirty = false;
+    }
+    return rc;
+}
+
 extern "C" int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID)
 {
     if (!ctx || !ID) return fail(ctx, GPX_EARG, "gpx_upload_id: null argument");
@@ -954,7 +1065,11 @@ extern "C" int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID)
         std::vector<uint16_t> c(ID, ID + n);
         rc = copy_in(ctx, ctx->ID, c.data(), n * 2);
     }
-    if (!rc) ctx->have_id = true;
+    if (!rc) {
+        ctx->have_id = true;
+        ctx->tslot_dirty = true;
+        rc = ensure_tslots(ctx);
+    }
     return rc;
 }
 
@@ -995,13 +1110,15 @@ extern "C" int gpx_upload_dispersive(gpx_ctx *ctx, const float *dc)
     if (!ctx->disp) {
         GPX_TRY(dalloc(ctx, (void **)&ctx->disp, host.size() * 4));
         GPX_TRY(dalloc(ctx, (void **)&ctx->dispflag, nm));
-        const size_t tb = (size_t)P * ctx->ntot * 4 * (cplx ? 2 : 1);
-        for (int c = 0; c < 3; ++c) GPX_TRY(dalloc(ctx, (void **)&ctx->T[c], tb));
     }
     ctx->disp_complex = cplx;
     int rc = copy_in(ctx, ctx->disp, host.data(), host.size() * 4);
     if (!rc) rc = copy_in(ctx, ctx->dispflag, flag.data(), nm);
-    if (!rc) ctx->have_disp = true;
+    if (!rc) {
+        ctx->have_disp = true;
+        ctx->tslot_dirty = true;
+        rc = ensure_tslots(ctx);
+    }
     return rc;
 }
 
@@ -1467,7 +1584,7 @@ static void base_args(const gpx_ctx *ctx, int field, StepArgs &a, int &block, si
         if (P > 0) {
             a.disp = ctx->disp;
             a.dispflag = ctx->dispflag;
-            for (int c = 0; c < 3; ++c) a.T[c] = ctx->T[c];
+            fill_targs(ctx, a.T);
             lds += (size_t)g.nmat * 3 * P * 4 * (ctx->disp_complex ? 2 : 1) + g.nmat;
         }
     } else {
@@ -1683,7 +1800,7 @@ static void fill_step_args(const gpx_ctx *ctx, int tc, int it, FusedArgs &a)
     if (g.maxpoles > 0) {
         a.disp = ctx->disp;
         a.dispflag = ctx->dispflag;
-        for (int c = 0; c < 3; ++c) a.T[c] = ctx->T[c];
+        fill_targs(ctx, a.T);
     }
     a.nmat = g.nmat;
     a.form = g.pml_formulation;
@@ -1913,6 +2030,7 @@ static int check_ready(gpx_ctx *ctx)
         return fail(ctx, GPX_ESTATE, "ID and coefficient tables must be uploaded before stepping");
     if (ctx->desc.maxpoles > 0 && !ctx->have_disp)
         return fail(ctx, GPX_ESTATE, "dispersive coefficients must be uploaded before stepping");
+    if (int rc = ensure_tslots(ctx)) return rc;
     return check_tls(ctx);
 }
 
@@ -2214,24 +2332,31 @@ extern "C" int gpx_upload_pml_phi(gpx_ctx *ctx, int slab, int which, const float
     return copy_in(ctx, p, in, b);
 }
 
+// T in the reference's layout, complex64 [3][P][ntot]; edges without poles
+// hold no auxiliary values here and read back as zero.
 extern "C" int gpx_download_dispersive_T(gpx_ctx *ctx, float *out)
 {
     if (!ctx || !out) return fail(ctx, GPX_EARG, "gpx_download_dispersive_T: null argument");
     const int P = ctx->desc.maxpoles;
-    if (!P || !ctx->T[0]) return fail(ctx, GPX_ESTATE, "gpx_download_dispersive_T: no dispersive state");
-    const size_t n = (size_t)P * ctx->ntot;
-    if (ctx->disp_complex) {
-        for (int c = 0; c < 3; ++c)
-            if (int rc = copy_out(ctx, out + (size_t)c * n * 2, ctx->T[c], n * 8)) return rc;
-        return 0;
-    }
-    std::vector<float> tmp(n);
+    if (!P) return fail(ctx, GPX_ESTATE, "gpx_download_dispersive_T: no dispersive state");
+    if (int rc = ensure_tslots(ctx)) return rc;
+    if (ctx->tslot_dirty || !ctx->T[0]) return fail(ctx, GPX_ESTATE, "gpx_download_dispersive_T: no dispersive state");
+    const size_t N = (size_t)ctx->ntot, w = ctx->disp_complex ? 2 : 1;
+    std::fill(out, out + (size_t)3 * P * N * 2, 0.0f);
+    std::vector<uint32_t> slot(N);
     for (int c = 0; c < 3; ++c) {
-        if (int rc = copy_out(ctx, tmp.data(), ctx->T[c], n * 4)) return rc;
-        float *o = out + (size_t)c * n * 2;
-        for (size_t q = 0; q < n; ++q) {
-            o[2 * q] = tmp[q];
-            o[2 * q + 1] = 0.0f;
+        const size_t n = (size_t)ctx->tn[c];
+        std::vector<float> t((size_t)P * (n ? n : 1) * w);
+        if (int rc = copy_out(ctx, slot.data(), ctx->tslot[c], N * 4)) return rc;
+        if (int rc = copy_out(ctx, t.data(), ctx->T[c], t.size() * 4)) return rc;
+        for (size_t e = 0; e < N; ++e) {
+            const uint32_t s = slot[e];
+            if (s == 0xffffffffu) continue;
+            for (int p = 0; p < P; ++p) {
+                float *o = out + (((size_t)c * P + p) * N + e) * 2;
+                o[0] = t[((size_t)p * n + s) * w];
+                if (w == 2) o[1] = t[((size_t)p * n + s) * w + 1];
+            }
         }
     }
     return 0;
@@ -2241,22 +2366,28 @@ extern "C" int gpx_upload_dispersive_T(gpx_ctx *ctx, const float *in)
 {
     if (!ctx || !in) return fail(ctx, GPX_EARG, "gpx_upload_dispersive_T: null argument");
     const int P = ctx->desc.maxpoles;
-    if (!P || !ctx->T[0]) return fail(ctx, GPX_ESTATE, "gpx_upload_dispersive_T: no dispersive state");
-    const size_t n = (size_t)P * ctx->ntot;
-    if (ctx->disp_complex) {
-        for (int c = 0; c < 3; ++c)
-            if (int rc = copy_in(ctx, ctx->T[c], in + (size_t)c * n * 2, n * 8)) return rc;
-        return 0;
-    }
-    std::vector<float> tmp(n);
+    if (!P) return fail(ctx, GPX_ESTATE, "gpx_upload_dispersive_T: no dispersive state");
+    if (int rc = ensure_tslots(ctx)) return rc;
+    if (ctx->tslot_dirty || !ctx->T[0])
+        return fail(ctx, GPX_ESTATE, "gpx_upload_dispersive_T: upload the indices and dispersive tables first");
+    const size_t N = (size_t)ctx->ntot, w = ctx->disp_complex ? 2 : 1;
+    std::vector<uint32_t> slot(N);
     for (int c = 0; c < 3; ++c) {
-        const float *s = in + (size_t)c * n * 2;
-        for (size_t q = 0; q < n; ++q) {
-            if (s[2 * q + 1] != 0.0f)
-                return fail(ctx, GPX_EARG, "gpx_upload_dispersive_T: imaginary part in a real-pole model");
-            tmp[q] = s[2 * q];
+        const size_t n = (size_t)ctx->tn[c];
+        std::vector<float> t((size_t)P * (n ? n : 1) * w, 0.0f);
+        if (int rc = copy_out(ctx, slot.data(), ctx->tslot[c], N * 4)) return rc;
+        for (size_t e = 0; e < N; ++e) {
+            const uint32_t s = slot[e];
+            if (s == 0xffffffffu) continue;
+            for (int p = 0; p < P; ++p) {
+                const float *v = in + (((size_t)c * P + p) * N + e) * 2;
+                if (w == 1 && v[1] != 0.0f)
+                    return fail(ctx, GPX_EARG, "gpx_upload_dispersive_T: imaginary part in a real-pole model");
+                t[((size_t)p * n + s) * w] = v[0];
+                if (w == 2) t[((size_t)p * n + s) * w + 1] = v[1];
+            }
         }
-        if (int rc = copy_in(ctx, ctx->T[c], tmp.data(), n * 4)) return rc;
+        if (int rc = copy_in(ctx, ctx->T[c], t.data(), t.size() * 4)) return rc;
     }
     return 0;
 }
@@ -2756,8 +2887,10 @@ static int zero_state(gpx_ctx *ctx)
             GPX_TRY(hipMemsetAsync(S.phin[q], 0, (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4, s));
     }
     if (c->desc.maxpoles > 0)
-        for (float *T : c->T)
-            if (T) GPX_TRY(hipMemsetAsync(T, 0, (size_t)c->desc.maxpoles * c->ntot * 4 * (c->disp_complex ? 2 : 1), s));
+        for (int q = 0; q < 3; ++q)
+            if (c->T[q])
+                GPX_TRY(hipMemsetAsync(c->T[q], 0, (size_t)c->desc.maxpoles * (size_t)(c->tn[q] > 0 ? c->tn[q] : 1) * 4 *
+                                                       (c->disp_complex ? 2 : 1), s));
     if (c->rxbuf) GPX_TRY(hipMemsetAsync(c->rxbuf, 0, (size_t)6 * c->desc.iterations * c->nrx * 4, s));
     GPX_TRY(hipStreamSynchronize(s));
     return 0;

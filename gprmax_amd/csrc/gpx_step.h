@@ -300,33 +300,46 @@ __device__ __forceinline__ void disp_b(const float *D, uint32_t m, float *tr, fl
     }
 }
 
+// Dispersive auxiliary values, compacted: component c keeps its P poles
+// only for the edges whose material has poles, in cell order,
+// T[c][p * n[c] + slot], with slot[c][cell] the edge's place in that order
+// (read only on such edges).
+struct TArgs {
+    float *T[3];
+    const uint32_t *slot[3];
+    long long n[3];
+};
+
+// Loads the P values of edge (c, po + to / 4) and returns its slot.
 template <int P, bool CPLX>
-__device__ __forceinline__ void t_load(float *const *T, int c, size_t N, size_t po, uint32_t to,
-                                       float *tr, float *ti)
+__device__ __forceinline__ uint32_t t_load(const TArgs &t, int c, size_t po, uint32_t to, float *tr, float *ti)
 {
+    const uint32_t s = ld(t.slot[c] + po, to);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
+        const size_t e = (size_t)p * (size_t)t.n[c] + s;
         if (CPLX) {
-            const float2 v = ld(reinterpret_cast<const float2 *>(T[c]) + p * N + po, 2 * to);
+            const float2 v = ld(reinterpret_cast<const float2 *>(t.T[c]) + e, 0u);
             tr[p] = v.x;
             ti[p] = v.y;
         } else {
-            tr[p] = ld((const float *)T[c] + p * N + po, to);
+            tr[p] = ld((const float *)t.T[c] + e, 0u);
             ti[p] = 0.f;
         }
     }
+    return s;
 }
 
 template <int P, bool CPLX>
-__device__ __forceinline__ void t_store(float *const *T, int c, size_t N, size_t po, uint32_t to,
-                                        const float *tr, const float *ti)
+__device__ __forceinline__ void t_store(const TArgs &t, int c, uint32_t s, const float *tr, const float *ti)
 {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
+        const size_t e = (size_t)p * (size_t)t.n[c] + s;
         if (CPLX)
-            st(reinterpret_cast<float2 *>(T[c]) + p * N + po, 2 * to, make_float2(tr[p], ti[p]));
+            st(reinterpret_cast<float2 *>(t.T[c]) + e, 0u, make_float2(tr[p], ti[p]));
         else
-            st(T[c] + p * N + po, to, tr[p]);
+            st(t.T[c] + e, 0u, tr[p]);
     }
 }
 
@@ -368,7 +381,7 @@ struct FusedArgs {
     int nR;
     const float *disp;
     const uint8_t *dispflag;
-    float *T[3];
+    TArgs T;
     int nmat, form, nslab;
     SlabArgs slabE[MAX_SLABS], slabH[MAX_SLABS];  // slabH: Phi read current, written next
     int it, iterations;
@@ -646,6 +659,7 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
                     float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
                     float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
                     float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
+                    uint32_t sx = 0, sy = 0, sz = 0;
                     const bool rx_ = i < nx && j >= 1 && j < ny && k >= 1 && k < nz;
                     const bool ry_ = i >= 1 && i < nx && j < ny && k >= 1 && k < nz;
                     const bool rz_ = i >= 1 && i < nx && j >= 1 && j < ny && k < nz;
@@ -653,9 +667,9 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
                     const bool dy_ = P > 0 && ry_ && lflag[m1[u]];
                     const bool dz_ = P > 0 && rz_ && lflag[m2[u]];
                     if (P > 0) {
-                        if (dx_) t_load<P, CPLX>(a.T, 0, N, po, to[u], txr, txi);
-                        if (dy_) t_load<P, CPLX>(a.T, 1, N, po, to[u], tyr, tyi);
-                        if (dz_) t_load<P, CPLX>(a.T, 2, N, po, to[u], tzr, tzi);
+                        if (dx_) sx = t_load<P, CPLX>(a.T, 0, po, to[u], txr, txi);
+                        if (dy_) sy = t_load<P, CPLX>(a.T, 1, po, to[u], tyr, tyi);
+                        if (dz_) sz = t_load<P, CPLX>(a.T, 2, po, to[u], tzr, tzi);
                     }
                     if (rx_) {
                         const float4 q = cE[m0[u]];
@@ -736,9 +750,9 @@ __device__ __forceinline__ void step_item(const FusedArgs &a, const StepLds &S, 
                         }
                     }
                     if (P > 0) {
-                        if (dx_) { disp_b<P, CPLX>(ldisp, m0[u], txr, txi, nex); t_store<P, CPLX>(a.T, 0, N, po, to[u], txr, txi); }
-                        if (dy_) { disp_b<P, CPLX>(ldisp, m1[u], tyr, tyi, ney); t_store<P, CPLX>(a.T, 1, N, po, to[u], tyr, tyi); }
-                        if (dz_) { disp_b<P, CPLX>(ldisp, m2[u], tzr, tzi, nez); t_store<P, CPLX>(a.T, 2, N, po, to[u], tzr, tzi); }
+                        if (dx_) { disp_b<P, CPLX>(ldisp, m0[u], txr, txi, nex); t_store<P, CPLX>(a.T, 0, sx, txr, txi); }
+                        if (dy_) { disp_b<P, CPLX>(ldisp, m1[u], tyr, tyi, ney); t_store<P, CPLX>(a.T, 1, sy, tyr, tyi); }
+                        if (dz_) { disp_b<P, CPLX>(ldisp, m2[u], tzr, tzi, nez); t_store<P, CPLX>(a.T, 2, sz, tzr, tzi); }
                     }
                     bst(wF, to[u], so, nex);
                     bst(wF, to[u], so + cs, ney);

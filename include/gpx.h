@@ -73,7 +73,11 @@ int gpx_destroy(gpx_ctx *ctx);
 const char *gpx_last_error(const gpx_ctx *ctx);
 
 /* Model data.  Replace the host->device copies of grid.py:251-272,
- * model_build_run.py:512-519 and 540-549. */
+ * model_build_run.py:512-519 and 540-549.  In a dispersive model the
+ * auxiliary values (T, grid.py:187-190) are kept only for edges whose
+ * material has poles; once both the indices and the dispersive tables are
+ * on the device those edges are listed and their values set to zero, and a
+ * later upload of either lists them again from zero. */
 int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID);
 int gpx_upload_coeffs(gpx_ctx *ctx, const float *coeffsE, const float *coeffsH);
 int gpx_upload_dispersive(gpx_ctx *ctx, const float *disp_complex);
@@ -193,6 +197,8 @@ int gpx_download_fields(gpx_ctx *ctx, float *out6);
 int gpx_upload_fields(gpx_ctx *ctx, const float *in6);
 int gpx_download_pml_phi(gpx_ctx *ctx, int slab, int which, float *out);
 int gpx_upload_pml_phi(gpx_ctx *ctx, int slab, int which, const float *in);
+/* T in the reference's layout, complex64 [3][P][nx+1][ny+1][nz+1]; edges
+ * without poles read back as zero and their uploaded values are ignored. */
 int gpx_download_dispersive_T(gpx_ctx *ctx, float *out_complex);
 int gpx_upload_dispersive_T(gpx_ctx *ctx, const float *in_complex);
 int gpx_download_snapshot(gpx_ctx *ctx, int index, float *out6);
