@@ -265,7 +265,9 @@ def main():
     ap.add_argument('--passes', action='store_true', help='the H and E passes (step form 0)')
     ap.add_argument('--split', action='store_true',
                     help='the split step (form 3): one kernel doing H and E for the interior, boundary passes '
-                         'around it; by default the library times it against the passes and keeps the faster')
+                         'around it; by default the library times it, form 5 and the passes and keeps the fastest')
+    ap.add_argument('--split-beside', action='store_true',
+                    help='the split step with the H boundary pass beside the interior kernel (form 5)')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
@@ -295,7 +297,8 @@ def main():
     info = device_info(0)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
-        s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (3 if args.split else 4))),
+        s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (
+                   3 if args.split else (5 if args.split_beside else 4)))),
                    args.tile)
         fused = s.step_kind()
         s.run(0, W)
@@ -314,7 +317,7 @@ def main():
     mcells = cells * K / (ms * 1e-3) / 1e6
     if fused:
         kernels = {'step': (algorithmic_bytes_per_step(G), ms4[0])}
-    elif form == 3:
+    elif form in (3, 5):
         # the fused box is the interior box less its first plane, row and column
         box3 = (box[0] + 1, box[1], box[2] + 1, box[3], box[4] + 1, box[5])
         bh, be = kernel_bytes(G, 1, box3), kernel_bytes(G, 0, box3)
@@ -352,9 +355,11 @@ def main():
                      'kernels_bytes': {k: v[0] for k, v in kernels.items()},
                      'step_achieved': round(step_gbs, 1),
                      'step_form': ('fused, {} cells per block'.format(fused) if fused else
-                                   {3: 'split', 2: 'two-pass side by side'}.get(form, 'two-pass')) +
-                                  (' (timed against the other and kept)' if not (args.fused or args.overlap or
-                                                                                   args.passes or args.split)
+                                   {3: 'split', 5: 'split, H boundary beside the interior kernel',
+                                    2: 'two-pass side by side'}.get(form, 'two-pass')) +
+                                  (' (timed against the others and kept)' if not (args.fused or args.overlap or
+                                                                                   args.passes or args.split or
+                                                                                   args.split_beside)
                                    else ''),
                      'interior_cells': interior},
         'device_bytes': mem,

@@ -759,9 +759,10 @@ struct gpx_ctx {
     int nframe3 = 0;                    // the plane offsets outside its rectangle
     int xchunk = 0, block = 0;
     int fused = 4;                   // step form (gpx_set_step): 0 passes, 1 fused step, 2 passes side
-                                     // by side, 3 split step, 4 (default) the faster of 3 and 0
-    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step
-    hipStream_t stream2 = nullptr;   // second stream of the side-by-side form
+                                     // by side, 3 split step, 4 (default) the fastest of 5, 3 and 0,
+                                     // 5 split step with the H boundary pass beside the interior kernel
+    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step, 2 form 5
+    hipStream_t stream2 = nullptr;   // second stream of forms 2 and 5
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int tile_cells = 0;              // cells per fused block (0 = auto)
     int cus = 0;                     // compute units of the device
@@ -1337,7 +1338,7 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
-    if (!ctx || fused < 0 || fused > 4 ||
+    if (!ctx || fused < 0 || fused > 5 ||
         !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
@@ -1687,14 +1688,20 @@ static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it, 
 // The same pass with the interior and boundary kernels running side by side:
 // the boundary kernel on the second stream, each planned for its share of
 // the resident blocks (by cell count, boundary cells weighted 2x).
+// The second stream and its fork and join events, made on first use.
+static hipError_t ensure_stream2(gpx_ctx *ctx)
+{
+    hipError_t e;
+    if (!ctx->stream2 && (e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking)) != hipSuccess) return e;
+    if (!ctx->ev_fork && (e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!ctx->ev_join && (e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
 static hipError_t launch_pass_overlap(gpx_ctx *ctx, int field, int flags, int it)
 {
     hipError_t e;
-    if (!ctx->stream2) {
-        if ((e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking)) != hipSuccess) return e;
-        if ((e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming)) != hipSuccess) return e;
-        if ((e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
-    }
+    if ((e = ensure_stream2(ctx)) != hipSuccess) return e;
     const double ni = (double)(ctx->box[1] - ctx->box[0]) * (ctx->box[3] - ctx->box[2]) * (ctx->box[5] - ctx->box[4]);
     const double nb = (double)ctx->ntot - (ni > 0 ? ni : 0);
     const double share = ni > 0 ? ni / (ni + 2.0 * nb) : 0.0;
@@ -1907,8 +1914,14 @@ static bool split_eligible(const gpx_ctx *ctx)
 
 static bool split_ok(const gpx_ctx *ctx)
 {
-    if (ctx->fused == 3) return split_eligible(ctx);
+    if (ctx->fused == 3 || ctx->fused == 5) return split_eligible(ctx);
     return ctx->fused == 4 && ctx->split_choice != 0 && split_eligible(ctx);
+}
+
+// Form 5, or form 4 after it chose form 5 (with split_ok).
+static bool split_beside(const gpx_ctx *ctx)
+{
+    return ctx->fused == 5 || (ctx->fused == 4 && ctx->split_choice == 2);
 }
 
 static hipError_t launch_interior(const gpx_ctx *ctx, int it)
@@ -2065,10 +2078,11 @@ struct EventSet {
 // and [3]; or H interior, H boundary, E interior and E boundary.
 static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4);
 
-// Form 4 picks between the split step and the passes on the first run long
-// enough to time both: after two warm-up iterations, TUNE_ITS iterations of
-// each, timed with events; the faster serves the rest of this run and later
-// runs.  The forms give the same bits, so the choice does not change results.
+// Form 4 picks among form 5, the split step and the passes on the first run
+// long enough to time them: after two warm-up iterations, TUNE_ITS
+// iterations of each, timed with events; the fastest serves the rest of this
+// run and later runs.  The forms give the same bits, so the choice does not
+// change results.
 static const int TUNE_ITS = 6;
 
 static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
@@ -2076,27 +2090,36 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
     if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || !split_eligible(ctx) ||
         n < 2 + 4 * TUNE_ITS)
         return run_steps_form(ctx, it0, n, ms4);
+    // choices in the order timed: form 5, the split step, the passes
+    static const int order[3] = {2, 1, 0};
     EventSet ev;
-    ev.e.assign(3, nullptr);
+    ev.e.assign(4, nullptr);
     for (auto &x : ev.e) GPX_TRY(hipEventCreate(&x));
-    ctx->split_choice = 1;
+    ctx->split_choice = order[0];
     int rc = run_steps_form(ctx, it0, 2, nullptr);
-    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[0], ctx->stream));
-    if (!rc) rc = run_steps_form(ctx, it0 + 2, TUNE_ITS, nullptr);
-    ctx->split_choice = 0;
-    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[1], ctx->stream));
-    if (!rc) rc = run_steps_form(ctx, it0 + 2 + TUNE_ITS, TUNE_ITS, nullptr);
-    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[2], ctx->stream));
-    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[2]));
-    float t_split = 0, t_pass = 0;
-    if (!rc) rc = check_hip(ctx, hipEventElapsedTime(&t_split, ev.e[0], ev.e[1]));
-    if (!rc) rc = check_hip(ctx, hipEventElapsedTime(&t_pass, ev.e[1], ev.e[2]));
+    for (int f = 0; f < 3 && !rc; ++f) {
+        ctx->split_choice = order[f];
+        rc = check_hip(ctx, hipEventRecord(ev.e[f], ctx->stream));
+        if (!rc) rc = run_steps_form(ctx, it0 + 2 + f * TUNE_ITS, TUNE_ITS, nullptr);
+    }
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[3], ctx->stream));
+    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[3]));
+    int best = 0;
+    float t_best = 0;
+    for (int f = 0; f < 3 && !rc; ++f) {
+        float t = 0;
+        rc = check_hip(ctx, hipEventElapsedTime(&t, ev.e[f], ev.e[f + 1]));
+        if (!rc && (f == 0 || t < t_best)) {
+            best = f;
+            t_best = t;
+        }
+    }
     if (rc) {
         ctx->split_choice = -1;
         return rc;
     }
-    ctx->split_choice = t_split <= t_pass ? 1 : 0;
-    const int done = 2 + 2 * TUNE_ITS;
+    ctx->split_choice = order[best];
+    const int done = 2 + 3 * TUNE_ITS;
     return run_steps_form(ctx, it0 + done, n - done, nullptr);
 }
 
@@ -2123,6 +2146,23 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
             if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
             GPX_TRY(launch_step(ctx, tc, it));
             if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+            continue;
+        }
+        if (split && !ev && split_beside(ctx)) {
+            // Form 5: the H boundary pass reads only the current state and
+            // writes only boundary cells, Phi and receiver samples of boundary
+            // cells, so it runs beside the interior kernel on the second
+            // stream; the E boundary pass waits for both.  (Runs with timing
+            // events take the order of form 3.)
+            GPX_TRY(ensure_stream2(ctx));
+            GPX_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+            GPX_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, ctx->stream2, 1.0, true));
+            GPX_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+            GPX_TRY(launch_interior(ctx, it));
+            GPX_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, 1.0, true));
+            for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
             continue;
         }
         if (split) {
@@ -2182,7 +2222,7 @@ extern "C" int gpx_step_form(const gpx_ctx *ctx)
 {
     if (!ctx) return GPX_EARG;
     if (step_tile(ctx)) return 1;
-    if (split_ok(ctx)) return 3;
+    if (split_ok(ctx)) return split_beside(ctx) ? 5 : 3;
     return ctx->fused == 2 ? 2 : 0;
 }
 

@@ -195,8 +195,9 @@ class HipSolver:
     def set_step(self, fused=4, tile_cells=0):
         """Step form of gpx_run: 0 the H and E passes, 1 the fused step, 2 the
         passes with interior and boundary kernels side by side, 3 the split
-        step, 4 (the library's default) the faster of 3 and 0, timed on the
-        first run of at least 26 iterations.  All give the same bits."""
+        step, 5 the split step with the H boundary pass beside the interior
+        kernel, 4 (the library's default) the fastest of 5, 3 and 0, timed on
+        the first run of at least 26 iterations.  All give the same bits."""
         self._check(self.lib.gpx_set_step(self.ctx, int(fused), int(tile_cells)))
 
     def step_kind(self):
@@ -204,11 +205,13 @@ class HipSolver:
         return int(self.lib.gpx_step_kind(self.ctx))
 
     def step_form(self):
-        """The form gpx_run uses: 0 passes, 1 fused, 2 side by side, 3 split."""
+        """The form gpx_run uses: 0 passes, 1 fused, 2 side by side, 3 split,
+        5 split with the H boundary pass beside the interior kernel."""
         return int(self.lib.gpx_step_form(self.ctx))
 
     def step_choice(self):
-        """The default form's choice: -1 not timed yet, 0 passes, 1 split."""
+        """The default form's choice: -1 not timed yet, 0 passes, 1 split,
+        2 split with the H boundary pass beside."""
         return int(self.lib.gpx_step_choice(self.ctx))
 
     def run(self, it0=0, n=None):

@@ -168,13 +168,16 @@ int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
  * interior and boundary kernels running side by side on two streams
  * (gpx_run_profiled always runs them one after the other); fused = 3: the
  * split step, one kernel doing H and E for the interior and the H and E
- * boundary passes around it, double-buffered; fused = 4 (default): the first
- * gpx_run of at least 26 iterations times the split step and the passes on
- * six iterations each and keeps the faster (gpx_step_choice).  tile_cells = cells per fused
- * block (256, 512 or 1024; 0 = auto).  All forms give identical results; the
+ * boundary passes around it, double-buffered; fused = 5: the split step with
+ * the H boundary pass running beside the interior kernel on a second stream
+ * (gpx_run_profiled runs them one after the other); fused = 4 (default): the
+ * first gpx_run of at least 26 iterations times form 5, the split step and
+ * the passes on six iterations each and keeps the fastest
+ * (gpx_step_choice).  tile_cells = cells per fused block (256, 512 or 1024;
+ * 0 = auto).  All forms give identical results; the
  * fused form needs a tile of at least twice nz+1 cells within the LDS budget,
  * the split step a model without dispersive poles, and the passes run when
- * the chosen form does not apply. */
+ * the chosen form does not apply (form 5 as the split step). */
 int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells);
 
 /* Cells per block of the fused step that gpx_run will use, or 0 when it will
@@ -182,11 +185,12 @@ int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells);
 int gpx_step_kind(const gpx_ctx *ctx);
 
 /* The form gpx_run will use: 0 passes, 1 fused step, 2 passes side by side,
- * 3 split step.  gpx_run_profiled reports the split step's interior kernel in
+ * 3 split step, 5 split step with the H boundary pass beside the interior
+ * kernel.  gpx_run_profiled reports the split step's interior kernel in
  * ms4[0] and its boundary passes in ms4[1] and ms4[3]. */
 int gpx_step_form(const gpx_ctx *ctx);
 
-/* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step. */
+/* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step, 2 form 5. */
 int gpx_step_choice(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);

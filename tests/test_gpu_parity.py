@@ -158,13 +158,15 @@ def _compare_runs(F, rxb, snaps, phis, T, Gc):
             assert np.array_equal(T[n], getattr(Gc, name)), name
 
 
-FORMS = dict(argvalues=[1, 0, 2, 3, 4], ids=['fused-step', 'two-pass', 'side-by-side', 'split', 'timed'])
+FORMS = dict(argvalues=[1, 0, 2, 3, 5, 4],
+             ids=['fused-step', 'two-pass', 'side-by-side', 'split', 'split-beside', 'timed'])
+CHOSEN_FORM = {0: 0, 1: 3, 2: 5}    # form 4's choice -> the form it runs
 
 
 def _expected_form(fused, G):
     # form 4 runs the split step until its first long run has timed it
-    if fused in (3, 4):
-        return 3 if G.maxpoles == 0 else 0
+    if fused in (3, 4, 5):
+        return (5 if fused == 5 else 3) if G.maxpoles == 0 else 0
     return fused
 
 
@@ -181,11 +183,11 @@ def test_fused_run_matches_oracle(case, fused):
         assert s.step_form() == _expected_form(fused, G)
         assert s.step_choice() == -1
         s.run(0, G.iterations)
-        # 40 iterations: form 4 timed both forms on iterations 2-13 and went on
-        # with the faster, switching forms in the middle of the run
+        # 40 iterations: form 4 timed three forms on iterations 2-19 and went
+        # on with the fastest, switching forms in the middle of the run
         if fused == 4 and not G.maxpoles:
-            assert s.step_choice() in (0, 1)
-            assert s.step_form() == (3 if s.step_choice() else 0)
+            assert s.step_choice() in (0, 1, 2)
+            assert s.step_form() == CHOSEN_FORM[s.step_choice()]
         else:
             assert s.step_choice() == -1
         F = s.download_fields()
@@ -197,24 +199,26 @@ def test_fused_run_matches_oracle(case, fused):
     _compare_runs(F, rxb, snaps, phis, T, Gc)
 
 
+@pytest.mark.parametrize('form', [3, 5], ids=['split', 'split-beside'])
 @pytest.mark.parametrize('xchunk', [0, 1, 2, 5])
 @pytest.mark.parametrize('case', ['horipml1', 'mripml2'])
-def test_split_step_from_random_state(case, xchunk):
-    """The split step from a random state (fields and Phi) on a grid with
-    several blocks per plane and short chunks: state and receiver samples
-    equal to the oracle stepped the same number of iterations."""
+def test_split_step_from_random_state(case, xchunk, form):
+    """The split step (its boundary passes after the interior kernel, or
+    the H boundary pass beside it) from a random state (fields and Phi) on a
+    grid with several blocks per plane and short chunks: state and receiver
+    samples equal to the oracle stepped the same number of iterations."""
     G = rich_model(seed=22, shape=WIDE, iterations=12, snapshots=False, **CASES[case])
     _random_state(G, 6)
     lib = oracle.load(np.float32)
     with _hip()(G) as s:
-        s.set_step(3)
+        s.set_step(form)
         s.set_launch(xchunk, 0)
-        assert s.step_form() == 3
+        assert s.step_form() == form
         _push_state(s, G)
         s.run(2, 6)
         for it in range(2, 8):
             osolver.step(lib, G, it, 8)
-        _assert_state_equal(s, G, 'split xchunk {}'.format(xchunk))
+        _assert_state_equal(s, G, 'split form {} xchunk {}'.format(form, xchunk))
         buf = s.download_rx()
     for r, rx in enumerate(G.rxs):
         for c, name in enumerate(COMPS):
@@ -241,7 +245,7 @@ def test_2d_run_matches_oracle(case, axis, fused):
     with _hip()(G) as s:
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
-        assert s.step_form() == (0 if fused in (3, 4) else fused)   # a 2-D grid has no fused box
+        assert s.step_form() == (0 if fused in (3, 4, 5) else fused)   # a 2-D grid has no fused box
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -458,8 +462,8 @@ def test_pass_batch_of_unlike_models():
 
 
 def test_default_form_times_once_and_resets():
-    """The default form times the split step and the passes on the first
-    run of at least 26 iterations only: short runs and profiled runs leave
+    """The default form times form 5, the split step and the passes on the
+    first run of at least 26 iterations only: short runs and profiled runs leave
     the choice open, gpx_set_step clears it, and the results stay the
     oracle's."""
     G = rich_model(seed=31, shape=WIDE, iterations=60, snapshots=False)
@@ -470,8 +474,8 @@ def test_default_form_times_once_and_resets():
         assert s.step_choice() == -1
         s.run_profiled(10, 5)              # profiled runs never time
         assert s.step_choice() == -1
-        s.run(15, 30)                      # times iterations 17-28, then keeps the faster
-        assert s.step_choice() in (0, 1)
+        s.run(15, 30)                      # times iterations 17-34, then keeps the fastest
+        assert s.step_choice() in (0, 1, 2)
         s.set_step(4)
         assert s.step_choice() == -1
         s.run(45, 15)
@@ -485,8 +489,9 @@ def test_default_form_times_once_and_resets():
             assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
 
 
+@pytest.mark.parametrize('form', [3, 5], ids=['split', 'split-beside'])
 @pytest.mark.parametrize('xchunk', [0, 3])
-def test_split_step_sources_on_recomputed_cells(xchunk):
+def test_split_step_sources_on_recomputed_cells(xchunk, form):
     """Magnetic dipoles on the cells whose new H the split step recomputes
     in-lane (the row before every block, the column before the box, the plane
     before every chunk) and on the box's own first row, plane and column: the
@@ -516,9 +521,9 @@ def test_split_step_sources_on_recomputed_cells(xchunk):
         return bld.finalise()
     G, Gc = model(), model()
     with _hip()(G) as s:
-        s.set_step(3)
+        s.set_step(form)
         s.set_launch(xchunk, 0)
-        assert s.step_form() == 3
+        assert s.step_form() == form
         box = s.interior_box()
         assert box[0] == 5 and box[2] == 5 and box[4] == 5
         s.run(0, G.iterations)
