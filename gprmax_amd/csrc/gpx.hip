@@ -759,9 +759,9 @@ struct gpx_ctx {
     int nframe3 = 0;                    // the plane offsets outside its rectangle
     int xchunk = 0, block = 0;
     int fused = 4;                   // step form (gpx_set_step): 0 passes, 1 fused step, 2 passes side
-                                     // by side, 3 split step, 4 (default) the fastest of 5, 3 and 0,
+                                     // by side, 3 split step, 4 (default) the fastest of 2, 5, 3 and 0,
                                      // 5 split step with the H boundary pass beside the interior kernel
-    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step, 2 form 5
+    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step, 2 form 5, 3 form 2
     hipStream_t stream2 = nullptr;   // second stream of forms 2 and 5
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int tile_cells = 0;              // cells per fused block (0 = auto)
@@ -1363,8 +1363,7 @@ extern "C" int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block)
 // for the boundary, unless fixed by gpx_set_launch): on the 300x300x200
 // half-space these beat both longer chunks and a persistent grid of resident
 // blocks by 11 %, since fewer planes are streamed at once.  Grids too small
-// to fill the GPU with such items take shorter chunks.  Side-by-side
-// launches keep a persistent grid of their share of the resident blocks.
+// to fill the GPU with such items take shorter chunks.
 static long long count_items(const gpx_ctx *ctx, bool bnd, int block, int xc, StepArgs *a, bool f3 = false)
 {
     const int *b = f3 ? ctx->box3 : ctx->box;
@@ -1601,7 +1600,7 @@ typedef void (*pass_kernel)(StepArgs);
 // `share` < 1 plans the kernel for that fraction of the resident blocks, so
 // that another kernel can run beside it.
 static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd, int field, int flags, int it,
-                                hipStream_t stream = nullptr, double share = 1.0, bool f3 = false)
+                                hipStream_t stream = nullptr, bool f3 = false)
 {
     StepArgs a;
     int block, grid;
@@ -1609,12 +1608,7 @@ static hipError_t launch_kernel(const gpx_ctx *ctx, pass_kernel kernel, bool bnd
     base_args(ctx, field, a, block, lds, f3);
     if (count_items(ctx, bnd, block, 1, nullptr, f3) <= 0) return hipSuccess;
     const int fill = resident_blocks(ctx, (const void *)kernel, block, lds);
-    int resident = 0;  // 0: one block per item
-    if (share < 1.0) {
-        resident = (int)(fill * share) / 8 * 8;
-        if (resident < 8) resident = 8;
-    }
-    plan(ctx, bnd, resident, block, a, grid, fill, f3);
+    plan(ctx, bnd, 0, block, a, grid, fill, f3);   // one block per item
     a.flags = flags;
     a.it = it;
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream ? stream : ctx->stream, a);
@@ -1685,9 +1679,6 @@ static hipError_t launch_pass(const gpx_ctx *ctx, int field, int flags, int it, 
     return e;
 }
 
-// The same pass with the interior and boundary kernels running side by side:
-// the boundary kernel on the second stream, each planned for its share of
-// the resident blocks (by cell count, boundary cells weighted 2x).
 // The second stream and its fork and join events, made on first use.
 static hipError_t ensure_stream2(gpx_ctx *ctx)
 {
@@ -1698,20 +1689,21 @@ static hipError_t ensure_stream2(gpx_ctx *ctx)
     return hipSuccess;
 }
 
+// The same pass with the interior and boundary kernels running side by side,
+// the boundary kernel on the second stream, each with one block per item.
+// Both read only the other field and write disjoint cells of their own.
 static hipError_t launch_pass_overlap(gpx_ctx *ctx, int field, int flags, int it)
 {
     hipError_t e;
     if ((e = ensure_stream2(ctx)) != hipSuccess) return e;
-    const double ni = (double)(ctx->box[1] - ctx->box[0]) * (ctx->box[3] - ctx->box[2]) * (ctx->box[5] - ctx->box[4]);
-    const double nb = (double)ctx->ntot - (ni > 0 ? ni : 0);
-    const double share = ni > 0 ? ni / (ni + 2.0 * nb) : 0.0;
+    const bool interior = ctx->box[0] < ctx->box[1] && ctx->box[2] < ctx->box[3] && ctx->box[4] < ctx->box[5];
     if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0)) != hipSuccess) return e;
-    if (ni > 0 && flags) {
-        e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it, ctx->stream, share);
+    if (interior && flags) {
+        e = launch_kernel(ctx, pick_kernel<false>(ctx, field), false, field, flags & ~F_PML_ALL, it, ctx->stream);
         if (e != hipSuccess) return e;
     }
-    e = launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it, ctx->stream2, 1.0 - share);
+    e = launch_kernel(ctx, pick_kernel<true>(ctx, field), true, field, flags, it, ctx->stream2);
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ctx->ev_join, ctx->stream2)) != hipSuccess) return e;
     return hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
@@ -1915,13 +1907,21 @@ static bool split_eligible(const gpx_ctx *ctx)
 static bool split_ok(const gpx_ctx *ctx)
 {
     if (ctx->fused == 3 || ctx->fused == 5) return split_eligible(ctx);
-    return ctx->fused == 4 && ctx->split_choice != 0 && split_eligible(ctx);
+    const int c = ctx->split_choice;
+    return ctx->fused == 4 && (c == -1 || c == 1 || c == 2) && split_eligible(ctx);
 }
 
 // Form 5, or form 4 after it chose form 5 (with split_ok).
 static bool split_beside(const gpx_ctx *ctx)
 {
     return ctx->fused == 5 || (ctx->fused == 4 && ctx->split_choice == 2);
+}
+
+// Form 2, or form 4 after it chose form 2 (when neither the fused step nor
+// the split step runs).
+static bool passes_beside(const gpx_ctx *ctx)
+{
+    return ctx->fused == 2 || (ctx->fused == 4 && ctx->split_choice == 3);
 }
 
 static hipError_t launch_interior(const gpx_ctx *ctx, int it)
@@ -2078,8 +2078,9 @@ struct EventSet {
 // and [3]; or H interior, H boundary, E interior and E boundary.
 static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4);
 
-// Form 4 picks among form 5, the split step and the passes on the first run
-// long enough to time them: after two warm-up iterations, TUNE_ITS
+// Form 4 picks among form 2, form 5, the split step and the passes (form 2
+// and the passes only, where the split step does not apply) on the first
+// run long enough to time them: after two warm-up iterations, TUNE_ITS
 // iterations of each, timed with events; the fastest serves the rest of this
 // run and later runs.  The forms give the same bits, so the choice does not
 // change results.
@@ -2087,26 +2088,28 @@ static const int TUNE_ITS = 6;
 
 static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
-    if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || !split_eligible(ctx) ||
-        n < 2 + 4 * TUNE_ITS)
+    if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || n < 2 + 4 * TUNE_ITS)
         return run_steps_form(ctx, it0, n, ms4);
-    // choices in the order timed: form 5, the split step, the passes
-    static const int order[3] = {2, 1, 0};
+    // choices in the order timed: form 2, form 5, the split step, the passes
+    static const int all[4] = {3, 2, 1, 0}, passes_only[2] = {3, 0};
+    const bool split = split_eligible(ctx);
+    const int *order = split ? all : passes_only;
+    const int nf = split ? 4 : 2;
     EventSet ev;
-    ev.e.assign(4, nullptr);
+    ev.e.assign(nf + 1, nullptr);
     for (auto &x : ev.e) GPX_TRY(hipEventCreate(&x));
     ctx->split_choice = order[0];
     int rc = run_steps_form(ctx, it0, 2, nullptr);
-    for (int f = 0; f < 3 && !rc; ++f) {
+    for (int f = 0; f < nf && !rc; ++f) {
         ctx->split_choice = order[f];
         rc = check_hip(ctx, hipEventRecord(ev.e[f], ctx->stream));
         if (!rc) rc = run_steps_form(ctx, it0 + 2 + f * TUNE_ITS, TUNE_ITS, nullptr);
     }
-    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[3], ctx->stream));
-    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[3]));
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[nf], ctx->stream));
+    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[nf]));
     int best = 0;
     float t_best = 0;
-    for (int f = 0; f < 3 && !rc; ++f) {
+    for (int f = 0; f < nf && !rc; ++f) {
         float t = 0;
         rc = check_hip(ctx, hipEventElapsedTime(&t, ev.e[f], ev.e[f + 1]));
         if (!rc && (f == 0 || t < t_best)) {
@@ -2119,7 +2122,7 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
         return rc;
     }
     ctx->split_choice = order[best];
-    const int done = 2 + 3 * TUNE_ITS;
+    const int done = 2 + nf * TUNE_ITS;
     return run_steps_form(ctx, it0 + done, n - done, nullptr);
 }
 
@@ -2157,11 +2160,11 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
             GPX_TRY(ensure_stream2(ctx));
             GPX_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
             GPX_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, ctx->stream2, 1.0, true));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, ctx->stream2, true));
             GPX_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
             GPX_TRY(launch_interior(ctx, it));
             GPX_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, 1.0, true));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, true));
             for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
             continue;
         }
@@ -2169,14 +2172,14 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
             if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
             GPX_TRY(launch_interior(ctx, it));
             if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
-            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, nullptr, 1.0, true));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 1), true, 1, hf, it, nullptr, true));
             if (ev) GPX_TRY(hipEventRecord(ev[2], ctx->stream));
-            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, 1.0, true));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, true));
             if (ev) GPX_TRY(hipEventRecord(ev[3], ctx->stream));
             for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
             continue;
         }
-        if (ctx->fused == 2 && !ms4) {
+        if (passes_beside(ctx) && !ms4) {
             GPX_TRY(launch_pass_overlap(ctx, 1, hf, it));
             GPX_TRY(launch_tl(ctx, 1, it));
             GPX_TRY(launch_pass_overlap(ctx, 0, ef, it));
@@ -2223,7 +2226,7 @@ extern "C" int gpx_step_form(const gpx_ctx *ctx)
     if (!ctx) return GPX_EARG;
     if (step_tile(ctx)) return 1;
     if (split_ok(ctx)) return split_beside(ctx) ? 5 : 3;
-    return ctx->fused == 2 ? 2 : 0;
+    return passes_beside(ctx) ? 2 : 0;
 }
 
 extern "C" int gpx_step_choice(const gpx_ctx *ctx)

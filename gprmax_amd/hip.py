@@ -196,8 +196,9 @@ class HipSolver:
         """Step form of gpx_run: 0 the H and E passes, 1 the fused step, 2 the
         passes with interior and boundary kernels side by side, 3 the split
         step, 5 the split step with the H boundary pass beside the interior
-        kernel, 4 (the library's default) the fastest of 5, 3 and 0, timed on
-        the first run of at least 26 iterations.  All give the same bits."""
+        kernel, 4 (the library's default) the fastest of 2, 5, 3 and 0 (of 2
+        and 0 where the split step does not apply), timed on the first run of
+        at least 26 iterations.  All give the same bits."""
         self._check(self.lib.gpx_set_step(self.ctx, int(fused), int(tile_cells)))
 
     def step_kind(self):
@@ -211,7 +212,7 @@ class HipSolver:
 
     def step_choice(self):
         """The default form's choice: -1 not timed yet, 0 passes, 1 split,
-        2 split with the H boundary pass beside."""
+        2 split with the H boundary pass beside, 3 passes side by side."""
         return int(self.lib.gpx_step_choice(self.ctx))
 
     def run(self, it0=0, n=None):

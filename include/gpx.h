@@ -165,15 +165,16 @@ int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
 /* Step form of gpx_run.  fused = 1: one kernel per iteration that does the
  * H and the E update with the state double-buffered on the device; fused = 0:
  * an H pass and an E pass per iteration; fused = 2: the passes with their
- * interior and boundary kernels running side by side on two streams
- * (gpx_run_profiled always runs them one after the other); fused = 3: the
+ * interior and boundary kernels running side by side on two streams, one
+ * block per item each (gpx_run_profiled always runs them one after the
+ * other); fused = 3: the
  * split step, one kernel doing H and E for the interior and the H and E
  * boundary passes around it, double-buffered; fused = 5: the split step with
  * the H boundary pass running beside the interior kernel on a second stream
  * (gpx_run_profiled runs them one after the other); fused = 4 (default): the
- * first gpx_run of at least 26 iterations times form 5, the split step and
- * the passes on six iterations each and keeps the fastest
- * (gpx_step_choice).  tile_cells = cells per fused block (256, 512 or 1024;
+ * first gpx_run of at least 26 iterations times form 2, form 5, the split
+ * step and the passes (form 2 and the passes where the split step does not
+ * apply) on six iterations each and keeps the fastest (gpx_step_choice).  tile_cells = cells per fused block (256, 512 or 1024;
  * 0 = auto).  All forms give identical results; the
  * fused form needs a tile of at least twice nz+1 cells within the LDS budget,
  * the split step a model without dispersive poles, and the passes run when
@@ -190,7 +191,8 @@ int gpx_step_kind(const gpx_ctx *ctx);
  * ms4[0] and its boundary passes in ms4[1] and ms4[3]. */
 int gpx_step_form(const gpx_ctx *ctx);
 
-/* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step, 2 form 5. */
+/* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step, 2 form 5,
+ * 3 form 2. */
 int gpx_step_choice(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);
