@@ -28,45 +28,55 @@ HBM_PEAK_GBS = 8000.0
 METRIC = 'Mcells/s + % HBM roofline; batched B-scan and adjoint-grad time vs plain (1 GPU)'
 
 
+def host_cpu():
+    """Model name and physical core count of the host, from lscpu."""
+    import subprocess
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return 'unknown CPU', None
+    info = {}
+    for ln in out.splitlines():
+        if ':' in ln:
+            k, v = ln.split(':', 1)
+            info[k.strip()] = v.strip()
+    try:
+        phys = int(info['Core(s) per socket']) * int(info['Socket(s)'])
+    except (KeyError, ValueError):
+        phys = None
+    return info.get('Model name', 'unknown CPU'), phys
+
+
 def cpu_baseline(workload, seconds=12.0):
-    """The reference's CPU path on the host cores, on the first iterations of
-    the same workload, until ~`seconds` of CPU work have run: its own Cython
-    kernels compiled from its sources into oracle/_ref (kind "reference"),
-    or, where those are not built, the oracle's C restatement of them (kind
-    "port").  Both give the same bits; the source updates are NumPy in both."""
-    import numpy as np  # noqa: F401
+    """The reference's CPU path, as the oracle's C restatement of its Cython
+    kernels (same loops, same OpenMP static schedule over i, float32, no FMA
+    contraction; kind "port"), on the first iterations of the same workload
+    until ~`seconds` of CPU work have run.  Threads: OMP_NUM_THREADS (the
+    host's CPU share for one GPU), reported as `cores`; lscpu's model and
+    physical core count go into the sample text."""
     from gprmax_amd.workloads import WORKLOADS
     from oracle import load
-    from oracle.solver import default_threads, initialise_state, load_reference, step, step_reference
+    from oracle.solver import default_threads, initialise_state, step
 
     G = WORKLOADS[workload]()
     threads = default_threads()
-    mods = load_reference('exact')
-    if mods is not None:
-        kind, what = 'reference', "the reference's Cython kernels built from its sources (-O3 -fopenmp -ffp-contract=off)"
-
-        def one(it):
-            step_reference(mods, G, it, threads)
-    else:
-        lib = load(G.floattype)
-        kind, what = 'port', 'the C restatement of the reference kernels'
-
-        def one(it):
-            step(lib, G, it, threads)
+    lib = load(G.floattype)
+    model, phys = host_cpu()
     initialise_state(G)
-    one(0)  # first touch of every array
+    step(lib, G, 0, threads)  # first touch of every array
     n, t0 = 0, time.perf_counter()
     while True:
-        one(n + 1)
+        step(lib, G, n + 1, threads)
         n += 1
         el = time.perf_counter() - t0
         if el >= seconds or n + 2 >= G.iterations:
             break
     cells = G.nx * G.ny * G.nz
     return {'value': round(cells * n / el / 1e6, 2), 'unit': 'Mcells/s', 'cores': threads,
-            'kind': kind,
-            'sample': '{} iterations (1..{}) of {}, {}, {} OpenMP threads, {:.1f} s'.format(
-                n, n, G.title, what, threads, el)}
+            'kind': 'port',
+            'sample': '{} iterations (1..{}) of {}, the C restatement of the reference kernels '
+                      '(-O3 -fopenmp -ffp-contract=off), {} OpenMP threads on {} ({} physical cores), '
+                      '{:.1f} s'.format(n, n, G.title, threads, model, phys if phys else 'unknown', el)}
 
 
 def measured_traffic(workload, kernel):
@@ -250,6 +260,34 @@ def bench_adjoint(args):
     return 0
 
 
+def step_form_label(fused, form, choice, default_form):
+    """The step form that ran, and how it was chosen."""
+    name = ('fused, {} cells per block'.format(fused) if fused else
+            {3: 'split', 5: 'split, H boundary beside the interior kernel',
+             2: 'two-pass side by side'}.get(form, 'two-pass'))
+    if not default_form:
+        return name + ' (set on the command line)'
+    if choice == -1:
+        return name + ' (default form, not timed: the run was too short to time the forms)'
+    return name + ' (default form: timed against the others and kept)'
+
+
+def calibrate(args):
+    """A device copy of known size, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs: what the counters read for exactly `bytes` read and written gives
+    the scale that turns the solver kernels' counts into HBM bytes."""
+    import torch
+    n = args.calibrate_mib * (1 << 20) // 4
+    src = torch.empty(n, dtype=torch.float32, device='cuda').uniform_()
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes_read': n * 4, 'copy_bytes_written': n * 4, 'reps': 3}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -265,13 +303,17 @@ def main():
     ap.add_argument('--passes', action='store_true', help='the H and E passes (step form 0)')
     ap.add_argument('--split', action='store_true',
                     help='the split step (form 3): one kernel doing H and E for the interior, boundary passes '
-                         'around it; by default the library times it, form 5 and the passes and keeps the fastest')
+                         'around it; by default the library times form 2, form 5, the split step and the passes '
+                         '(form 2 and the passes where the split step does not apply) and keeps the fastest')
     ap.add_argument('--split-beside', action='store_true',
                     help='the split step with the H boundary pass beside the interior kernel (form 5)')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
     ap.add_argument('--fd-cells', type=int, default=0, help='finite-difference check cells (adjoint workload)')
+    ap.add_argument('--calibrate', action='store_true',
+                    help='only run a device copy of --calibrate-mib MiB (counter calibration under rocprofv3)')
+    ap.add_argument('--calibrate-mib', type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this solver runs on one GPU; --gpus {} is not supported '
@@ -279,6 +321,8 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.calibrate:
+        return calibrate(args)
     if args.workload == 'bscan':
         return bench_bscan(args)
     if args.workload == 'adjoint':
@@ -295,6 +339,7 @@ def main():
     if W + K > G.iterations:
         raise SystemExit('warmup + steps ({}) exceed the workload iterations ({})'.format(W + K, G.iterations))
     info = device_info(0)
+    default_form = not (args.fused or args.overlap or args.passes or args.split or args.split_beside)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
         s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (
@@ -305,7 +350,8 @@ def main():
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
         s.synchronize()
-        form = s.step_form()   # after the runs: the default form times two forms on its first long run
+        form = s.step_form()       # after the runs: the default form times four forms on its first long run
+        choice = s.step_choice()   # -1: the default form has not timed them (a run shorter than 26 iterations)
         # Per-kernel durations (HIP events around every launch on the solver's
         # stream), in a separate run after the timed region.
         n_prof = min(50, G.iterations - W - K)
@@ -354,13 +400,7 @@ def main():
                      'kernels_ms': {k: round(v[1], 5) for k, v in kernels.items()},
                      'kernels_bytes': {k: v[0] for k, v in kernels.items()},
                      'step_achieved': round(step_gbs, 1),
-                     'step_form': ('fused, {} cells per block'.format(fused) if fused else
-                                   {3: 'split', 5: 'split, H boundary beside the interior kernel',
-                                    2: 'two-pass side by side'}.get(form, 'two-pass')) +
-                                  (' (timed against the others and kept)' if not (args.fused or args.overlap or
-                                                                                   args.passes or args.split or
-                                                                                   args.split_beside)
-                                   else ''),
+                     'step_form': step_form_label(fused, form, choice, default_form),
                      'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),

@@ -723,8 +723,10 @@ struct gpx_ctx {
     int plane = 0;
     hipStream_t stream = nullptr;
     float *F[6] = {nullptr};  // Ex, Ey, Ez, Hx, Hy, Hz (current state)
-    float *Fn[6] = {nullptr}; // the next state written by the fused step
-    float *Fmem[2] = {nullptr, nullptr};  // the two states, 6 x fstride floats each
+    float *Fn[6] = {nullptr}; // the next state written by the double-buffered forms (1, 3, 5)
+    float *Fmem[2] = {nullptr, nullptr};  // the two states, 6 x fstride floats each; the second is
+                                          // allocated only when a double-buffered form runs
+    size_t next_bytes = 0;    // bytes of the second state and of the second H-PML Phi copies
     long long fstride = 0;    // floats between components: ntot rounded up to 64
     void *ID = nullptr;       // [6][ntot] as uint8, uint16 or uint32
     int id_bytes = 4;         // 2 when every index < nmat fits 16 bits, else 4
@@ -830,6 +832,17 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
         return fail(nullptr, GPX_ELIMIT, "gpx_create: at most " + std::to_string(MAX_POLES) + " dispersive poles are supported");
     if (d->pml_formulation != 0 && d->pml_formulation != 1)
         return fail(nullptr, GPX_EARG, "gpx_create: pml_formulation must be 0 (HORIPML) or 1 (MRIPML)");
+    {
+        // The kernels address a field array with 32-bit byte offsets (buffer
+        // instructions: plane offset plus lane offset), so a component may
+        // hold at most 2^30 - 1 points; larger grids are refused here rather
+        // than wrapping.
+        const unsigned long long pts = (unsigned long long)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+        if (pts * 4ull > 0xFFFFFFFFull)
+            return fail(nullptr, GPX_ELIMIT,
+                        "gpx_create: " + std::to_string(pts) + " points per field component exceed the limit of " +
+                            std::to_string(0xFFFFFFFFull / 4) + " (32-bit byte offsets)");
+    }
     size_t lds = (size_t)d->nmat * 5 * 4 + (size_t)d->nmat * 3 * d->maxpoles * 2 * 4 + d->nmat;
     if (lds > (size_t)MAX_LDS_BYTES)
         return fail(nullptr, GPX_ELIMIT, "gpx_create: too many materials for the LDS coefficient table");
@@ -855,13 +868,11 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
         return rc;
     }
     ctx->fstride = (ctx->ntot + 63) / 64 * 64;
-    for (int s = 0; s < 2; ++s) {
-        e = dalloc(ctx, (void **)&ctx->Fmem[s], (size_t)6 * ctx->fstride * 4);
-        if (e != hipSuccess) goto fail_alloc;
-    }
+    e = dalloc(ctx, (void **)&ctx->Fmem[0], (size_t)6 * ctx->fstride * 4);
+    if (e != hipSuccess) goto fail_alloc;
     for (int c = 0; c < 6; ++c) {
         ctx->F[c] = ctx->Fmem[0] + c * ctx->fstride;
-        ctx->Fn[c] = ctx->Fmem[1] + c * ctx->fstride;
+        ctx->Fn[c] = nullptr;
     }
     ctx->id_bytes = d->nmat <= 65536 ? 2 : 4;
     e = dalloc(ctx, (void **)&ctx->ID, 6 * (size_t)ctx->ntot * ctx->id_bytes, false);
@@ -939,6 +950,63 @@ static int copy_out(gpx_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     GPX_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     GPX_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- second state
+// The double-buffered forms write the next state into a second copy of the
+// fields (the fused step and the split step) and the fused step also the
+// H-PML Phi into a second copy.  The copies are made on first use, equal to
+// the current state, so a model that only ever updates in place (the passes,
+// dispersive models, models with transmission lines) holds one field state.
+static int ensure_next_state(gpx_ctx *ctx, bool phi)
+{
+    const size_t fb = (size_t)6 * ctx->fstride * 4;
+    if (!ctx->Fmem[1]) {
+        // Without a second state the current one is Fmem[0].
+        float *m = nullptr;
+        GPX_TRY(dalloc(ctx, (void **)&m, fb, false));
+        ctx->Fmem[1] = m;
+        ctx->next_bytes += fb;
+        for (int c = 0; c < 6; ++c) ctx->Fn[c] = m + c * ctx->fstride;
+        GPX_TRY(hipMemcpyAsync(m, ctx->Fmem[0], fb, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (phi) {
+        for (Slab &S : ctx->slabs)
+            for (int q = 0; q < 2; ++q) {
+                if (S.phin[q]) continue;
+                const size_t b = (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4;
+                GPX_TRY(dalloc(ctx, (void **)&S.phin[q], b, false));
+                ctx->next_bytes += b;
+                GPX_TRY(hipMemcpyAsync(S.phin[q], S.phi[q + 2], b, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+    }
+    return 0;
+}
+
+// Gives the second copies back (the default form settled on a form that
+// updates in place).  The current state is whichever copy F points into.
+static int release_next_state(gpx_ctx *ctx)
+{
+    if (!ctx->Fmem[1]) return 0;
+    GPX_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->stream2) GPX_TRY(hipStreamSynchronize(ctx->stream2));
+    const size_t fb = (size_t)6 * ctx->fstride * 4;
+    if (ctx->F[0] == ctx->Fmem[1]) std::swap(ctx->Fmem[0], ctx->Fmem[1]);
+    GPX_TRY(hipFree(ctx->Fmem[1]));
+    ctx->Fmem[1] = nullptr;
+    for (int c = 0; c < 6; ++c) ctx->Fn[c] = nullptr;
+    ctx->bytes -= fb;
+    ctx->next_bytes -= fb;
+    for (Slab &S : ctx->slabs)
+        for (int q = 0; q < 2; ++q) {
+            if (!S.phin[q]) continue;
+            const size_t b = (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4;
+            GPX_TRY(hipFree(S.phin[q]));
+            S.phin[q] = nullptr;
+            ctx->bytes -= b;
+            ctx->next_bytes -= b;
+        }
     return 0;
 }
 
@@ -1167,7 +1235,6 @@ extern "C" int gpx_add_pml(gpx_ctx *ctx, int dir, int xs, int xf, int ys, int yf
     for (int q = 0; q < 4; ++q) {
         size_t b = (size_t)order * s.dims[q][0] * s.dims[q][1] * s.dims[q][2] * 4;
         GPX_TRY(dalloc(ctx, (void **)&s.phi[q], b));
-        if (q >= 2) GPX_TRY(dalloc(ctx, (void **)&s.phin[q - 2], b));
     }
     const size_t rb = (size_t)4 * order * t * 4;
     GPX_TRY(dalloc(ctx, (void **)&s.eR, rb, false));
@@ -2122,6 +2189,9 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
         return rc;
     }
     ctx->split_choice = order[best];
+    // the passes, alone or side by side, update in place
+    if (order[best] == 0 || order[best] == 3)
+        if ((rc = release_next_state(ctx))) return rc;
     const int done = 2 + nf * TUNE_ITS;
     return run_steps_form(ctx, it0 + done, n - done, nullptr);
 }
@@ -2132,6 +2202,8 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
     const int hf = h_flags(ctx), ef = e_flags(ctx);
     const int tc = step_tile(ctx);
     const bool split = !tc && split_ok(ctx);
+    if (n > 0 && (tc || split))
+        if (int rc = ensure_next_state(ctx, tc != 0)) return rc;
     // With ms4, six events per iteration are recorded around the kernels and
     // read once at the end, so the host never waits between iterations and
     // the gaps between events are the kernels' own.
@@ -2337,10 +2409,12 @@ extern "C" int gpx_download_fields(gpx_ctx *ctx, float *out6)
 extern "C" int gpx_upload_fields(gpx_ctx *ctx, const float *in6)
 {
     if (!ctx || !in6) return fail(ctx, GPX_EARG, "gpx_upload_fields: null argument");
-    // Both states: cells that no update touches keep their value in either.
+    // Both states, where there are two: cells that no update touches keep
+    // their value in either.
     for (int c = 0; c < 6; ++c) {
         if (int rc = copy_in(ctx, ctx->F[c], in6 + (size_t)c * ctx->ntot, (size_t)ctx->ntot * 4)) return rc;
-        if (int rc = copy_in(ctx, ctx->Fn[c], in6 + (size_t)c * ctx->ntot, (size_t)ctx->ntot * 4)) return rc;
+        if (ctx->Fn[c])
+            if (int rc = copy_in(ctx, ctx->Fn[c], in6 + (size_t)c * ctx->ntot, (size_t)ctx->ntot * 4)) return rc;
     }
     return 0;
 }
@@ -2370,7 +2444,7 @@ extern "C" int gpx_upload_pml_phi(gpx_ctx *ctx, int slab, int which, const float
     float *p;
     size_t b;
     if (int rc = phi_ref(ctx, slab, which, &p, &b)) return rc;
-    if (which >= 2)  // H Phi is double-buffered like the fields
+    if (which >= 2 && ctx->slabs[slab].phin[which - 2])  // H Phi, double-buffered for the fused step
         if (int rc = copy_in(ctx, ctx->slabs[slab].phin[which - 2], in, b)) return rc;
     return copy_in(ctx, p, in, b);
 }
@@ -2450,7 +2524,13 @@ extern "C" int gpx_device_info(int device, char *name, int namelen, size_t *tota
     hipError_t e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) return fail(nullptr, (int)e, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     if (name && namelen > 0) {
-        snprintf(name, (size_t)namelen, "%s (%s)", prop.name, prop.gcnArchName);
+        // Some runtimes leave the marketing name empty; the architecture and
+        // the compute-unit count still name the part.
+        if (prop.name[0])
+            snprintf(name, (size_t)namelen, "%s (%s)", prop.name, prop.gcnArchName);
+        else
+            snprintf(name, (size_t)namelen, "AMD %s, %d CUs, %zu GiB", prop.gcnArchName, prop.multiProcessorCount,
+                     (size_t)(prop.totalGlobalMem >> 30));
     }
     if (totalmem) *totalmem = prop.totalGlobalMem;
     if (cus) *cus = prop.multiProcessorCount;
@@ -2636,6 +2716,7 @@ static int batch_prepare(gpx_batch *b, int tc, int xchunk, BatchPlan &p)
             hipError_t e = upload_rare(c);
             if (e != hipSuccess) return bfail(b, (int)e, std::string("gpx_batch: ") + hipGetErrorString(e));
         }
+        if (int rc = ensure_next_state(c, true)) return bfail(b, rc, "gpx_batch: " + c->err);
         const size_t l = step_lds(c, tc);
         if (l > p.lds) p.lds = l;
         if (c->desc.nx + 1 > maxnx1) maxnx1 = c->desc.nx + 1;
@@ -2789,7 +2870,7 @@ static int batch_prepare_passes(gpx_batch *b, PassPlan &p)
     double acc = 0;
     for (int t = 0; t < n; ++t) {
         const gpx_ctx *c = b->ctx[t];
-        const double bytes = (double)c->bytes - 6.0 * c->fstride * 4;  // the second state is not used
+        const double bytes = (double)(c->bytes - c->next_bytes);  // the second copies are not used
         if (p.groups.empty() || acc + bytes > b->group_bytes) {
             p.groups.push_back(PassGroup());
             p.groups.back().g0 = t;
@@ -2922,12 +3003,14 @@ static int zero_state(gpx_ctx *ctx)
 {
     gpx_ctx *c = ctx;
     hipStream_t s = c->stream;
-    for (float *m : c->Fmem) GPX_TRY(hipMemsetAsync(m, 0, (size_t)6 * c->fstride * 4, s));
+    for (float *m : c->Fmem)
+        if (m) GPX_TRY(hipMemsetAsync(m, 0, (size_t)6 * c->fstride * 4, s));
     for (const Slab &S : c->slabs) {
         for (int q = 0; q < 4; ++q)
             GPX_TRY(hipMemsetAsync(S.phi[q], 0, (size_t)S.order * S.dims[q][0] * S.dims[q][1] * S.dims[q][2] * 4, s));
         for (int q = 0; q < 2; ++q)
-            GPX_TRY(hipMemsetAsync(S.phin[q], 0, (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4, s));
+            if (S.phin[q])
+                GPX_TRY(hipMemsetAsync(S.phin[q], 0, (size_t)S.order * S.dims[q + 2][0] * S.dims[q + 2][1] * S.dims[q + 2][2] * 4, s));
     }
     if (c->desc.maxpoles > 0)
         for (int q = 0; q < 3; ++q)

@@ -9,8 +9,25 @@ from gprmax_amd.materials import add_dispersion
 from gprmax_amd.pml import CFS
 
 
+def randomise_tables(G, seed):
+    """Replace the coefficient tables (every column of updatecoeffsE/H drawn
+    on its own) and every slab's R tables with random values, so that a
+    swapped coefficient column (CBx/CBy/CBz, DBx/DBy/DBz, srce/srcm) or a
+    profile read at the wrong index changes the results.  The model keeps
+    its geometry, sources and dispersive tables."""
+    rng = np.random.default_rng(seed)
+    n = len(G.updatecoeffsE)
+    G.updatecoeffsE = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    G.updatecoeffsH = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    for p in G.pmls:
+        for name in ('ERA', 'ERB', 'ERE', 'ERF', 'HRA', 'HRB', 'HRE', 'HRF'):
+            a = getattr(p, name)
+            setattr(p, name, rng.uniform(0.5, 1.5, a.shape).astype(a.dtype))
+    return G
+
+
 def rich_model(seed=0, shape=(26, 22, 19), iterations=40, pml=4, order=1, formulation='HORIPML',
-               poles=0, kind='debye', nmat=9, snapshots=True):
+               poles=0, kind='debye', nmat=9, snapshots=True, cell=(1e-3, 1e-3, 1e-3)):
     rng = np.random.default_rng(seed)
     nx, ny, nz = shape
     cfs = None
@@ -19,7 +36,7 @@ def rich_model(seed=0, shape=(26, 22, 19), iterations=40, pml=4, order=1, formul
         b.alpha.max = 0.05
         b.kappa.scalingprofile, b.kappa.min, b.kappa.max = 'quadratic', 1, 3
         cfs = [a, b]
-    bld = ModelBuilder(nx, ny, nz, 1e-3, 1e-3, 1e-3, iterations, pml_cells=pml,
+    bld = ModelBuilder(nx, ny, nz, *cell, iterations, pml_cells=pml,
                        pml_formulation=formulation, cfs=cfs)
     mats = []
     for n in range(nmat):

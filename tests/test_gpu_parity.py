@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from oracle import solver as osolver
-from gpu_models import plane_model, rich_model
+from gpu_models import plane_model, randomise_tables, rich_model
 from models import BASIC_MODELS, PLANE_MODELS
 
 pytestmark = pytest.mark.gpu
@@ -85,7 +85,26 @@ def test_each_substep_matches_oracle(case):
     """update_magnetic, every H-PML slab, magnetic sources, update_electric /
     dispersive A, every E-PML slab, electric sources, dispersive B, receivers
     and snapshots, each from the same random state."""
-    G = rich_model(seed=7, **CASES[case])
+    _check_substeps(rich_model(seed=7, **CASES[case]))
+
+
+# Cells of 1, 1.5 and 2 mm: the three CB (DB) columns of a physical table and
+# the three slab spacings all differ.
+ANISO = (1e-3, 1.5e-3, 2e-3)
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_each_substep_matches_oracle_anisotropic_random_tables(case):
+    """The sub-steps on cells of 1 x 1.5 x 2 mm with every coefficient
+    column and every PML R table drawn at random: a swapped CBx/CBy/CBz or
+    DBx/DBy/DBz column, a wrong source column or a slab spacing or profile
+    taken from the wrong axis breaks the match."""
+    G = randomise_tables(rich_model(seed=17, cell=ANISO, **CASES[case]), 23)
+    assert len({G.dx, G.dy, G.dz}) == 3
+    _check_substeps(G)
+
+
+def _check_substeps(G):
     lib = oracle.load(np.float32)
     f = lib.funcs
     it = 9
@@ -225,6 +244,31 @@ def test_split_step_from_random_state(case, xchunk, form):
     for r, rx in enumerate(G.rxs):
         for c, name in enumerate(COMPS):
             assert np.array_equal(buf[c, 2:8, r], rx.outputs[name][2:8]), (r, name)
+
+
+@pytest.mark.parametrize('fused', **FORMS)
+@pytest.mark.parametrize('case', list(CASES))
+def test_run_from_random_state_anisotropic_random_tables(case, fused):
+    """Every step form from a random state (fields, Phi, T) on 1 x 1.5 x 2 mm
+    cells with random coefficient and R tables, on a grid with several blocks
+    per plane: state and receiver samples equal to the oracle's.  Forms the
+    model cannot take run the passes, as gpx_run does."""
+    G = randomise_tables(rich_model(seed=27, shape=WIDE, iterations=12, snapshots=False, cell=ANISO,
+                                    **CASES[case]), 29)
+    _random_state(G, 31)
+    lib = oracle.load(np.float32)
+    with _hip()(G) as s:
+        s.set_step(fused)
+        s.set_launch(3, 0)
+        _push_state(s, G)
+        s.run(2, 8)
+        for it in range(2, 10):
+            osolver.step(lib, G, it, 8)
+        _assert_state_equal(s, G, 'form {} from a random state'.format(fused))
+        buf = s.download_rx()
+    for r, rx in enumerate(G.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(buf[c, 2:10, r], rx.outputs[name][2:10]), (r, name)
 
 
 PLANE_CASES = {
@@ -446,8 +490,9 @@ def test_pass_batch_of_unlike_models():
     ref = _single_runs(models, 18)
     for groups in ('one', 'each', 'pairs'):
         solvers = [_hip()(G) for G in models]
-        # a model's grouped state: its bytes less the fused step's second field copy
-        state = [s.memory_bytes() - 6 * 4 * (G.nx + 1) * (G.ny + 1) * (G.nz + 1) for s, G in zip(solvers, models)]
+        # a model's grouped state: its bytes (a model holds a second field
+        # state only once a double-buffered form has run)
+        state = [s.memory_bytes() for s in solvers]
         group_bytes = {'one': 0, 'each': 1, 'pairs': max(state[0] + state[1], state[2] + state[3]) + 4096}[groups]
         try:
             with HipBatch(solvers) as b:

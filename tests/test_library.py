@@ -48,6 +48,25 @@ def test_create_rejects_bad_descriptors_before_touching_a_device():
     assert lib.gpx_create(None, ctypes.byref(ctx)) < 0
 
 
+def test_create_refuses_grids_past_the_32_bit_offset_limit():
+    """A field component of 2^30 points or more would need byte offsets past
+    4 GiB; gpx_create refuses it (GPX_ELIMIT) before touching a device, rather
+    than letting the kernels' 32-bit offsets wrap.  The reference accepts any
+    grid its memory check passes (gprMax/grid.py:227-245), so the refusal
+    names the limit."""
+    from gprmax_amd.hip import GpxDesc, library
+    lib = library()
+    ctx = ctypes.c_void_p()
+    for shape in [(1024, 1024, 1024), (1023, 1023, 1024), (4095, 4095, 63), (2, 2**16, 2**14)]:
+        pts = (shape[0] + 1) * (shape[1] + 1) * (shape[2] + 1)
+        assert pts * 4 > 0xFFFFFFFF
+        d = GpxDesc(*shape, 5, 2, 0, 0, 0)
+        assert lib.gpx_create(ctypes.byref(d), ctypes.byref(ctx)) == -3, shape
+        msg = lib.gpx_last_error(None).decode()
+        assert 'points per field component' in msg and str(pts) in msg, msg
+        assert not ctx.value
+
+
 def test_oracle_and_product_are_separate():
     """The product package never imports the oracle."""
     pkg = os.path.join(ROOT, 'gprmax_amd')
