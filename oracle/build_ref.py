@@ -1,11 +1,11 @@
 """Compile the reference's own Cython kernels of the solve path into oracle/_ref.
 
 TEST INFRASTRUCTURE.  Used only in a container where the reference tree is
-mounted, to pin the C restatement in ``oracle/fdtd_oracle.c`` bit for bit and
-to time the reference's CPU kernels.  Nothing here is copied: the .pyx sources
-are compiled where they lie; the generated C and the built modules go to
-``oracle/_ref/`` (git-ignored; it travels with GPU runs, where bench.py times
-the exact build as the CPU baseline).
+mounted, to pin the C restatement in ``oracle/fdtd_oracle.c`` and the host
+edge build bit for bit, and to time the reference's CPU kernels beside the C
+restatement.  Nothing here is copied: the .pyx sources are compiled where
+they lie; the generated C and the built modules go to ``oracle/_ref/``
+(git-ignored, and listed in .gpurunignore: it never travels to a GPU host).
 
 Two builds are made:
   exact/   -O3 -fopenmp -ffp-contract=off   (every product rounded, no FMA)
@@ -32,6 +32,10 @@ MODULES = [
     'gprMax/pml_updates/pml_updates_magnetic_HORIPML_ext',
     'gprMax/pml_updates/pml_updates_electric_MRIPML_ext',
     'gprMax/pml_updates/pml_updates_magnetic_MRIPML_ext',
+    # the edge build with dielectric averaging (yee_cell_build_ext cimports
+    # the rigid-edge accessors, so both are built)
+    'gprMax/yee_cell_setget_rigid_ext',
+    'gprMax/yee_cell_build_ext',
 ]
 FLAVOURS = {
     'exact': ['-O3', '-fopenmp', '-ffp-contract=off'],

@@ -55,3 +55,31 @@ def ref_modules():
     finally:
         sys.path.remove(REF_TREE)
     return mods
+
+
+@pytest.fixture(scope='session')
+def ref_edge_build():
+    """The reference's compiled edge build (gprMax/yee_cell_build_ext.pyx),
+    from oracle/_ref.  The module takes its Material class from
+    gprMax.materials at import; it only constructs materials and sets their
+    er, se, mr, sm and type, so it is handed this build's Material class,
+    whose attributes are the reference's."""
+    if not os.path.isdir(REF_TREE):
+        pytest.skip('reference kernels not built (oracle/build_ref.py)')
+    import importlib
+    import types
+    from gprmax_amd.materials import Material
+    sys.path.insert(0, REF_TREE)
+    shim = types.ModuleType('gprMax.materials')
+    shim.Material = Material
+    saved = sys.modules.get('gprMax.materials')
+    sys.modules['gprMax.materials'] = shim
+    try:
+        mod = importlib.import_module('gprMax.yee_cell_build_ext')
+    finally:
+        sys.path.remove(REF_TREE)
+        if saved is None:
+            del sys.modules['gprMax.materials']
+        else:
+            sys.modules['gprMax.materials'] = saved
+    return mod

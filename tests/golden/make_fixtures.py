@@ -8,6 +8,10 @@ reader and writes ``tests/golden/<model>[_test].npz`` holding:
 * ``dt``, ``dxdydz``, ``nxyz``, ``iterations``
 * ``rx_position``, ``src_position`` (metres), ``src_type``, ``version``
 
+and, from ``tests/other_codes/vs_MoM_MATLAB/antenna_bowtie_fs/antenna_bowtie_fs.out``
+(gprMax 3.0.9), ``tests/golden/antenna_bowtie_fs_tl.npz``: the incident voltage
+and current of its transmission line with the line's ``dl`` and resistance.
+
 Run once in a container that has the reference mounted:
     python tests/golden/make_fixtures.py /root/reference
 """
@@ -42,6 +46,22 @@ def convert(path, out):
                         version=np.array(root.get('gprMax', '')))
 
 
+def convert_tl(path, out):
+    """The incident voltage and current of the first transmission line of a
+    reference output file (``/tls/tl1/Vinc``, ``Iinc``) with the line's and
+    the run's attributes."""
+    d = h5mini.read_file(path)
+    root = d['/']['attrs']
+    tl = d['/tls/tl1']['attrs']
+    np.savez_compressed(out, Vinc=np.asarray(d['/tls/tl1/Vinc']['data'], dtype=np.float32),
+                        Iinc=np.asarray(d['/tls/tl1/Iinc']['data'], dtype=np.float32),
+                        dt=np.float64(root['dt']), iterations=np.int64(root['Iterations']),
+                        dxdydz=np.asarray(root.get('dx_dy_dz', root.get('dx, dy, dz')), dtype=np.float64),
+                        dl=np.float64(tl['dl']), resistance=np.float64(tl['Resistance']),
+                        position=np.asarray(tl['Position'], dtype=np.float64),
+                        version=np.array(root.get('gprMax', '')))
+
+
 def main(reference):
     here = os.path.dirname(os.path.abspath(__file__))
     base = os.path.join(reference, 'tests', 'models_basic')
@@ -49,6 +69,8 @@ def main(reference):
         convert(os.path.join(base, model, model + '_ref.out'), os.path.join(here, model + '_ref.npz'))
         convert(os.path.join(base, model, model + '.out'), os.path.join(here, model + '_test.npz'))
     convert(os.path.join(base, ANALYTICAL, ANALYTICAL + '.out'), os.path.join(here, ANALYTICAL + '_test.npz'))
+    bowtie = os.path.join(reference, 'tests', 'other_codes', 'vs_MoM_MATLAB', 'antenna_bowtie_fs')
+    convert_tl(os.path.join(bowtie, 'antenna_bowtie_fs.out'), os.path.join(here, 'antenna_bowtie_fs_tl.npz'))
 
 
 if __name__ == '__main__':

@@ -120,3 +120,20 @@ BASIC_MODELS = {
     'hertzian_dipole_dispersive': hertzian_dipole_dispersive,
 }
 BASIC_MODELS.update(PLANE_MODELS)
+
+
+def edge_model(seed, shape=(30, 26, 21), nmat=6, frac=0.05):
+    """A grid whose edges need averaging everywhere: random cell materials in
+    the lower half, an averaged box and cylinder, and rigidly painted random
+    voxels (some of them PEC and free space) on top."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    b = ModelBuilder(*shape, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
+    mats = [b.add_material(2 + 0.7 * n, 0.002 * n, 1 + 0.1 * n, 0, 'm{}'.format(n)) for n in range(nmat)]
+    nx, ny, nz = shape
+    b.G.solid[:, :, :nz // 2] = rng.integers(2, 2 + nmat, (nx, ny, nz // 2))
+    b.box(nx // 4, nx // 2, 2, ny - 3, nz // 3, nz - 2, mats[1], averaging=True)
+    b.cylinder(0.5e-3 * nx, 0.5e-3 * ny, 0, 0.5e-3 * nx, 0.5e-3 * ny, 1e-3 * nz, 4e-3, mats[2], averaging=True)
+    mask = rng.random(shape) < frac
+    b.paint_voxels(mask, rng.integers(0, 2 + nmat, shape).astype(np.uint32))
+    return b.G

@@ -1,26 +1,19 @@
 """Edge material indices built on the GPU (gprmax_amd.edges) against the host
 build (grid.build_electric_components / build_magnetic_components, itself
-checked against the reference's edge-by-edge loop in test_host_model.py)."""
+pinned bit for bit to the reference's compiled yee_cell_build_ext in
+test_host_model.py)."""
 
 import numpy as np
 import pytest
 
-from gprmax_amd.grid import ModelBuilder, build_electric_components, build_magnetic_components
+from gprmax_amd.grid import build_electric_components, build_magnetic_components
+from models import edge_model
 
 pytestmark = pytest.mark.gpu
 
 
 def _model(seed, shape=(30, 26, 21), nmat=6, frac=0.05):
-    rng = np.random.default_rng(seed)
-    b = ModelBuilder(*shape, 1e-3, 1e-3, 1e-3, 10, pml_cells=0)
-    mats = [b.add_material(2 + 0.7 * n, 0.002 * n, 1 + 0.1 * n, 0, 'm{}'.format(n)) for n in range(nmat)]
-    nx, ny, nz = shape
-    b.G.solid[:, :, :nz // 2] = rng.integers(2, 2 + nmat, (nx, ny, nz // 2))
-    b.box(nx // 4, nx // 2, 2, ny - 3, nz // 3, nz - 2, mats[1], averaging=True)
-    b.cylinder(0.5e-3 * nx, 0.5e-3 * ny, 0, 0.5e-3 * nx, 0.5e-3 * ny, 1e-3 * nz, 4e-3, mats[2], averaging=True)
-    mask = rng.random(shape) < frac
-    b.paint_voxels(mask, rng.integers(0, 2 + nmat, shape).astype(np.uint32))
-    return b.G
+    return edge_model(seed, shape, nmat, frac)
 
 
 @pytest.mark.parametrize('seed', [0, 1, 2])

@@ -297,6 +297,44 @@ def test_vectorised_edge_build_matches_edge_loop(seed):
         assert (a.er, a.se, a.mr, a.sm) == (b.er, b.se, b.mr, b.sm)
 
 
+class _RefGrid:
+    """The attributes the reference's edge build reads from its grid."""
+    IDlookup = {'Ex': 0, 'Ey': 1, 'Ez': 2, 'Hx': 3, 'Hy': 4, 'Hz': 5}
+
+    def __init__(self, G):
+        self.nx, self.ny, self.nz = G.nx, G.ny, G.nz
+        self.materials = G.materials
+        self.ID = G.ID
+
+
+@pytest.mark.parametrize('seed', [0, 1, 2])
+def test_edge_build_matches_the_reference_compiled_build(seed, ref_edge_build):
+    """The reference's own yee_cell_build_ext (compiled from its .pyx into
+    oracle/_ref) and this build's vectorised host build on the same grid:
+    identical edge indices, and identical averaged materials created in the
+    same order with the same names and constitutive values.  The rigid flags
+    go to the reference as its 12 (E) and 6 (H) edge slots per cell, all set
+    for a rigid cell (set_rigid_E / set_rigid_H write every slot of a cell).
+    The GPU edge build is compared with the host build in test_gpu_edges.py."""
+    from gprmax_amd.grid import build_electric_components, build_magnetic_components
+    from models import edge_model
+    G1, G2 = edge_model(seed), edge_model(seed)
+    build_electric_components(G1)
+    build_magnetic_components(G1)
+    rigidE = np.ascontiguousarray(np.repeat(G2.rigidcellE[None], 12, axis=0).astype(np.int8))
+    rigidH = np.ascontiguousarray(np.repeat(G2.rigidcellH[None], 6, axis=0).astype(np.int8))
+    R = _RefGrid(G2)
+    ref_edge_build.build_electric_components(G2.solid, rigidE, G2.ID, R)
+    ref_edge_build.build_magnetic_components(G2.solid, rigidH, G2.ID, R)
+    assert np.array_equal(G1.ID, G2.ID)
+    assert len(G1.materials) > 50
+    assert [m.ID for m in G1.materials] == [m.ID for m in G2.materials]
+    assert [m.numID for m in G2.materials] == list(range(len(G2.materials)))
+    for a, b in zip(G1.materials, G2.materials):
+        assert (a.er, a.se, a.mr, a.sm) == (b.er, b.se, b.mr, b.sm), a.ID
+        assert a.type == b.type, a.ID
+
+
 def test_device_tables_from_reference_like_objects():
     """solve_hip reads attributes only: the reference's own source, line,
     snapshot and grid objects (which lack this package's helper methods)

@@ -439,3 +439,35 @@ def test_reference_kernel_step_matches_the_port(ref_modules, poles):
     for rx, rxc in zip(G.rxs, Gc.rxs):
         for name in rx.outputs:
             assert np.array_equal(rx.outputs[name], rxc.outputs[name])
+
+
+@pytest.mark.parametrize('workload', ['freespace_ricker', 'hertzian_dipole_hs'])
+def test_reference_as_its_own_build_compiles_it_within_tolerance(ref_modules, workload):
+    """The reference's kernels built with its own flags (setup.py:176-180:
+    -O3 -march=native, which lets gcc contract to FMA) against the oracle's
+    uncontracted float32 restatement, which the GPU equals bit for bit: the
+    receiver traces agree within 1e-4 of the peak |E| (E components) and of
+    the peak |H| (H components), the stated tolerance.  Workloads: the 100^3
+    free-space Ricker (500 iterations) and the reference's half-space dipole
+    model (1559 iterations)."""
+    from oracle import solver as osolver
+    from gprmax_amd.workloads import freespace_ricker
+    mods = osolver.load_reference('native')
+    if mods is None:
+        pytest.skip('native reference build missing')
+    make = freespace_ricker if workload == 'freespace_ricker' else BASIC_MODELS['hertzian_dipole_hs']
+    G, Gc = make(), make()
+    osolver.initialise_state(G)
+    for it in range(G.iterations):
+        osolver.step_reference(mods, G, it, 8)
+    solve_cpu(1, 1, Gc, nthreads=8)
+    for rx, rxc in zip(G.rxs, Gc.rxs):
+        for group in (COMPS[:3], COMPS[3:]):
+            a = np.stack([rx.outputs[c] for c in group]).astype(np.float64)
+            b = np.stack([rxc.outputs[c] for c in group]).astype(np.float64)
+            peak = np.abs(a).max()
+            assert peak > 0
+            err = np.abs(a - b).max() / peak
+            assert err <= 1e-4, (workload, group, err)
+    same = all(np.array_equal(rx.outputs[c], rxc.outputs[c]) for rx, rxc in zip(G.rxs, Gc.rxs) for c in COMPS)
+    print(workload, 'native build bit-identical to the oracle:', same)

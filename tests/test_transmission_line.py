@@ -4,11 +4,15 @@ oracle.
 The host computes the incident voltage and current with NumPy, vectorised
 as the reference writes it; the oracle restates every line update element
 by element with explicit float64/float32 casts.  Agreement of the two pins
-the promotion both rely on (a planted float32 coefficient breaks it).  The
-reference holds no transmission-line output files, so the coupled solve is
-pinned by the travelling-wave relation V = R I of the matched incident line
-and by the GPU comparison against the oracle; against the reference's own
-outputs the line is parity unpinned.
+the promotion both rely on (a planted float32 coefficient breaks it).
+
+Against the reference's own output: the bowtie antenna model of
+tests/other_codes/vs_MoM_MATLAB (gprMax 3.0.9, 15,579 iterations) holds its
+line's incident voltage and current.  The incident run (update_current,
+update_voltage with the injector and the absorbing end) reproduces them with
+the file's cell length dl.  The coupled part (the loop current from H and
+E = -V/d) is pinned by the travelling-wave relation V = R I of the matched
+incident line and by the GPU comparison against the oracle.
 """
 
 import numpy as np
@@ -115,3 +119,37 @@ def test_coupled_oracle_solve_records_line_outputs():
     # on the y = 0 face the loop current is 0 (gprMax/grid.py:422-423)
     face = G.transmissionlines[3]
     assert face.ycoord == 0 and np.all(face.Itotal[1:] == 0)
+
+
+def test_incident_run_matches_the_reference_bowtie_output():
+    """The line of antenna_bowtie_fs.in (x-directed, 50 ohm, gaussian 1.5 GHz
+    of amplitude 1, 1 mm cells, 30 ns = 15,579 iterations) against /tls/tl1
+    of the reference's output file.  gprMax 3.0.9 took the line cell as
+    c dt (the file's dl attribute, 5.7735e-4 m); the current version takes
+    sqrt(3) c dt (gprMax/sources.py:299), so the run uses the file's dl.
+    Iinc agrees sample for sample; Vinc agrees one sample later in the file,
+    which stored the voltage one iteration later than the current version
+    does (the same recording-order change as the v3.0.13 H fixtures).  The
+    remaining ~1e-6 of the peak is the float32 rounding of the line
+    coefficients under the NumPy 1 promotion that version ran with."""
+    ref = np.load('tests/golden/antenna_bowtie_fs_tl.npz')
+    its, dt = int(ref['iterations']), float(ref['dt'])
+    assert its == 15579 and str(ref['version']) == '3.0.9'
+    b = ModelBuilder(200, 200, 100, 1e-3, 1e-3, 1e-3, its, pml_cells=10)
+    assert b.G.dt == dt
+    b.add_waveform('gaussian', 1, 1.5e9, 'mypulse')
+    tl = TransmissionLine(b.G)
+    tl.resistance = float(ref['resistance'])
+    tl.dl = np.float64(ref['dl'])
+    assert abs(tl.dl - 299792458.0 * dt) <= 1e-15 * tl.dl
+    cell = tuple(int(round(v / 1e-3)) for v in ref['position'])
+    b._place(tl, 'x', cell, 'mypulse', None, None)
+    tl.calculate_incident_V_I(b.G)
+    V, I = tl.Vinc, tl.Iinc
+    Vf, If = ref['Vinc'], ref['Iinc']
+    pv, pi = np.abs(Vf).max(), np.abs(If).max()
+    assert pv > 0 and pi > 0
+    assert np.abs(I - If).max() <= 2e-6 * pi
+    assert np.abs(V[:-1] - Vf[1:]).max() <= 2e-6 * pv
+    # without the one-sample shift the voltages differ by a percent of the peak
+    assert np.abs(V - Vf).max() > 1e-3 * pv
