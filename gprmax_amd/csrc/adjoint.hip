@@ -9,10 +9,23 @@
 // material (E edge (c, i, j, k) takes cell (i, j, k), the reference's
 // build_voxel without averaging), mu_r = 1 and no magnetic loss (DA = 1,
 // DB = dt / (mu0 d)), HORIPML with one CFS term on any set of slabs
-// (gprMax/pml_updates/pml_updates_*_HORIPML_ext.pyx, order1_*), one
-// z-directed Hertzian dipole per solve (gprMax/sources.py:163-193), and Ez
-// receivers sampled at the start of each iteration
-// (gprMax/fields_outputs.py:40-67).
+// (gprMax/pml_updates/pml_updates_*_HORIPML_ext.pyx, order1_*), optional
+// Debye poles per cell with real auxiliary values (dispersive parts A and B,
+// gprMax/fields_updates_ext.pyx:113-235; coefficients of
+// gprMax/materials.py:92-143), one z-directed Hertzian dipole per solve
+// (gprMax/sources.py:163-193), and Ez receivers sampled at the start of
+// each iteration (gprMax/fields_outputs.py:40-67).
+//
+// With poles, an updated E edge of a cell with coefficients (ee, eqt, zt)
+// per pole p does, around its E update:
+//   A: phi = sum_p ee_p T_p;  T_p = eqt_p T_p + zt_p E^n;
+//      E^{n+1} = CA E^n + CB curl H^{n+1} - IEA phi  (+ PML, source)
+//   B: T_p = T_p - zt_p E^{n+1}
+// so T^{n+1}_p = eqt_p T^n_p + zt_p (E^n - E^{n+1}).  Its reverse, with lT_p
+// = dL/dT^{n+1}_p: the adjoint of E^{n+1} seen by the E update is
+// lE* = lE - sum_p zt_p lT_p (k_adj_disp, before the gathers, which then
+// use lE*); then lE^n = CA lE* + sum_p zt_p lT_p, lT^n_p = eqt_p lT_p -
+// IEA ee_p lE*, and dL/dIEA gains -lE* phi^n (phi^n saved by the recompute).
 //
 // Iteration n maps the state s^n = (E^n, H^n, PhiE^n, PhiH^n) to s^{n+1}
 // linearly.  Its adjoint, applied from n = N-1 down to 0 to the adjoint state
@@ -43,6 +56,7 @@
 namespace {
 
 constexpr int INV_MAX_SLABS = 6;
+constexpr int INV_MAX_POLES = 4;
 
 // One PML slab of the inversion grid (order 1).  Boxes are the corrected
 // cells of each field; the profile index along the normal is p = pend - c
@@ -79,6 +93,11 @@ struct InvArgs {
     float sfac;                          // dl / (dx dy dz)
     const float *waves;                  // [iterations]
     int it, iterations;
+    long long ncell;
+    int P;                               // Debye poles (0: none)
+    const float *dc;                     // [3][P][ncell]: ee, eqt, zt of every cell and pole
+    float *T[3];                         // forward: T of E component c, [P][ntot]; adjoint: lT
+    float *Ps[3];                        // forward: where to save phi^n (or null); adjoint: phi^n
     int nrx;
     const int *rxc;                      // [nrx][3]
     float *traces;                       // [nrx][iterations]
@@ -243,12 +262,36 @@ __global__ void k_inv_e(InvArgs a)
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
         const int oc = cell ? cidx(a, i, j, k) : 0;
         const float CA = cell ? a.CA[oc] : 0.f, CB = cell ? a.CB[oc] : 0.f, IEA = cell ? a.IEA[oc] : 0.f;
-        if (e_upd(a, 0, i, j, k))
+        const bool upd[3] = {e_upd(a, 0, i, j, k), e_upd(a, 1, i, j, k), e_upd(a, 2, i, j, k)};
+        // dispersive part A: phi from T^n, then T advanced with E^n
+        float phi[3] = {0.f, 0.f, 0.f}, tv[3][INV_MAX_POLES];
+        const bool disp = a.P > 0 && cell;
+        if (disp) {
+            for (int c = 0; c < 3; ++c) {
+                if (!upd[c]) continue;
+                float ph = 0.f;
+                for (int p = 0; p < a.P; ++p) {
+                    const float ee = a.dc[(long long)p * a.ncell + oc];
+                    const float eq = a.dc[(long long)(a.P + p) * a.ncell + oc];
+                    const float zt = a.dc[(long long)(2 * a.P + p) * a.ncell + oc];
+                    const float t = a.T[c][(long long)p * a.ntot + o];
+                    ph = ph + ee * t;
+                    tv[c][p] = eq * t + zt * e[c];
+                }
+                phi[c] = ph;
+            }
+            if (a.Ps[0])
+                for (int c = 0; c < 3; ++c) a.Ps[c][o] = phi[c];
+        }
+        if (upd[0])
             e[0] = CA * e[0] + CB * (a.H[2][o] - a.H[2][om[1]]) - CB * (a.H[1][o] - a.H[1][om[2]]);
-        if (e_upd(a, 1, i, j, k))
+        if (upd[1])
             e[1] = CA * e[1] + CB * (a.H[0][o] - a.H[0][om[2]]) - CB * (a.H[2][o] - a.H[2][om[0]]);
-        if (e_upd(a, 2, i, j, k))
+        if (upd[2])
             e[2] = CA * e[2] + CB * (a.H[1][o] - a.H[1][om[0]]) - CB * (a.H[0][o] - a.H[0][om[1]]);
+        if (disp)
+            for (int c = 0; c < 3; ++c)
+                if (upd[c]) e[c] = e[c] - IEA * phi[c];
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
             if (!in_e(S, i, j, k)) continue;
@@ -270,6 +313,14 @@ __global__ void k_inv_e(InvArgs a)
         }
         if (i == a.si && j == a.sj && k == a.sk)
             e[2] = e[2] - IEA * a.waves[a.it] * a.sfac;
+        if (disp)  // dispersive part B
+            for (int c = 0; c < 3; ++c) {
+                if (!upd[c]) continue;
+                for (int p = 0; p < a.P; ++p) {
+                    const float zt = a.dc[(long long)(2 * a.P + p) * a.ncell + oc];
+                    a.T[c][(long long)p * a.ntot + o] = tv[c][p] - zt * e[c];
+                }
+            }
         a.E[0][o] = e[0];
         a.E[1][o] = e[1];
         a.E[2][o] = e[2];
@@ -308,6 +359,44 @@ __device__ __forceinline__ float h_pml_ldd(const InvArgs &a, const InvSlab &S, i
     const float lF = a.H[comp][fidx(a, i, j, k)];
     const float mu = (w == 0 ? S.PH1 : S.PH2)[q[w]];
     return (float)sign * a.cH * RA01 * lF - RF0 * mu;
+}
+
+// Reverse of the dispersive parts of updated E edge (c, o) of cell oc, given
+// lE* = dL/dE^{n+1} with the B part folded in (k_adj_disp): lE^n gains
+// sum_p zt_p lT_p, lT_p becomes eqt_p lT_p - IEA ee_p lE*, and dL/dIEA
+// gains -lE* phi^n.
+__device__ __forceinline__ void disp_reverse(const InvArgs &a, int c, int o, int oc, float IEA, float lEs,
+                                             float &out, float &giea)
+{
+    float add = 0.f;
+    for (int p = 0; p < a.P; ++p) {
+        const float ee = a.dc[(long long)p * a.ncell + oc];
+        const float eq = a.dc[(long long)(a.P + p) * a.ncell + oc];
+        const float zt = a.dc[(long long)(2 * a.P + p) * a.ncell + oc];
+        float *lt = a.T[c] + (long long)p * a.ntot + o;
+        const float l = *lt;
+        add = add + zt * l;
+        *lt = eq * l - IEA * ee * lEs;
+    }
+    out = out + add;
+    giea += -lEs * a.Ps[c][o];
+}
+
+// lE* = lE - sum_p zt_p lT_p on the updated E edges of cells with poles: the
+// adjoint of E^{n+1} as the E update sees it (B subtracts zt_p E^{n+1} from
+// T^{n+1}).  Runs before the E gathers of the same iteration.
+__global__ void k_adj_disp(InvArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if (k >= a.nz || j >= a.ny || i >= a.nx) return;
+    const int o = fidx(a, i, j, k), oc = cidx(a, i, j, k);
+    for (int c = 0; c < 3; ++c) {
+        if (!e_upd(a, c, i, j, k)) continue;
+        float l = a.E[c][o];
+        for (int p = 0; p < a.P; ++p)
+            l = l - a.dc[(long long)(2 * a.P + p) * a.ncell + oc] * a.T[c][(long long)p * a.ntot + o];
+        a.E[c][o] = l;
+    }
 }
 
 // CB of the cell of E edge (i, j, k) times lE_c there, when that edge is updated.
@@ -375,10 +464,17 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             }
             a.gCA[oc] += gca;
             a.gCB[oc] += gcb;
-            if (i == a.si && j == a.sj && k == a.sk) a.gIEA[oc] += -lE[2] * a.waves[a.it] * a.sfac;
-            a.En[0][o] = CA * lE[0];
-            a.En[1][o] = CA * lE[1];
-            a.En[2][o] = CA * lE[2];
+            float giea = 0.f;
+            if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
+            float out[3] = {CA * lE[0], CA * lE[1], CA * lE[2]};
+            if (a.P > 0) {
+                const float IEA = a.IEA[oc];
+                for (int c = 0; c < 3; ++c) disp_reverse(a, c, o, oc, IEA, lE[c], out[c], giea);
+            }
+            if (giea != 0.f) a.gIEA[oc] += giea;
+            a.En[0][o] = out[0];
+            a.En[1][o] = out[1];
+            a.En[2][o] = out[2];
             return;
         }
         // Ex(i,j,k) = ... + CB (Hz(i,j,k) - Hz(i,j-1,k)) - CB (Hy(i,j,k) - Hy(i,j,k-1))
@@ -438,6 +534,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                 gca += lE[c] * a.Es[c][o];
                 gcb += lE[c] * curl[c];
                 out[c] = CA * lE[c];
+                if (a.P > 0) disp_reverse(a, c, o, oc, IEA, lE[c], out[c], giea);
             }
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
             a.gCA[oc] += gca;
@@ -600,6 +697,10 @@ struct gpx_inv {
     float *par = nullptr;          // CA, CB, IEA: 3 x ncell
     float *grad = nullptr;         // gCA, gCB, gIEA: 3 x ncell
     float *waves = nullptr;
+    int P = 0;                     // Debye poles
+    float *dc = nullptr;           // ee, eqt, zt: 3 x P x ncell
+    float *T = nullptr;            // T of Ex, Ey, Ez: 3 x P x ntot
+    float *LT = nullptr;           // its adjoint
     int *rxc = nullptr;
     float *traces = nullptr, *resid = nullptr;
     int nrx = 0, si = -1, sj = -1, sk = -1;
@@ -718,7 +819,8 @@ extern "C" int gpx_inv_destroy(gpx_inv *v)
     if (!v) return 0;
     hipSetDevice(v->desc.device);
     if (v->stream) hipStreamSynchronize(v->stream);
-    for (float *p : {v->F, v->L, v->L2, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid})
+    for (float *p : {v->F, v->L, v->L2, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid, v->dc,
+                     v->T, v->LT})
         if (p) hipFree(p);
     if (v->rxc) hipFree(v->rxc);
     for (auto &S : v->slabs) {
@@ -827,8 +929,38 @@ extern "C" int gpx_inv_set_model(gpx_inv *v, const float *CA, const float *CB, c
     return 0;
 }
 
+extern "C" int gpx_inv_set_dispersive(gpx_inv *v, int P, const float *ee, const float *eqt, const float *zt)
+{
+    if (!v || P < 0 || P > INV_MAX_POLES || (P > 0 && (!ee || !eqt || !zt)))
+        return ifail(v, GPX_EARG, "gpx_inv_set_dispersive: bad argument (at most 4 poles)");
+    hipSetDevice(v->desc.device);
+    if (P != v->P) {
+        for (float **p : {&v->dc, &v->T, &v->LT, &v->ck, &v->seg}) {
+            if (*p) hipFree(*p);
+            *p = nullptr;
+        }
+        v->P = P;
+        // the segment store keeps phi^n of the three E components beside E^n, H^{n+1}
+        INV_TRY(hipMalloc((void **)&v->seg, (size_t)v->desc.K * (P ? 9 : 6) * v->ntot * 4));
+        if (P) {
+            INV_TRY(hipMalloc((void **)&v->dc, (size_t)3 * P * v->ncell * 4));
+            INV_TRY(hipMalloc((void **)&v->T, (size_t)3 * P * v->ntot * 4));
+            INV_TRY(hipMalloc((void **)&v->LT, (size_t)3 * P * v->ntot * 4));
+        }
+    }
+    const size_t b = (size_t)P * v->ncell * 4;
+    if (P) {
+        INV_TRY(hipMemcpy(v->dc, ee, b, hipMemcpyHostToDevice));
+        INV_TRY(hipMemcpy(v->dc + (size_t)P * v->ncell, eqt, b, hipMemcpyHostToDevice));
+        INV_TRY(hipMemcpy(v->dc + (size_t)2 * P * v->ncell, zt, b, hipMemcpyHostToDevice));
+    }
+    v->have_forward = false;
+    return 0;
+}
+
 static int n_checkpoints(const gpx_inv *v) { return (v->desc.iterations + v->desc.K - 1) / v->desc.K; }
-static long long ck_stride(const gpx_inv *v) { return 6 * v->ntot + v->phi_total; }
+static long long ck_stride(const gpx_inv *v) { return 6 * v->ntot + 3 * v->P * v->ntot + v->phi_total; }
+static long long seg_stride(const gpx_inv *v) { return (v->P ? 9 : 6) * v->ntot; }
 
 static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
 {
@@ -870,6 +1002,10 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     a.rxc = v->rxc;
     a.traces = v->traces;
     a.resid = v->resid;
+    a.ncell = v->ncell;
+    a.P = v->P;
+    a.dc = v->dc;
+    for (int c = 0; c < 3 && v->P; ++c) a.T[c] = (adjoint ? v->LT : v->T) + (size_t)c * v->P * v->ntot;
     memcpy(a.fast, v->fast, sizeof(a.fast));
     memcpy(a.shell, v->shell, sizeof(a.shell));
     return a;
@@ -904,6 +1040,10 @@ static int checkpoint(gpx_inv *v, int c, bool save)
     };
     INV_TRY(cp(dst, v->F, 6 * v->ntot));
     long long off = 6 * v->ntot;
+    if (v->P) {
+        INV_TRY(cp(dst + off, v->T, 3 * v->P * v->ntot));
+        off += 3 * v->P * v->ntot;
+    }
     for (auto &S : v->slabs)
         for (int q = 0; q < 4; ++q) {
             INV_TRY(cp(dst + off, S.phi[q], S.n[q]));
@@ -951,6 +1091,7 @@ extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, c
     if (nrx) INV_TRY(hipMemcpy(v->rxc, rx, (size_t)nrx * 3 * 4, hipMemcpyHostToDevice));
     if (!v->ck) INV_TRY(hipMalloc((void **)&v->ck, (size_t)n_checkpoints(v) * ck_stride(v) * 4));
     INV_TRY(hipMemsetAsync(v->F, 0, 6 * v->ntot * 4, v->stream));
+    if (v->P) INV_TRY(hipMemsetAsync(v->T, 0, (size_t)3 * v->P * v->ntot * 4, v->stream));
     for (auto &S : v->slabs)
         for (int q = 0; q < 4; ++q) INV_TRY(hipMemsetAsync(S.phi[q], 0, S.n[q] * 4, v->stream));
     hipEvent_t e0, e1;
@@ -983,6 +1124,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     INV_TRY(hipMemsetAsync(v->L, 0, 6 * v->ntot * 4, v->stream));
     INV_TRY(hipMemsetAsync(v->L2, 0, 3 * v->ntot * 4, v->stream));
     INV_TRY(hipMemsetAsync(v->grad, 0, 3 * v->ncell * 4, v->stream));
+    if (v->P) INV_TRY(hipMemsetAsync(v->LT, 0, (size_t)3 * v->P * v->ntot * 4, v->stream));
     for (auto &S : v->slabs)
         for (int q = 0; q < 4; ++q) {
             INV_TRY(hipMemsetAsync(S.mu[q], 0, S.n[q] * 4, v->stream));
@@ -1001,25 +1143,28 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
         // recompute the segment, storing E^n and H^{n+1}
         if (int rc = checkpoint(v, c, false)) return rc;
         for (int it = n0; it < n1; ++it) {
-            float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
+            float *slot = v->seg + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, false, it);
-            for (int q = 0; q < 3; ++q) {  // the kernels save E^n and H^{n+1} here
+            for (int q = 0; q < 3; ++q) {  // the kernels save E^n, H^{n+1} (and phi^n) here
                 a.Es[q] = slot + q * v->ntot;
                 a.Hs[q] = slot + (3 + q) * v->ntot;
+                if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
             }
             hipLaunchKernelGGL(k_inv_h, G, dim3(128), 0, v->stream, a);
             hipLaunchKernelGGL(k_inv_e, G, dim3(128), 0, v->stream, a);
         }
         // adjoint sweep over the segment
         for (int it = n1 - 1; it >= n0; --it) {
-            float *slot = v->seg + (long long)(it - n0) * 6 * v->ntot;
+            float *slot = v->seg + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, true, it);
             for (int q = 0; q < 3; ++q) {
                 a.Es[q] = slot + q * v->ntot;
                 a.Hs[q] = slot + (3 + q) * v->ntot;
+                if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
                 a.E[q] = lE[cur] + q * v->ntot;
                 a.En[q] = lE[cur ^ 1] + q * v->ntot;
             }
+            if (v->P) hipLaunchKernelGGL(k_adj_disp, G, dim3(128), 0, v->stream, a);
             for (int s = 0; s < a.nslab; ++s) {
                 const auto &S = v->slabs[s];
                 float *const *P = cur == 0 ? S.mu : S.mu2;

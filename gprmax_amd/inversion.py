@@ -6,7 +6,8 @@ The forward model is the reference's solver loop (gprMax/model_build_run.py:
 without averaging: E edge (c, i, j, k) takes the coefficients of cell
 (i, j, k) (gprMax/materials.py:178-201), mu_r = 1, no magnetic loss, a
 10-cell (by default) HORIPML with one CFS term on every face, z-directed
-Hertzian dipoles and Ez receivers.  ``gpx_inv_backward`` returns dL/dCA,
+Hertzian dipoles and Ez receivers, optionally with fixed Debye poles per
+cell (``set_debye``).  ``gpx_inv_backward`` returns dL/dCA,
 dL/dCB and dL/dIEA per cell; this module applies the chain rule to
 dL/d(eps_r) and dL/d(sigma).
 
@@ -43,6 +44,7 @@ SIGNATURES = {
     'gpx_inv_last_error': ([_P], ctypes.c_char_p),
     'gpx_inv_add_pml': ([_P] + [_I] * 7 + [_P, _P], _I),
     'gpx_inv_set_model': ([_P, _P, _P, _P], _I),
+    'gpx_inv_set_dispersive': ([_P, _I, _P, _P, _P], _I),
     'gpx_inv_forward': ([_P, _I, _I, _I, _F, _P, _I, _P, _P], _I),
     'gpx_inv_backward': ([_P, _P, _P, _P, _P], _I),
     'gpx_inv_timings': ([_P, ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
@@ -97,6 +99,31 @@ class InversionGeometry:
         else:
             m[...] = True
         self.interior = m   # cells whose every edge lies outside the PML slabs
+        self.debye = None
+
+    def set_debye(self, deltaer, tau):
+        """Debye poles per cell, fixed during the inversion: deltaer and tau
+        [P][nx][ny][nz] (deltaer 0 where a cell has fewer poles).  Per pole
+        and cell, float64, the coefficients of gprMax/materials.py:92-143:
+        w = deltaer / tau, q = -1 / tau, eqt = exp(q dt), eqt2 = exp(q dt / 2),
+        zt = (w / q)(1 - eqt) / dt, zt2 = (w / q)(1 - eqt2); the update uses
+        ee = e0 eqt2, eqt and zt, and EA gains -(e0 / dt) sum zt2 (Z).
+        None removes the poles."""
+        if deltaer is None:
+            self.debye = None
+            return
+        de = np.asarray(deltaer, dtype=np.float64)
+        tau = np.asarray(tau, dtype=np.float64)
+        if de.ndim != 4 or de.shape != tau.shape or de.shape[1:] != (self.nx, self.ny, self.nz):
+            raise GeneralError('Debye parameters must be [P][nx][ny][nz]')
+        if de.shape[0] > 4 or np.any(tau <= 0):
+            raise GeneralError('at most 4 Debye poles, with positive relaxation times')
+        dt = self.dt
+        w, q = de / tau, -1.0 / tau
+        eqt, eqt2 = np.exp(q * dt), np.exp(q * (dt / 2))
+        zt = (w / q) * (1 - eqt) / dt
+        zt2 = (w / q) * (1 - eqt2)
+        self.debye = {'ee': e0 * eqt2, 'eqt': eqt, 'zt': zt, 'Z': (e0 / dt) * zt2.sum(axis=0)}
 
     def waveform(self, freq=900e6, kind='ricker', amp=1.0):
         w = Waveform()
@@ -111,11 +138,14 @@ class InversionGeometry:
         a = e0 / self.dt
         EA = a * er + 0.5 * se
         EB = a * er - 0.5 * se
+        if self.debye is not None:
+            EA = EA - self.debye['Z']
+            EB = EB - self.debye['Z']
         CA, IEA = EB / EA, 1.0 / EA
         CB = (1.0 / self.d) * IEA
         EA2 = EA * EA
-        der = {'CA': a * se / EA2, 'IEA': -a / EA2}
-        dse = {'CA': -a * er / EA2, 'IEA': -0.5 / EA2}
+        der = {'CA': a * (EA - EB) / EA2, 'IEA': -a / EA2}
+        dse = {'CA': -0.5 * (EA + EB) / EA2, 'IEA': -0.5 / EA2}
         der['CB'] = der['IEA'] / self.d
         dse['CB'] = dse['IEA'] / self.d
         f32 = [np.ascontiguousarray(x, dtype=np.float32) for x in (CA, CB, IEA)]
@@ -162,6 +192,14 @@ class InversionGrid(InversionGeometry):
             pass
 
     # -- solves --------------------------------------------------------------
+    def set_debye(self, deltaer, tau):
+        super().set_debye(deltaer, tau)
+        if self.debye is None:
+            self._check(self.lib.gpx_inv_set_dispersive(self.v, 0, None, None, None))
+            return
+        f32 = [np.ascontiguousarray(self.debye[k], dtype=np.float32) for k in ('ee', 'eqt', 'zt')]
+        self._check(self.lib.gpx_inv_set_dispersive(self.v, f32[0].shape[0], *(_ptr(x) for x in f32)))
+
     def set_model(self, er, se):
         (CA, CB, IEA), der, dse = self.coefficients(er, se)
         self._check(self.lib.gpx_inv_set_model(self.v, _ptr(CA), _ptr(CB), _ptr(IEA)))

@@ -284,6 +284,14 @@ int gpx_inv_add_pml(gpx_inv *v, int direction, int xs, int xf, int ys, int yf, i
                     const float *eR, const float *hR);
 /* Per-cell coefficients, float32 [nx][ny][nz] each. */
 int gpx_inv_set_model(gpx_inv *v, const float *CA, const float *CB, const float *IEA);
+/* Debye poles per cell with real auxiliary values, the reference's dispersive
+ * parts A and B (gprMax/fields_updates_ext.pyx:113-235) on every updated E
+ * edge: P poles (0 removes them, at most 4), and per pole float32
+ * [P][nx][ny][nz] the coefficients ee = e0 Re(eqt2), eqt and zt of
+ * gprMax/materials.py:92-143 (zeros in cells without poles).  The poles are
+ * fixed: the gradient stays with respect to CA, CB and IEA, whose EA the
+ * caller forms with the -(e0/dt) sum Re(zt2) term of materials.py:127-128. */
+int gpx_inv_set_dispersive(gpx_inv *v, int P, const float *ee, const float *eqt, const float *zt);
 /* All iterations from the zero state with a z-directed Hertzian dipole on
  * the Ez edge (si, sj, sk), Ez -= IEA * waves[n] * sfac; receivers rx int32
  * [nrx][3]; traces float32 [nrx][iterations] (Ez at the start of each

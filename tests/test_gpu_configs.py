@@ -226,12 +226,15 @@ def _adjoint_setup(shape, iterations, K, seed):
     return rng, geo, waves
 
 
-def test_config5_adjoint_40_cube_matches_float64_autograd():
+@pytest.mark.parametrize('poles', [0, 3])
+def test_config5_adjoint_40_cube_matches_float64_autograd(poles):
     """40^3 cells of 2 mm, 10-cell PML, random true model (er in [4, 10],
     sigma in [1e-3, 2e-2]) and a perturbed homogeneous start model, two
     sources and six receivers, 150 iterations with K = 64 (two whole segments
-    and a partial one): the GPU gradient equals the float64 PyTorch autograd
-    gradient within 1e-3 (relative L2 over the cells outside the PML)."""
+    and a partial one), with no poles or with 3 Debye poles (deltaer ~
+    U[0.5, 5], tau = 10^U[-11, -9] s) on 40 % of the cells: the GPU gradient
+    equals the float64 PyTorch autograd gradient within 1e-3 (relative L2
+    over the cells outside the PML)."""
     import torch
     from gprmax_amd.inversion import InversionGrid, misfit_and_gradient
     from torch_fdtd import forward_traces, misfit
@@ -241,6 +244,11 @@ def test_config5_adjoint_40_cube_matches_float64_autograd():
     se_t = 1e-3 + 1.9e-2 * rng.random(shape)
     er0 = 7.0 + 0.3 * rng.standard_normal(shape)
     se0 = 1.05e-2 + 1e-3 * rng.standard_normal(shape)
+    if poles:
+        mask = rng.random(shape) < 0.4
+        de = rng.uniform(0.5, 5, (poles,) + shape) * mask
+        tau = 10 ** rng.uniform(-11, -9, (poles,) + shape)
+        geo.set_debye(de, tau)
     src = [(15, 20, 25), (24, 18, 25)]
     rx = [(12 + 4 * n, 14 + 2 * n, 25) for n in range(6)]
     with torch.no_grad():
@@ -251,6 +259,8 @@ def test_config5_adjoint_40_cube_matches_float64_autograd():
     L_ref.backward()
     grid = InversionGrid(shape, 2e-3, geo.iterations, pml=10, er_pml=7.0, K=64)
     try:
+        if poles:
+            grid.set_debye(de, tau)
         L, ger, gse = misfit_and_gradient(grid, er0, se0, src, rx, obs, waves)
     finally:
         grid.close()
@@ -258,7 +268,7 @@ def test_config5_adjoint_40_cube_matches_float64_autograd():
     assert abs(L - L_ref.item()) <= 1e-4 * L_ref.item()
     for ours, ref in ((ger, er.grad.numpy()), (gse, se.grad.numpy())):
         err = np.linalg.norm(ours[m] - ref[m]) / np.linalg.norm(ref[m])
-        print('config 5 (40^3): relative L2 gradient error', err, flush=True)
+        print('config 5 (40^3, {} poles): relative L2 gradient error'.format(poles), err, flush=True)
         assert err < 1e-3, err
 
 

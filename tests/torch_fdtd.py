@@ -17,12 +17,25 @@ _E_PAIRS = {0: ((1, 2, -1), (2, 1, +1)), 1: ((0, 2, +1), (2, 0, -1)), 2: ((0, 1,
 _H_PAIRS = {0: ((1, 2, +1), (2, 1, -1)), 1: ((0, 2, -1), (2, 0, +1)), 2: ((0, 1, +1), (1, 0, -1))}
 
 
-def coefficients(er, se, dt, d):
+def coefficients(er, se, dt, d, Z=None):
+    """CA, CB, IEA of gprMax/materials.py:120-143; Z = (e0/dt) sum Re(zt2)
+    per cell for cells with Debye poles."""
     a = e0 / dt
     EA = a * er + 0.5 * se
     EB = a * er - 0.5 * se
+    if Z is not None:
+        EA = EA - Z
+        EB = EB - Z
     IEA = 1.0 / EA
     return EB / EA, (1.0 / d) * IEA, IEA
+
+
+# Update ranges of Ex, Ey, Ez (gprMax/fields_updates_ext.pyx:81-107); an
+# edge takes the coefficients of the cell of the same index.
+def _e_ranges(nx, ny, nz):
+    return [(slice(0, nx), slice(1, ny), slice(1, nz)),
+            (slice(1, nx), slice(0, ny), slice(1, nz)),
+            (slice(1, nx), slice(1, ny), slice(0, nz))]
 
 
 def _slab_geometry(pml, field):
@@ -54,7 +67,16 @@ def forward_traces(er, se, grid, source, receivers, waves):
     er, se: float64 tensors (nx, ny, nz)."""
     nx, ny, nz, d, dt = grid.nx, grid.ny, grid.nz, grid.d, grid.dt
     f64 = torch.float64
-    CA, CB, IEA = coefficients(er, se, dt, d)
+    deb = getattr(grid, 'debye', None)
+    if deb is not None:
+        Zt = torch.tensor(deb['Z'])
+        dcoef = {k: torch.tensor(deb[k]) for k in ('ee', 'eqt', 'zt')}
+        P = deb['ee'].shape[0]
+    CA, CB, IEA = coefficients(er, se, dt, d, Zt if deb is not None else None)
+    rng = _e_ranges(nx, ny, nz)
+    if deb is not None:   # T of each E component on its update range, per pole
+        shapes = [(nx, ny - 1, nz - 1), (nx - 1, ny, nz - 1), (nx - 1, ny - 1, nz)]
+        T = [[torch.zeros(sh, dtype=f64) for _ in range(P)] for sh in shapes]
     DB = (1.0 / d) * (dt / m0)
     cH = dt / m0
     shp = (nx + 1, ny + 1, nz + 1)
@@ -111,6 +133,20 @@ def forward_traces(er, se, grid, source, receivers, waves):
                                  + CB[1:nx, 1:ny, 0:nz] * (Hy[1:nx, 1:ny, 0:nz] - Hy[0:nx - 1, 1:ny, 0:nz])
                                  - CB[1:nx, 1:ny, 0:nz] * (Hx[1:nx, 1:ny, 0:nz] - Hx[1:nx, 0:ny - 1, 0:nz]))
         nE = [nEx, nEy, nEz]
+        if deb is not None:
+            # dispersive part A (gprMax/fields_updates_ext.pyx:113-179):
+            # phi from T^n, T advanced with E^n, E reduced by IEA phi
+            Tn = []
+            for c, rc in enumerate(rng):
+                phi = 0.0
+                Tc = []
+                for p in range(P):
+                    phi = phi + dcoef['ee'][p][rc] * T[c][p]
+                    Tc.append(dcoef['eqt'][p][rc] * T[c][p] + dcoef['zt'][p][rc] * E[c][rc])
+                Tn.append(Tc)
+                upd = nE[c].clone()
+                upd[rc] = nE[c][rc] - IEA[rc] * phi
+                nE[c] = upd
         for (ge, gh, R, phiE, phiH) in geo:
             axis, cells, nbrs, shape, n = ge
             c = IEA[cells]
@@ -125,6 +161,8 @@ def forward_traces(er, se, grid, source, receivers, waves):
         ez = nE[2].clone()
         ez[si, sj, sk] = nE[2][si, sj, sk] - IEA[si, sj, sk] * float(waves[n_it]) * sfac
         nE[2] = ez
+        if deb is not None:   # dispersive part B (fields_updates_ext.pyx:182-235)
+            T = [[Tn[c][p] - dcoef['zt'][p][rc] * nE[c][rc] for p in range(P)] for c, rc in enumerate(rng)]
         E = nE
         H = [Hx, Hy, Hz]
     return torch.stack(traces, dim=1)
