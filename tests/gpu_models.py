@@ -9,20 +9,33 @@ from gprmax_amd.materials import add_dispersion
 from gprmax_amd.pml import CFS
 
 
-def randomise_tables(G, seed):
+def randomise_tables(G, seed, scaled=False):
     """Replace the coefficient tables (every column of updatecoeffsE/H drawn
     on its own) and every slab's R tables with random values, so that a
     swapped coefficient column (CBx/CBy/CBz, DBx/DBy/DBz, srce/srcm) or a
     profile read at the wrong index changes the results.  The model keeps
-    its geometry, sources and dispersive tables."""
+    its geometry, sources and dispersive tables.
+
+    scaled=False draws every entry from U[-1, 1] (R from U[0.5, 1.5]), for
+    single sub-steps.  scaled=True multiplies every physical entry by its own
+    factor from U[0.5, 1.5] (R entries by U[0.9, 1.1]), which keeps the
+    magnitudes (the PML divides by cell sizes of a millimetre) so that runs
+    of several iterations stay finite."""
     rng = np.random.default_rng(seed)
     n = len(G.updatecoeffsE)
-    G.updatecoeffsE = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
-    G.updatecoeffsH = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    if scaled:
+        G.updatecoeffsE = (G.updatecoeffsE * rng.uniform(0.5, 1.5, (n, 5))).astype(np.float32)
+        G.updatecoeffsH = (G.updatecoeffsH * rng.uniform(0.5, 1.5, (n, 5))).astype(np.float32)
+    else:
+        G.updatecoeffsE = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+        G.updatecoeffsH = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
     for p in G.pmls:
         for name in ('ERA', 'ERB', 'ERE', 'ERF', 'HRA', 'HRB', 'HRE', 'HRF'):
             a = getattr(p, name)
-            setattr(p, name, rng.uniform(0.5, 1.5, a.shape).astype(a.dtype))
+            if scaled:
+                setattr(p, name, (a * rng.uniform(0.9, 1.1, a.shape)).astype(a.dtype))
+            else:
+                setattr(p, name, rng.uniform(0.5, 1.5, a.shape).astype(a.dtype))
     return G
 
 

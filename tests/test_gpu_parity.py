@@ -250,12 +250,14 @@ def test_split_step_from_random_state(case, xchunk, form):
 @pytest.mark.parametrize('case', list(CASES))
 def test_run_from_random_state_anisotropic_random_tables(case, fused):
     """Every step form from a random state (fields, Phi, T) on 1 x 1.5 x 2 mm
-    cells with random coefficient and R tables, on a grid with several blocks
-    per plane: state and receiver samples equal to the oracle's.  Forms the
-    model cannot take run the passes, as gpx_run does."""
+    cells with every coefficient and R table entry scaled by its own random
+    factor, on a grid with several blocks per plane: state and receiver
+    samples equal to the oracle's (and finite).  Forms the model cannot take
+    run the passes, as gpx_run does."""
     G = randomise_tables(rich_model(seed=27, shape=WIDE, iterations=12, snapshots=False, cell=ANISO,
-                                    **CASES[case]), 29)
+                                    **CASES[case]), 29, scaled=True)
     _random_state(G, 31)
+    assert all(np.isfinite(getattr(G, c)).all() for c in COMPS)
     lib = oracle.load(np.float32)
     with _hip()(G) as s:
         s.set_step(fused)
@@ -264,6 +266,7 @@ def test_run_from_random_state_anisotropic_random_tables(case, fused):
         s.run(2, 8)
         for it in range(2, 10):
             osolver.step(lib, G, it, 8)
+        assert all(np.isfinite(getattr(G, c)).all() for c in COMPS), 'the oracle run overflowed'
         _assert_state_equal(s, G, 'form {} from a random state'.format(fused))
         buf = s.download_rx()
     for r, rx in enumerate(G.rxs):
