@@ -273,7 +273,8 @@ def step_form_label(fused, form, choice, default_form):
     """The step form that ran, and how it was chosen."""
     name = ('fused, {} cells per block'.format(fused) if fused else
             {3: 'split', 5: 'split, H boundary beside the interior kernel',
-             2: 'two-pass side by side'}.get(form, 'two-pass'))
+             2: 'two-pass side by side', 6: 'two-pass merged, one launch per pass',
+             7: 'split, H boundary in the interior launch'}.get(form, 'two-pass'))
     if not default_form:
         return name + ' (set on the command line)'
     if choice == -1:
@@ -310,10 +311,14 @@ def main():
     ap.add_argument('--fused', action='store_true', help='one fused kernel per step instead of the H and E passes')
     ap.add_argument('--overlap', action='store_true', help='H and E passes with interior and boundary kernels side by side')
     ap.add_argument('--passes', action='store_true', help='the H and E passes (step form 0)')
+    ap.add_argument('--merged', action='store_true',
+                    help='the passes as one launch each over every cell (form 6)')
+    ap.add_argument('--split-merged', action='store_true',
+                    help='the split step with the H boundary pass inside the interior launch (form 7)')
     ap.add_argument('--split', action='store_true',
                     help='the split step (form 3): one kernel doing H and E for the interior, boundary passes '
-                         'around it; by default the library times form 2, form 5, the split step and the passes '
-                         '(form 2 and the passes where the split step does not apply) and keeps the fastest')
+                         'around it; by default the library times forms 7, 6, 2, 5, the split step and the passes '
+                         '(forms 6, 2 and the passes where the split step does not apply) and keeps the fastest')
     ap.add_argument('--split-beside', action='store_true',
                     help='the split step with the H boundary pass beside the interior kernel (form 5)')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
@@ -348,18 +353,20 @@ def main():
     if W + K > G.iterations:
         raise SystemExit('warmup + steps ({}) exceed the workload iterations ({})'.format(W + K, G.iterations))
     info = device_info(0)
-    default_form = not (args.fused or args.overlap or args.passes or args.split or args.split_beside)
+    default_form = not (args.fused or args.overlap or args.passes or args.split or args.split_beside or args.merged
+                        or args.split_merged)
     with HipSolver(G) as s:
         s.set_launch(args.xchunk, args.block)
         s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (
-                   3 if args.split else (5 if args.split_beside else 4)))),
+                   3 if args.split else (5 if args.split_beside else (6 if args.merged else (
+                       7 if args.split_merged else 4)))))),
                    args.tile)
         fused = s.step_kind()
         s.run(0, W)
         s.synchronize()
         ms, _, _ = s.run_timed(W, K, per_kernel=False)
         s.synchronize()
-        form = s.step_form()       # after the runs: the default form times four forms on its first long run
+        form = s.step_form()       # after the runs: the default form times six forms on its first long run
         choice = s.step_choice()   # -1: the default form has not timed them (a run shorter than 26 iterations)
         # Per-kernel durations (HIP events around every launch on the solver's
         # stream), in a separate run after the timed region.
@@ -372,12 +379,19 @@ def main():
     mcells = cells * K / (ms * 1e-3) / 1e6
     if fused:
         kernels = {'step': (algorithmic_bytes_per_step(G), ms4[0])}
-    elif form in (3, 5):
+    elif form in (3, 5, 7):
         # the fused box is the interior box less its first plane, row and column
         box3 = (box[0] + 1, box[1], box[2] + 1, box[3], box[4] + 1, box[5])
         bh, be = kernel_bytes(G, 1, box3), kernel_bytes(G, 0, box3)
         kernels = {'interior': (bh[0] + be[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),
                    'e_boundary': (be[1], ms4[3])}
+        if form == 7:
+            # the H boundary pass runs inside the interior kernel's launch
+            kernels = {'interior_and_h_boundary': (bh[0] + be[0] + bh[1], ms4[0]), 'e_boundary': (be[1], ms4[3])}
+    elif form == 6:
+        # one launch per pass over every cell
+        bh, be = kernel_bytes(G, 1, box), kernel_bytes(G, 0, box)
+        kernels = {'h_pass': (bh[0] + bh[1], ms4[0]), 'e_pass': (be[0] + be[1], ms4[2])}
     else:
         bh, be = kernel_bytes(G, 1, box), kernel_bytes(G, 0, box)
         kernels = {'h_interior': (bh[0], ms4[0]), 'h_boundary': (bh[1], ms4[1]),

@@ -78,6 +78,10 @@ struct StepArgs {
     int nseg_a, nseg_b, items_a1, items_a2;
     const int *frame;
     int nframe;
+    // One launch over every cell (k_h_merged, k_e_merged): the chunks of the
+    // interior box's x range (m_nchunk, the interior and the frame items share
+    // the chunk length xc) and the number of items of both kinds (m_items).
+    int m_nchunk, m_items;
 };
 
 
@@ -584,6 +588,76 @@ __global__ void k_e_pass_many(const StepArgs *__restrict__ args, PassMany m)
 }
 
 
+// ---------------------------------------------------------------- merged passes
+// One launch per pass over every cell.  Block order (before the per-XCD
+// remap): the boundary planes below the interior box, then per x chunk of the
+// box its interior items followed by its frame items, then the boundary planes
+// above.  The frame rows' short z runs share their first and last 64-byte
+// sectors with the interior rows next to them; with the items of one chunk
+// adjacent in one launch on one XCD, the second reader of such a sector finds
+// it in L2 instead of fetching it again from HBM, as it does when the frame
+// waits for a kernel of its own.  Returns true for a boundary item and sets
+// `item` to its index in the interior or the boundary numbering.
+__device__ __forceinline__ bool merged_item(const StepArgs &a, int L, int &item)
+{
+    if (L < a.items_a1) {
+        item = L;
+        return true;
+    }
+    L -= a.items_a1;
+    const int per = a.nseg + a.nseg_b;
+    const int mid = a.m_nchunk * per;
+    if (L < mid) {
+        const int c = L / per, r = L - c * per;
+        if (r < a.nseg) {
+            item = c * a.nseg + r;
+            return false;
+        }
+        item = a.items_a1 + a.items_a2 + c * a.nseg_b + (r - a.nseg);
+        return true;
+    }
+    item = a.items_a1 + (L - mid);
+    return true;
+}
+
+// One item per block; the pass bodies' item loop runs once (a.nitems = 2^30).
+template <class IDT>
+__global__ void k_h_merged(StepArgs a)
+{
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.m_items) return;
+    int item;
+    if (merged_item(a, L, item)) h_pass_run<IDT, true>(a, item, 1 << 30, a.it);
+    else h_pass_run<IDT, false>(a, item, 1 << 30, a.it);
+}
+
+template <class IDT, int P, bool CPLX>
+__global__ void k_e_merged(StepArgs a)
+{
+    const int L = logical_block(a.per_xcd);
+    if (L >= a.m_items) return;
+    int item;
+    if (merged_item(a, L, item)) e_pass_run<IDT, P, CPLX, true>(a, item, 1 << 30, a.it);
+    else e_pass_run<IDT, P, CPLX, false>(a, item, 1 << 30, a.it);
+}
+
+
+// Form 7: the split step's interior items (interior_run) and its H boundary
+// items in one launch of INT_BLOCK threads, in merged_item's order.  `hb` is
+// the H boundary pass of the split step (current state in, next state out)
+// with hb.nseg set to the interior's segment count.  Neither kind of item
+// reads what the other writes (form 5 runs the same two on two streams).
+template <class IDT>
+__global__ void __launch_bounds__(INT_BLOCK) k_interior_hb(IntArgs ai, StepArgs hb)
+{
+    const int L = logical_block(hb.per_xcd);
+    if (L >= hb.m_items) return;
+    int item;
+    if (merged_item(hb, L, item)) h_pass_run<IDT, true>(hb, item, 1 << 30, hb.it);
+    else interior_run<IDT>(ai, item);
+}
+
+
 // ---------------------------------------------------------------- snapshots
 // Cell averages of gprMax/snapshots_ext.pyx:58-71 over the slices of
 // gprMax/snapshots.py:78-80; one thread per output cell.
@@ -761,9 +835,12 @@ struct gpx_ctx {
     int nframe3 = 0;                    // the plane offsets outside its rectangle
     int xchunk = 0, block = 0;
     int fused = 4;                   // step form (gpx_set_step): 0 passes, 1 fused step, 2 passes side
-                                     // by side, 3 split step, 4 (default) the fastest of 2, 5, 3 and 0,
-                                     // 5 split step with the H boundary pass beside the interior kernel
-    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step, 2 form 5, 3 form 2
+                                     // by side, 3 split step, 4 (default) the fastest of 7, 6, 2, 5, 3 and 0,
+                                     // 5 split step with the H boundary pass beside the interior kernel,
+                                     // 6 passes as one launch each over every cell, 7 split step with the
+                                     // H boundary pass in the interior kernel's launch
+    int split_choice = -1;           // form 4: -1 not yet timed, 0 passes, 1 split step, 2 form 5, 3 form 2,
+                                     // 4 form 6, 5 form 7
     hipStream_t stream2 = nullptr;   // second stream of forms 2 and 5
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int tile_cells = 0;              // cells per fused block (0 = auto)
@@ -1405,7 +1482,7 @@ extern "C" int gpx_add_snapshot(gpx_ctx *ctx, int xs, int ys, int zs, int dx, in
 
 extern "C" int gpx_set_step(gpx_ctx *ctx, int fused, int tile_cells)
 {
-    if (!ctx || fused < 0 || fused > 5 ||
+    if (!ctx || fused < 0 || fused > 7 ||
         !(tile_cells == 0 || tile_cells == 256 || tile_cells == 512 || tile_cells == 1024))
         return fail(ctx, GPX_EARG, "gpx_set_step: bad argument");
     ctx->fused = fused;
@@ -1776,6 +1853,68 @@ static hipError_t launch_pass_overlap(gpx_ctx *ctx, int field, int flags, int it
     return hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
 }
 
+template <class IDT>
+static pass_kernel e_merged(int P, bool C)
+{
+    switch (P) {
+    case 0: return k_e_merged<IDT, 0, false>;
+    case 1: return C ? k_e_merged<IDT, 1, true> : k_e_merged<IDT, 1, false>;
+    case 2: return C ? k_e_merged<IDT, 2, true> : k_e_merged<IDT, 2, false>;
+    case 3: return C ? k_e_merged<IDT, 3, true> : k_e_merged<IDT, 3, false>;
+    default: return C ? k_e_merged<IDT, 4, true> : k_e_merged<IDT, 4, false>;
+    }
+}
+
+static pass_kernel pick_merged(const gpx_ctx *ctx, int field)
+{
+    if (field == 1) return ctx->id_bytes == 2 ? k_h_merged<uint16_t> : k_h_merged<uint32_t>;
+    const int P = ctx->desc.maxpoles;
+    const bool C = ctx->disp_complex != 0;
+    return ctx->id_bytes == 2 ? e_merged<uint16_t>(P, C) : e_merged<uint32_t>(P, C);
+}
+
+// Items of the merged pass with chunks of xc planes (both kinds), and the
+// map's fields of `a` when given.
+static long long count_merged(const gpx_ctx *ctx, int block, int xc, StepArgs *a)
+{
+    StepArgs t;
+    StepArgs &q = a ? *a : t;
+    const long long ni = count_items(ctx, false, block, xc, &q);
+    const long long nb = count_items(ctx, true, block, xc, &q);
+    const int len = ctx->box[1] - ctx->box[0];
+    q.m_nchunk = len > 0 ? (len + xc - 1) / xc : 0;
+    q.m_items = (int)(ni + nb);
+    q.nitems = 1 << 30;
+    if (ni == 0) q.nseg = 0;  // an empty box: no interior item in any chunk
+    return ni + nb;
+}
+
+// One pass as one launch over every cell (form 6).
+static hipError_t launch_merged(const gpx_ctx *ctx, int field, int flags, int it)
+{
+    StepArgs a;
+    int block;
+    size_t lds;
+    base_args(ctx, field, a, block, lds);
+    const pass_kernel kernel = pick_merged(ctx, field);
+    const int NX1 = ctx->desc.nx + 1;
+    int xc = ctx->xchunk ? ctx->xchunk : 4;
+    if (xc > NX1) xc = NX1;
+    if (!ctx->xchunk) {
+        const int fill = resident_blocks(ctx, (const void *)kernel, block, lds);
+        while (xc > 2 && count_merged(ctx, block, xc, nullptr) < fill) --xc;
+    }
+    a.xc = xc;
+    const long long items = count_merged(ctx, block, xc, &a);
+    if (items <= 0) return hipSuccess;
+    const long long g = (items + 7) / 8 * 8;
+    a.per_xcd = (int)(g / 8);
+    a.flags = flags;
+    a.it = it;
+    hipLaunchKernelGGL(kernel, dim3((unsigned int)g), dim3(block), lds, ctx->stream, a);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- fused step
 constexpr int STEP_LDS_MAX = 64 * 1024;
 
@@ -1973,9 +2112,15 @@ static bool split_eligible(const gpx_ctx *ctx)
 
 static bool split_ok(const gpx_ctx *ctx)
 {
-    if (ctx->fused == 3 || ctx->fused == 5) return split_eligible(ctx);
+    if (ctx->fused == 3 || ctx->fused == 5 || ctx->fused == 7) return split_eligible(ctx);
     const int c = ctx->split_choice;
-    return ctx->fused == 4 && (c == -1 || c == 1 || c == 2) && split_eligible(ctx);
+    return ctx->fused == 4 && (c == -1 || c == 1 || c == 2 || c == 5) && split_eligible(ctx);
+}
+
+// Form 7, or form 4 after it chose form 7 (with split_ok).
+static bool split_merged(const gpx_ctx *ctx)
+{
+    return ctx->fused == 7 || (ctx->fused == 4 && ctx->split_choice == 5);
 }
 
 // Form 5, or form 4 after it chose form 5 (with split_ok).
@@ -1991,30 +2136,23 @@ static bool passes_beside(const gpx_ctx *ctx)
     return ctx->fused == 2 || (ctx->fused == 4 && ctx->split_choice == 3);
 }
 
-static hipError_t launch_interior(const gpx_ctx *ctx, int it)
+// Form 6, or form 4 after it chose form 6 (when neither the fused step nor
+// the split step runs).
+static bool passes_merged(const gpx_ctx *ctx)
 {
-    IntArgs a;
+    return ctx->fused == 6 || (ctx->fused == 4 && ctx->split_choice == 4);
+}
+
+// The interior kernel's arguments except its chunk length and item counts.
+static void interior_args(const gpx_ctx *ctx, int it, IntArgs &a)
+{
     memset(&a, 0, sizeof(a));
     const gpx_desc &g = ctx->desc;
     a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
     a.plane = ctx->plane;
     memcpy(a.box, ctx->box3, sizeof(a.box));
-    const int B = INT_BLOCK;
     const long long cells = (long long)(a.box[3] - a.box[2]) * (a.box[5] - a.box[4]);
-    a.nseg = (int)((cells + B - 1) / B);
-    const int nxb = a.box[1] - a.box[0];
-    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior<uint16_t> : (const void *)k_interior<uint32_t>;
-    const size_t lds = (size_t)g.nmat * 40;
-    int xc = ctx->xchunk ? ctx->xchunk : 4;
-    if (!ctx->xchunk) {
-        const int fill = resident_blocks(ctx, k, B, lds);
-        while (xc > 2 && (long long)a.nseg * ((nxb + xc - 1) / xc) < fill) --xc;
-    }
-    if (xc > nxb) xc = nxb;
-    a.xc = xc;
-    a.nitems = a.nseg * ((nxb + xc - 1) / xc);
-    const int grid = (a.nitems + 7) / 8 * 8;
-    a.per_xcd = grid / 8;
+    a.nseg = (int)((cells + INT_BLOCK - 1) / INT_BLOCK);
     a.Fc = ctx->F[0];
     a.Fx = ctx->Fn[0];
     a.cs = (uint32_t)(ctx->fstride * 4);
@@ -2033,8 +2171,70 @@ static hipError_t launch_interior(const gpx_ctx *ctx, int it)
     a.rxc = ctx->rxc;
     a.rxbuf = ctx->rxbuf;
     a.planeflag = ctx->planeflag;
+}
+
+static hipError_t launch_interior(const gpx_ctx *ctx, int it)
+{
+    IntArgs a;
+    interior_args(ctx, it, a);
+    const int B = INT_BLOCK;
+    const int nxb = a.box[1] - a.box[0];
+    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior<uint16_t> : (const void *)k_interior<uint32_t>;
+    const size_t lds = (size_t)ctx->desc.nmat * 40;
+    int xc = ctx->xchunk ? ctx->xchunk : 4;
+    if (!ctx->xchunk) {
+        const int fill = resident_blocks(ctx, k, B, lds);
+        while (xc > 2 && (long long)a.nseg * ((nxb + xc - 1) / xc) < fill) --xc;
+    }
+    if (xc > nxb) xc = nxb;
+    a.xc = xc;
+    a.nitems = a.nseg * ((nxb + xc - 1) / xc);
+    const int grid = (a.nitems + 7) / 8 * 8;
+    a.per_xcd = grid / 8;
     if (ctx->id_bytes == 2) hipLaunchKernelGGL(k_interior<uint16_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
     else hipLaunchKernelGGL(k_interior<uint32_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
+    return hipGetLastError();
+}
+
+// Form 7: the interior kernel and the H boundary pass of the split step as
+// one launch (k_interior_hb), both with chunks of xc planes.
+static hipError_t launch_interior_hb(const gpx_ctx *ctx, int hflags, int it)
+{
+    IntArgs a;
+    interior_args(ctx, it, a);
+    StepArgs hb;
+    int block;
+    size_t lds_b;
+    base_args(ctx, 1, hb, block, lds_b, true);
+    const int nxb = a.box[1] - a.box[0];
+    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior_hb<uint16_t> : (const void *)k_interior_hb<uint32_t>;
+    const size_t lds_i = (size_t)ctx->desc.nmat * 40;
+    const size_t lds = lds_i > lds_b ? lds_i : lds_b;
+    auto total = [&](int xc) {
+        return (long long)a.nseg * ((nxb + xc - 1) / xc) + count_items(ctx, true, INT_BLOCK, xc, nullptr, true);
+    };
+    int xc = ctx->xchunk ? ctx->xchunk : 4;
+    if (!ctx->xchunk) {
+        const int fill = resident_blocks(ctx, k, INT_BLOCK, lds);
+        while (xc > 2 && total(xc) < fill) --xc;
+    }
+    if (xc > nxb) xc = nxb;
+    a.xc = xc;
+    a.nitems = a.nseg * ((nxb + xc - 1) / xc);
+    const long long nb = count_items(ctx, true, INT_BLOCK, xc, &hb, true);
+    hb.xc = xc;
+    hb.nseg = a.nseg;
+    hb.m_nchunk = (nxb + xc - 1) / xc;
+    hb.m_items = (int)(a.nitems + nb);
+    hb.nitems = 1 << 30;
+    const int grid = (hb.m_items + 7) / 8 * 8;
+    hb.per_xcd = grid / 8;
+    hb.flags = hflags;
+    hb.it = it;
+    if (ctx->id_bytes == 2)
+        hipLaunchKernelGGL(k_interior_hb<uint16_t>, dim3(grid), dim3(INT_BLOCK), lds, ctx->stream, a, hb);
+    else
+        hipLaunchKernelGGL(k_interior_hb<uint32_t>, dim3(grid), dim3(INT_BLOCK), lds, ctx->stream, a, hb);
     return hipGetLastError();
 }
 
@@ -2145,23 +2345,23 @@ struct EventSet {
 // and [3]; or H interior, H boundary, E interior and E boundary.
 static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4);
 
-// Form 4 picks among form 2, form 5, the split step and the passes (form 2
-// and the passes only, where the split step does not apply) on the first
-// run long enough to time them: after two warm-up iterations, TUNE_ITS
-// iterations of each, timed with events; the fastest serves the rest of this
-// run and later runs.  The forms give the same bits, so the choice does not
-// change results.
-static const int TUNE_ITS = 6;
+// Form 4 picks among forms 7, 6, 2, 5, the split step and the passes (forms
+// 6, 2 and the passes only, where the split step does not apply) on the
+// first run long enough to time them: after two warm-up iterations, TUNE_ITS
+// iterations of each, timed with events; the fastest serves the rest of
+// this run and later runs.  The forms give the same bits, so the choice does
+// not change results.
+static const int TUNE_ITS = 4;
 
 static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
 {
-    if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || n < 2 + 4 * TUNE_ITS)
+    if (ms4 || ctx->fused != 4 || ctx->split_choice != -1 || step_tile(ctx) || n < 2 + 6 * TUNE_ITS)
         return run_steps_form(ctx, it0, n, ms4);
-    // choices in the order timed: form 2, form 5, the split step, the passes
-    static const int all[4] = {3, 2, 1, 0}, passes_only[2] = {3, 0};
+    // choices in the order timed: form 7, form 6, form 2, form 5, the split step, the passes
+    static const int all[6] = {5, 4, 3, 2, 1, 0}, passes_only[3] = {4, 3, 0};
     const bool split = split_eligible(ctx);
     const int *order = split ? all : passes_only;
-    const int nf = split ? 4 : 2;
+    const int nf = split ? 6 : 3;
     EventSet ev;
     ev.e.assign(nf + 1, nullptr);
     for (auto &x : ev.e) GPX_TRY(hipEventCreate(&x));
@@ -2189,8 +2389,8 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
         return rc;
     }
     ctx->split_choice = order[best];
-    // the passes, alone or side by side, update in place
-    if (order[best] == 0 || order[best] == 3)
+    // the passes, alone, side by side or merged, update in place
+    if (order[best] == 0 || order[best] == 3 || order[best] == 4)
         if ((rc = release_next_state(ctx))) return rc;
     const int done = 2 + nf * TUNE_ITS;
     return run_steps_form(ctx, it0 + done, n - done, nullptr);
@@ -2240,6 +2440,18 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
             for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
             continue;
         }
+        if (split && split_merged(ctx)) {
+            // events: the merged launch between ev[0] and ev[1] (ev[2] right
+            // after), the E boundary pass between ev[2] and ev[3]
+            if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            GPX_TRY(launch_interior_hb(ctx, hf, it));
+            if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+            if (ev) GPX_TRY(hipEventRecord(ev[2], ctx->stream));
+            GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, true));
+            if (ev) GPX_TRY(hipEventRecord(ev[3], ctx->stream));
+            for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
+            continue;
+        }
         if (split) {
             if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
             GPX_TRY(launch_interior(ctx, it));
@@ -2249,6 +2461,19 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
             GPX_TRY(launch_kernel(ctx, pick_kernel<true>(ctx, 0), true, 0, ef, it, nullptr, true));
             if (ev) GPX_TRY(hipEventRecord(ev[3], ctx->stream));
             for (int c = 0; c < 6; ++c) std::swap(ctx->F[c], ctx->Fn[c]);
+            continue;
+        }
+        if (passes_merged(ctx)) {
+            // events: the H pass between ev[0] and ev[1], the E pass between
+            // ev[3] and ev[4]
+            if (ev) GPX_TRY(hipEventRecord(ev[0], ctx->stream));
+            GPX_TRY(launch_merged(ctx, 1, hf, it));
+            if (ev) GPX_TRY(hipEventRecord(ev[1], ctx->stream));
+            GPX_TRY(launch_tl(ctx, 1, it));
+            if (ev) GPX_TRY(hipEventRecord(ev[3], ctx->stream));
+            GPX_TRY(launch_merged(ctx, 0, ef, it));
+            if (ev) GPX_TRY(hipEventRecord(ev[4], ctx->stream));
+            GPX_TRY(launch_tl(ctx, 0, it));
             continue;
         }
         if (passes_beside(ctx) && !ms4) {
@@ -2269,10 +2494,11 @@ static int run_steps_form(gpx_ctx *ctx, int it0, int n, float *ms4)
         // (begin, end) event pairs of each kernel slot: fused step; split
         // step (interior, H boundary, -, E boundary); passes (H interior, H
         // boundary, E interior, E boundary)
-        static const int pairs[3][4][2] = {{{0, 1}, {-1, -1}, {-1, -1}, {-1, -1}},
+        static const int pairs[4][4][2] = {{{0, 1}, {-1, -1}, {-1, -1}, {-1, -1}},
                                            {{0, 1}, {1, 2}, {-1, -1}, {2, 3}},
-                                           {{0, 1}, {1, 2}, {3, 4}, {4, 5}}};
-        const int form = tc ? 0 : (split ? 1 : 2);
+                                           {{0, 1}, {1, 2}, {3, 4}, {4, 5}},
+                                           {{0, 1}, {-1, -1}, {3, 4}, {-1, -1}}};
+        const int form = tc ? 0 : (split ? 1 : (passes_merged(ctx) ? 3 : 2));
         for (int i = 0; i < n; ++i) {
             hipEvent_t *ev = &evs.e[(size_t)i * per];
             for (int q = 0; q < 4; ++q) {
@@ -2297,7 +2523,8 @@ extern "C" int gpx_step_form(const gpx_ctx *ctx)
 {
     if (!ctx) return GPX_EARG;
     if (step_tile(ctx)) return 1;
-    if (split_ok(ctx)) return split_beside(ctx) ? 5 : 3;
+    if (split_ok(ctx)) return split_beside(ctx) ? 5 : (split_merged(ctx) ? 7 : 3);
+    if (passes_merged(ctx)) return 6;
     return passes_beside(ctx) ? 2 : 0;
 }
 

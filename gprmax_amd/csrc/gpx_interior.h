@@ -60,8 +60,9 @@ __device__ __forceinline__ float int_msrc(const IntArgs &a, const float *sH, int
     return h;
 }
 
+// The body for work item `item` (< a.nitems; block-uniform).
 template <class IDT>
-__global__ void __launch_bounds__(INT_BLOCK) k_interior(IntArgs a)
+__device__ __forceinline__ void interior_run(const IntArgs &a, int item)
 {
     extern __shared__ float4 lds4[];
     const int nm = a.nmat;
@@ -71,8 +72,6 @@ __global__ void __launch_bounds__(INT_BLOCK) k_interior(IntArgs a)
     stage_coef(cH, sH, a.coefH, nm);
     __syncthreads();
 
-    const int item = logical_block(a.per_xcd);
-    if (item >= a.nitems) return;
     const int it = a.it;
     const int B = blockDim.x;
     const int NZ1 = a.nz + 1;
@@ -271,6 +270,14 @@ __global__ void __launch_bounds__(INT_BLOCK) k_interior(IntArgs a)
         ey = cur.ey1;
         ez = cur.ez1;
     }
+}
+
+template <class IDT>
+__global__ void __launch_bounds__(INT_BLOCK) k_interior(IntArgs a)
+{
+    const int item = logical_block(a.per_xcd);
+    if (item >= a.nitems) return;
+    interior_run<IDT>(a, item);
 }
 
 }  // namespace gpx

@@ -196,7 +196,9 @@ class HipSolver:
         """Step form of gpx_run: 0 the H and E passes, 1 the fused step, 2 the
         passes with interior and boundary kernels side by side, 3 the split
         step, 5 the split step with the H boundary pass beside the interior
-        kernel, 4 (the library's default) the fastest of 2, 5, 3 and 0 (of 2
+        kernel, 6 the passes as one launch each over every cell, 7 the split
+        step with the H boundary pass inside the interior kernel's launch, 4
+        (the library's default) the fastest of 7, 6, 2, 5, 3 and 0 (of 6, 2
         and 0 where the split step does not apply), timed on the first run of
         at least 26 iterations.  All give the same bits."""
         self._check(self.lib.gpx_set_step(self.ctx, int(fused), int(tile_cells)))
@@ -207,12 +209,15 @@ class HipSolver:
 
     def step_form(self):
         """The form gpx_run uses: 0 passes, 1 fused, 2 side by side, 3 split,
-        5 split with the H boundary pass beside the interior kernel."""
+        5 split with the H boundary pass beside the interior kernel, 6 merged
+        passes, 7 split with the H boundary pass in the interior launch."""
         return int(self.lib.gpx_step_form(self.ctx))
 
     def step_choice(self):
         """The default form's choice: -1 not timed yet, 0 passes, 1 split,
-        2 split with the H boundary pass beside, 3 passes side by side."""
+        2 split with the H boundary pass beside, 3 passes side by side, 4
+        merged passes, 5 split with the H boundary pass in the interior
+        launch."""
         return int(self.lib.gpx_step_choice(self.ctx))
 
     def run(self, it0=0, n=None):

@@ -171,10 +171,14 @@ int gpx_set_launch(gpx_ctx *ctx, int xchunk, int block);
  * split step, one kernel doing H and E for the interior and the H and E
  * boundary passes around it, double-buffered; fused = 5: the split step with
  * the H boundary pass running beside the interior kernel on a second stream
- * (gpx_run_profiled runs them one after the other); fused = 4 (default): the
- * first gpx_run of at least 26 iterations times form 2, form 5, the split
- * step and the passes (form 2 and the passes where the split step does not
- * apply) on six iterations each and keeps the fastest (gpx_step_choice).  tile_cells = cells per fused block (256, 512 or 1024;
+ * (gpx_run_profiled runs them one after the other); fused = 6: the passes,
+ * each as one launch over every cell, the interior and boundary items of
+ * each x chunk next to each other; fused = 7: the split step with the H
+ * boundary pass's items inside the interior kernel's launch, likewise
+ * interleaved; fused = 4 (default): the first gpx_run of at least 26
+ * iterations times forms 7, 6, 2, 5, the split step and the passes (forms 6,
+ * 2 and the passes where the split step does not apply) on four iterations
+ * each and keeps the fastest (gpx_step_choice).  tile_cells = cells per fused block (256, 512 or 1024;
  * 0 = auto).  All forms give identical results; the
  * fused form needs a tile of at least twice nz+1 cells within the LDS budget,
  * the split step a model without dispersive poles, and the passes run when
@@ -187,12 +191,15 @@ int gpx_step_kind(const gpx_ctx *ctx);
 
 /* The form gpx_run will use: 0 passes, 1 fused step, 2 passes side by side,
  * 3 split step, 5 split step with the H boundary pass beside the interior
- * kernel.  gpx_run_profiled reports the split step's interior kernel in
- * ms4[0] and its boundary passes in ms4[1] and ms4[3]. */
+ * kernel, 6 merged passes, 7 split step with the H boundary pass in the
+ * interior kernel's launch.  gpx_run_profiled reports the split step's
+ * interior kernel in ms4[0] and its boundary passes in ms4[1] and ms4[3]
+ * (form 7: the merged launch in ms4[0], ms4[1] = 0), and the merged passes'
+ * H and E launches in ms4[0] and ms4[2]. */
 int gpx_step_form(const gpx_ctx *ctx);
 
 /* Form 4's choice: -1 not timed yet, 0 the passes, 1 the split step, 2 form 5,
- * 3 form 2. */
+ * 3 form 2, 4 form 6, 5 form 7. */
 int gpx_step_choice(const gpx_ctx *ctx);
 
 int gpx_synchronize(gpx_ctx *ctx);

@@ -177,15 +177,16 @@ def _compare_runs(F, rxb, snaps, phis, T, Gc):
             assert np.array_equal(T[n], getattr(Gc, name)), name
 
 
-FORMS = dict(argvalues=[1, 0, 2, 3, 5, 4],
-             ids=['fused-step', 'two-pass', 'side-by-side', 'split', 'split-beside', 'timed'])
-CHOSEN_FORM = {0: 0, 1: 3, 2: 5, 3: 2}    # form 4's choice -> the form it runs
+FORMS = dict(argvalues=[1, 0, 2, 6, 3, 5, 7, 4],
+             ids=['fused-step', 'two-pass', 'side-by-side', 'merged', 'split', 'split-beside', 'split-merged',
+                  'timed'])
+CHOSEN_FORM = {0: 0, 1: 3, 2: 5, 3: 2, 4: 6, 5: 7}    # form 4's choice -> the form it runs
 
 
 def _expected_form(fused, G):
     # form 4 runs the split step until its first long run has timed it
-    if fused in (3, 4, 5):
-        return (5 if fused == 5 else 3) if G.maxpoles == 0 else 0
+    if fused in (3, 4, 5, 7):
+        return (fused if fused in (5, 7) else 3) if G.maxpoles == 0 else 0
     return fused
 
 
@@ -202,12 +203,12 @@ def test_fused_run_matches_oracle(case, fused):
         assert s.step_form() == _expected_form(fused, G)
         assert s.step_choice() == -1
         s.run(0, G.iterations)
-        # 40 iterations: form 4 timed four forms on iterations 2-25 (two,
-        # the passes alone and side by side, on iterations 2-13 of dispersive
-        # models) and went on with the fastest, switching forms in the middle
-        # of the run
+        # 40 iterations: form 4 timed six forms on iterations 2-25 (three,
+        # the passes alone, side by side and merged, on iterations 2-13 of
+        # dispersive models) and went on with the fastest, switching forms in
+        # the middle of the run
         if fused == 4:
-            assert s.step_choice() in ((0, 3) if G.maxpoles else (0, 1, 2, 3))
+            assert s.step_choice() in ((0, 3, 4) if G.maxpoles else (0, 1, 2, 3, 4, 5))
             assert s.step_form() == CHOSEN_FORM[s.step_choice()]
         else:
             assert s.step_choice() == -1
@@ -220,7 +221,7 @@ def test_fused_run_matches_oracle(case, fused):
     _compare_runs(F, rxb, snaps, phis, T, Gc)
 
 
-@pytest.mark.parametrize('form', [3, 5], ids=['split', 'split-beside'])
+@pytest.mark.parametrize('form', [3, 5, 7], ids=['split', 'split-beside', 'split-merged'])
 @pytest.mark.parametrize('xchunk', [0, 1, 2, 5])
 @pytest.mark.parametrize('case', ['horipml1', 'mripml2'])
 def test_split_step_from_random_state(case, xchunk, form):
@@ -294,7 +295,7 @@ def test_2d_run_matches_oracle(case, axis, fused):
     with _hip()(G) as s:
         s.set_step(fused)
         assert (s.step_kind() > 0) == (fused == 1)
-        assert s.step_form() == (0 if fused in (3, 4, 5) else fused)   # a 2-D grid has no fused box
+        assert s.step_form() == (0 if fused in (3, 4, 5, 7) else fused)   # a 2-D grid has no fused box
         s.run(0, G.iterations)
         F = s.download_fields()
         rxb = s.download_rx()
@@ -368,7 +369,7 @@ def test_step_forms_agree_on_a_wide_grid(poles):
     """All step forms give the same bits on a grid with several tiles and
     several blocks per plane."""
     out = []
-    for fused in (1, 0, 2, 3, 4):
+    for fused in (1, 0, 2, 6, 3, 7, 4):
         G = rich_model(seed=8, shape=WIDE, iterations=30, poles=poles, order=2)
         with _hip()(G) as s:
             s.set_step(fused)
@@ -512,7 +513,7 @@ def test_pass_batch_of_unlike_models():
 
 
 def test_default_form_times_once_and_resets():
-    """The default form times four forms on the first run of at least 26
+    """The default form times six forms on the first run of at least 26
     iterations only: short runs and profiled runs leave
     the choice open, gpx_set_step clears it, and the results stay the
     oracle's."""
@@ -525,7 +526,7 @@ def test_default_form_times_once_and_resets():
         s.run_profiled(10, 5)              # profiled runs never time
         assert s.step_choice() == -1
         s.run(15, 30)                      # times iterations 17-40, then keeps the fastest
-        assert s.step_choice() in (0, 1, 2, 3)
+        assert s.step_choice() in (0, 1, 2, 3, 4, 5)
         s.set_step(4)
         assert s.step_choice() == -1
         s.run(45, 15)
@@ -539,7 +540,7 @@ def test_default_form_times_once_and_resets():
             assert np.array_equal(rxb[c, :, r], rx.outputs[name]), (r, name)
 
 
-@pytest.mark.parametrize('form', [3, 5], ids=['split', 'split-beside'])
+@pytest.mark.parametrize('form', [3, 5, 7], ids=['split', 'split-beside', 'split-merged'])
 @pytest.mark.parametrize('xchunk', [0, 3])
 def test_split_step_sources_on_recomputed_cells(xchunk, form):
     """Magnetic dipoles on the cells whose new H the split step recomputes

@@ -19,7 +19,8 @@ def _hip():
     return HipSolver
 
 
-@pytest.mark.parametrize('fused', [0, 1, 2, 4], ids=['two-pass', 'fused-requested', 'side-by-side', 'default'])
+@pytest.mark.parametrize('fused', [0, 1, 2, 6, 4], ids=['two-pass', 'fused-requested', 'side-by-side', 'merged',
+                                                   'default'])
 @pytest.mark.parametrize('case', [('HORIPML', 1), ('MRIPML', 2)], ids=['horipml1', 'mripml2'])
 def test_lines_match_oracle(case, fused):
     form, order = case
@@ -27,10 +28,10 @@ def test_lines_match_oracle(case, fused):
     Gc = tl_model(iterations=100, order=order, formulation=form)
     with _hip()(G) as s:
         s.set_step(fused)
-        assert s.step_form() == (2 if fused == 2 else 0)
+        assert s.step_form() == (fused if fused in (2, 6) else 0)
         s.run(0, G.iterations)
-        if fused == 4:   # timed the passes alone and side by side, kept the faster
-            assert s.step_choice() in (0, 3)
+        if fused == 4:   # timed the passes alone, side by side and merged, kept the fastest
+            assert s.step_choice() in (0, 3, 4)
         F = s.download_fields()
         rxb = s.download_rx()
         phis = [[s.download_phi(n, w) for w in range(4)] for n in range(len(G.pmls))]
