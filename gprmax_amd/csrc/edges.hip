@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -174,6 +175,149 @@ __global__ void k_edges_scatter(EdgeArgs a)
 }
 
 std::string g_edge_err;
+
+// ---------------------------------------------------------------- no-allocation variant
+// gpx_edges_static keeps its hash tables in one file-scope device array per
+// GPU instead of memory it allocates.  Layout (STATIC_CAP slots):
+//   ukeys, ufirst  u64[STATIC_CAP]    sets not given by the caller (classify table)
+//   kkeys          u64[STATIC_CAP]    sets the caller resolved (lookup table)
+//   kids           u32[STATIC_CAP]    their materials
+//   ckeys, cfirst  u64[STATIC_CAP]    the classify table compacted, table order
+//   skeys          u64[STATIC_MAX]    staging of the caller's resolved sets
+//   sids           u32[STATIC_MAX]
+//   words          u32[64]            0: overflow, 1: duplicate key, 2: count
+// Tables are at most half full (STATIC_MAX = STATIC_CAP / 2 sets).
+constexpr unsigned int STATIC_CAP = 1u << 16;
+constexpr unsigned int STATIC_MAX = STATIC_CAP / 2;
+constexpr size_t SO_UKEYS = 0;
+constexpr size_t SO_UFIRST = SO_UKEYS + (size_t)STATIC_CAP * 8;
+constexpr size_t SO_KKEYS = SO_UFIRST + (size_t)STATIC_CAP * 8;
+constexpr size_t SO_KIDS = SO_KKEYS + (size_t)STATIC_CAP * 8;
+constexpr size_t SO_CKEYS = SO_KIDS + (size_t)STATIC_CAP * 4;
+constexpr size_t SO_CFIRST = SO_CKEYS + (size_t)STATIC_CAP * 8;
+constexpr size_t SO_SKEYS = SO_CFIRST + (size_t)STATIC_CAP * 8;
+constexpr size_t SO_SIDS = SO_SKEYS + (size_t)STATIC_MAX * 8;
+constexpr size_t SO_WORDS = SO_SIDS + (size_t)STATIC_MAX * 4;
+constexpr size_t STATIC_BYTES = SO_WORDS + 256;
+
+struct StaticTables {
+    unsigned long long *ukeys, *ufirst, *kkeys, *ckeys, *cfirst, *skeys;
+    uint32_t *kids, *sids, *words;
+};
+
+StaticTables static_tables(unsigned char *base)
+{
+    StaticTables t;
+    t.ukeys = (unsigned long long *)(base + SO_UKEYS);
+    t.ufirst = (unsigned long long *)(base + SO_UFIRST);
+    t.kkeys = (unsigned long long *)(base + SO_KKEYS);
+    t.kids = (uint32_t *)(base + SO_KIDS);
+    t.ckeys = (unsigned long long *)(base + SO_CKEYS);
+    t.cfirst = (unsigned long long *)(base + SO_CFIRST);
+    t.skeys = (unsigned long long *)(base + SO_SKEYS);
+    t.sids = (uint32_t *)(base + SO_SIDS);
+    t.words = (uint32_t *)(base + SO_WORDS);
+    return t;
+}
+
+// Empty tables, zero flags (a kernel, so the clearing is an ordinary launch).
+__global__ void k_static_clear(StaticTables t)
+{
+    const unsigned int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < STATIC_CAP) {
+        t.ukeys[g] = KEY_EMPTY;
+        t.ufirst[g] = KEY_EMPTY;
+        t.kkeys[g] = KEY_EMPTY;
+        t.kids[g] = 0;
+    }
+    if (g < 64) t.words[g] = 0;
+}
+
+// The caller's resolved sets into the lookup table; a key given twice sets
+// the duplicate flag.
+__global__ void k_static_known(StaticTables t, int n)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const unsigned long long key = t.skeys[g];
+    unsigned int s = key_hash(key, STATIC_CAP);
+    for (unsigned int probe = 0; probe < STATIC_CAP; ++probe, s = (s + 1) & (STATIC_CAP - 1)) {
+        const unsigned long long prev = atomicCAS(&t.kkeys[s], KEY_EMPTY, key);
+        if (prev == KEY_EMPTY) {
+            t.kids[s] = t.sids[g];
+            return;
+        }
+        if (prev == key) {
+            atomicOr(&t.words[1], 1u);
+            return;
+        }
+    }
+    atomicOr(&t.words[0], 1u);
+}
+
+// Uniform edges get their material, mixed edges of a resolved set get the
+// set's material, the other mixed edges enter the classify table with their
+// first edge.  The lookup table is complete before this kernel starts.
+__global__ void k_edges_static(EdgeArgs a, StaticTables t)
+{
+    const long long NY1 = a.ny + 1, NZ1 = a.nz + 1;
+    const long long ntot = (long long)(a.nx + 1) * NY1 * NZ1;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ntot) return;
+    const int i = (int)(g / (NY1 * NZ1)), j = (int)((g / NZ1) % NY1), k = (int)(g % NZ1);
+    uint32_t uniform;
+    const unsigned long long key = edge_key(a, i, j, k, &uniform);
+    if (uniform != 0xffffffffu) a.ID[(size_t)a.comp * ntot + g] = uniform;
+    if (key == KEY_EMPTY) return;
+    unsigned int s = key_hash(key, STATIC_CAP);
+    for (unsigned int probe = 0; probe < STATIC_CAP; ++probe, s = (s + 1) & (STATIC_CAP - 1)) {
+        const unsigned long long kk = t.kkeys[s];
+        if (kk == key) {
+            a.ID[(size_t)a.comp * ntot + g] = t.kids[s];
+            return;
+        }
+        if (kk == KEY_EMPTY) break;
+    }
+    const unsigned long long index = (unsigned long long)a.comp * ntot + g;
+    s = key_hash(key, STATIC_CAP);
+    for (unsigned int probe = 0; probe < STATIC_MAX; ++probe, s = (s + 1) & (STATIC_CAP - 1)) {
+        const unsigned long long prev = atomicCAS(&t.ukeys[s], KEY_EMPTY, key);
+        if (prev == KEY_EMPTY || prev == key) {
+            atomicMin(&t.ufirst[s], index);
+            return;
+        }
+    }
+    atomicOr(&t.words[0], 1u);
+}
+
+// The classify table's sets, compacted in table order (one block of 1024
+// threads, 64 consecutive slots per thread); words[2] = their number.
+__global__ void __launch_bounds__(1024) k_static_compact(StaticTables t)
+{
+    constexpr int PER = STATIC_CAP / 1024;
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const unsigned int s0 = (unsigned int)tid * PER;
+    uint32_t n = 0;
+    for (int q = 0; q < PER; ++q) n += t.ukeys[s0 + q] != KEY_EMPTY;
+    part[tid] = n;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan
+        const uint32_t v = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t o = part[tid] - n;
+    for (int q = 0; q < PER; ++q) {
+        const unsigned long long key = t.ukeys[s0 + q];
+        if (key == KEY_EMPTY) continue;
+        t.ckeys[o] = key;
+        t.cfirst[o] = t.ufirst[s0 + q];
+        ++o;
+    }
+    if (tid == 1023) t.words[2] = part[1023];
+}
 
 }  // namespace
 
@@ -350,5 +494,98 @@ extern "C" int gpx_edges_download(gpx_edges *b, uint32_t *ID)
     EDGE_TRY(hipSetDevice(b->device));
     EDGE_TRY(hipMemcpyAsync(ID, b->ID, 6 * b->ntot * 4, hipMemcpyDeviceToHost, b->stream));
     EDGE_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- no-allocation entry point
+// The device array behind gpx_edges_static (3.1 MiB; each GPU that runs it
+// holds its own copy until the process ends).  Host code never passes this
+// name to a kernel or a copy (on the host it is only a stand-in for the
+// symbol); it uses the device address hipGetSymbolAddress gives for the
+// current GPU.
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
+namespace {
+constexpr int STATIC_DEVICES = 64;
+std::mutex g_static_mu;
+unsigned char *g_static_ptr[STATIC_DEVICES] = {nullptr};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current.
+int static_base(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return efail(nullptr, e != hipSuccess ? (int)e : GPX_EARG,
+                         std::string("gpx_edges_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_static_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_static_ptr[device];
+    return 0;
+}
+}  // namespace
+
+extern "C" int gpx_edges_static(int device, int nx, int ny, int nz, const uint32_t *solid, const uint8_t *rigid,
+                                uint32_t *ID, int field, int nknown, const unsigned long long *known_keys,
+                                const uint32_t *known_ids, unsigned long long *keys, unsigned long long *first,
+                                int max, int *count)
+{
+    gpx_edges *b = nullptr;  // errors go to gpx_edges_last_error(NULL)
+    if (device < 0 || device >= STATIC_DEVICES || nx < 1 || ny < 1 || nz < 1 || !solid || !rigid || !ID ||
+        (field != 0 && field != 1) || nknown < 0 || (nknown && (!known_keys || !known_ids)) || max < 0 || !count)
+        return efail(b, GPX_EARG, "gpx_edges_static: bad argument");
+    if (nknown > (int)STATIC_MAX)
+        return efail(b, GPX_ELIMIT, "gpx_edges_static: at most " + std::to_string(STATIC_MAX) +
+                                         " resolved material sets per call (" + std::to_string(nknown) + " given)");
+    EDGE_TRY(hipSetDevice(device));
+    unsigned char *base = nullptr;
+    if (int rc = static_base(device, &base)) return rc;
+    const StaticTables t = static_tables(base);
+    hipStream_t s = nullptr;  // the default stream: ordered after the caller's work on it
+    hipLaunchKernelGGL(k_static_clear, dim3(STATIC_CAP / 256), dim3(256), 0, s, t);
+    EDGE_TRY(hipGetLastError());
+    if (nknown) {
+        EDGE_TRY(hipMemcpyAsync(t.skeys, known_keys, (size_t)nknown * 8, hipMemcpyHostToDevice, s));
+        EDGE_TRY(hipMemcpyAsync(t.sids, known_ids, (size_t)nknown * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_static_known, dim3((nknown + 255) / 256), dim3(256), 0, s, t, nknown);
+        EDGE_TRY(hipGetLastError());
+    }
+    EdgeArgs a;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.solid = solid;
+    a.rigid = rigid;
+    a.ID = ID;
+    a.keys = nullptr; a.first = nullptr; a.cap = 0; a.overflow = nullptr; a.resolved = nullptr;
+    const long long ntot = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+    const int B = 256;
+    const unsigned int grid = (unsigned int)((ntot + B - 1) / B);
+    for (int c = 3 * field; c < 3 * field + 3; ++c) {
+        a.comp = c;
+        hipLaunchKernelGGL(k_edges_static, dim3(grid), dim3(B), 0, s, a, t);
+        EDGE_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_static_compact, dim3(1), dim3(1024), 0, s, t);
+    EDGE_TRY(hipGetLastError());
+    uint32_t words[3] = {0, 0, 0};
+    EDGE_TRY(hipMemcpyAsync(words, t.words, sizeof(words), hipMemcpyDeviceToHost, s));
+    EDGE_TRY(hipStreamSynchronize(s));
+    if (words[1]) return efail(b, GPX_EARG, "gpx_edges_static: a resolved material set is given twice");
+    if (words[0])
+        return efail(b, GPX_ELIMIT, "gpx_edges_static: more than " + std::to_string(STATIC_MAX) +
+                                         " distinct material sets to resolve (the table is full)");
+    const int n = (int)words[2];
+    if (n > (int)STATIC_MAX)
+        return efail(b, GPX_ELIMIT, "gpx_edges_static: more than " + std::to_string(STATIC_MAX) +
+                                         " distinct material sets to resolve (" + std::to_string(n) + ")");
+    const int m = n < max ? n : max;
+    if (m > 0) {
+        if (keys) EDGE_TRY(hipMemcpyAsync(keys, t.ckeys, (size_t)m * 8, hipMemcpyDeviceToHost, s));
+        if (first) EDGE_TRY(hipMemcpyAsync(first, t.cfirst, (size_t)m * 8, hipMemcpyDeviceToHost, s));
+        EDGE_TRY(hipStreamSynchronize(s));
+    }
+    *count = n;
     return 0;
 }

@@ -27,7 +27,11 @@ SIGNATURES = {
     'gpx_edges_classify': ([_P, _I, _P, _P, _I, ctypes.POINTER(_I)], _I),
     'gpx_edges_resolve': ([_P, _I, _P, _I], _I),
     'gpx_edges_download': ([_P, _P], _I),
+    'gpx_edges_static': ([_I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _I, ctypes.POINTER(_I)], _I),
 }
+
+# Sets per call of gpx_edges_static (given or returned).
+STATIC_MAX_SETS = 1 << 15
 
 # The cells around an edge in the reference's numID1.. order, per component.
 _OFFSETS = [((0, 0, 0), (0, -1, 0), (0, -1, -1), (0, 0, -1)),
@@ -49,6 +53,24 @@ def _lib():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _resolve_sets(G, solid, field, first, ntot):
+    """The material of each mixed set, found or created in the order of the
+    sets' first edges (the reference's loop order), as the host build does."""
+    first = np.asarray(first).astype(np.int64)
+    numids = np.zeros(len(first), dtype=np.uint32)
+    cache = {}
+    for s in np.argsort(first, kind='stable'):
+        comp, point = divmod(int(first[s]), ntot)
+        i, rest = divmod(point, (G.ny + 1) * (G.nz + 1))
+        j, k = divmod(rest, G.nz + 1)
+        row = [int(solid[i + a, j + b, k + c]) for a, b, c in _OFFSETS[comp]]
+        names = [G.materials[m].ID for m in row]
+        if any('+' in nm for nm in names):
+            raise GeneralError('the device edge build needs material names without "+"')
+        numids[s] = _average_material(G, names, row, cache, field == 0)
+    return numids
 
 
 def build_components_on_device(G, device=0, capacity=1 << 16):
@@ -75,18 +97,7 @@ def build_components_on_device(G, device=0, capacity=1 << 16):
             if rc:
                 raise GeneralError('gpx_edges_classify failed ({}): {}'.format(rc, lib.gpx_edges_last_error(h).decode()))
             n = count.value
-            first = first[:n].astype(np.int64)
-            numids = np.zeros(n, dtype=np.uint32)
-            cache = {}
-            for s in np.argsort(first, kind='stable'):
-                comp, point = divmod(int(first[s]), ntot)
-                i, rest = divmod(point, (G.ny + 1) * (G.nz + 1))
-                j, k = divmod(rest, G.nz + 1)
-                row = [int(solid[i + a, j + b, k + c]) for a, b, c in _OFFSETS[comp]]
-                names = [G.materials[m].ID for m in row]
-                if any('+' in nm for nm in names):
-                    raise GeneralError('the device edge build needs material names without "+"')
-                numids[s] = _average_material(G, names, row, cache, field == 0)
+            numids = _resolve_sets(G, solid, field, first[:n], ntot)
             rc = lib.gpx_edges_resolve(h, field, _ptr(numids), n)
             if rc:
                 raise GeneralError('gpx_edges_resolve failed ({}): {}'.format(rc, lib.gpx_edges_last_error(h).decode()))
@@ -96,3 +107,38 @@ def build_components_on_device(G, device=0, capacity=1 << 16):
     finally:
         lib.gpx_edges_destroy(h)
     G.ID[...] = ID
+
+
+def build_components_static(G, solid_dev, rigidE_dev, rigidH_dev, ID_dev, device=0):
+    """The same build through gpx_edges_static, which allocates nothing: the
+    caller's device arrays (addresses as integers; solid uint32 [nx][ny][nz],
+    rigid uint8 [nx][ny][nz], ID uint32 [6][nx+1][ny+1][nz+1] holding the
+    geometry's indices) are read and ID is written in place on `device`.
+    Averaged materials are created in G.materials as the host build does; the
+    host copy of the cell materials, G.solid, names each set's cells."""
+    if len(G.materials) >= 0xFFFF:
+        raise GeneralError('the device edge build needs fewer than 65535 cell materials')
+    lib = _lib()
+    solid = np.ascontiguousarray(G.solid, dtype=np.uint32)
+    ntot = (G.nx + 1) * (G.ny + 1) * (G.nz + 1)
+    keys = np.zeros(STATIC_MAX_SETS, dtype=np.uint64)
+    first = np.zeros(STATIC_MAX_SETS, dtype=np.uint64)
+    count = ctypes.c_int()
+
+    def call(field, rigid, known_keys, known_ids):
+        rc = lib.gpx_edges_static(device, G.nx, G.ny, G.nz, solid_dev, rigid, ID_dev, field, len(known_keys),
+                                  _ptr(known_keys), _ptr(known_ids), _ptr(keys), _ptr(first), len(keys),
+                                  ctypes.byref(count))
+        if rc:
+            raise GeneralError('gpx_edges_static failed ({}): {}'.format(rc, lib.gpx_edges_last_error(None).decode()))
+        return count.value
+
+    for field, rigid in ((0, rigidE_dev), (1, rigidH_dev)):
+        none_k, none_i = np.zeros(1, dtype=np.uint64), np.zeros(1, dtype=np.uint32)
+        n = call(field, rigid, none_k[:0], none_i[:0])
+        if n:
+            set_keys = keys[:n].copy()
+            numids = _resolve_sets(G, solid, field, first[:n], ntot)
+            left = call(field, rigid, set_keys, numids)
+            if left:
+                raise GeneralError('gpx_edges_static: {} sets left unresolved'.format(left))

@@ -340,6 +340,29 @@ int gpx_edges_classify(gpx_edges *b, int field, unsigned long long *keys, unsign
 int gpx_edges_resolve(gpx_edges *b, int field, const uint32_t *numids, int count);
 int gpx_edges_download(gpx_edges *b, uint32_t *ID);
 
+/* ---- Edge build that allocates nothing.
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the work of gpx_edges_classify and gpx_edges_resolve on
+ * the caller's device arrays, with the hash tables in a fixed 3.1 MiB device
+ * array of the library (one per GPU, kept until the process ends) instead of
+ * allocated memory.  solid, rigid (rigidE for field 0, rigidH for field 1) and
+ * ID are DEVICE pointers on `device`, laid out as for gpx_edges_create; keys,
+ * first, known_keys, known_ids and count are host pointers.  One call writes
+ * the field's uniform edges, writes the mixed edges whose set is among the
+ * `nknown` (known_keys -> known_ids) pairs, and returns the other sets as
+ * gpx_edges_classify does (at most `max` written, *count their number, table
+ * order).  The usual sequence per field: call with nknown = 0, resolve the
+ * returned sets in first-edge order, call again with them (then *count is
+ * 0).  At most 32768 sets per call, given or returned (GPX_ELIMIT beyond).
+ * Runs on the default stream and returns when the work is done; calls that
+ * overlap on one GPU share the array and race. */
+int gpx_edges_static(int device, int nx, int ny, int nz, const uint32_t *solid, const uint8_t *rigid,
+                     uint32_t *ID, int field, int nknown, const unsigned long long *known_keys,
+                     const uint32_t *known_ids, unsigned long long *keys, unsigned long long *first, int max,
+                     int *count);
+
 #ifdef __cplusplus
 }
 #endif

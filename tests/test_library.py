@@ -74,3 +74,21 @@ def test_oracle_and_product_are_separate():
         if name.endswith('.py'):
             src = open(os.path.join(pkg, name)).read()
             assert 'import oracle' not in src and 'from oracle' not in src, name
+
+
+def test_static_edge_build_refuses_too_many_sets_before_touching_a_device():
+    """gpx_edges_static holds its tables in a fixed device array: more
+    resolved sets than it holds are refused with a clear error (GPX_ELIMIT)
+    before any device work."""
+    import numpy as np
+    from gprmax_amd.edges import STATIC_MAX_SETS, _lib
+    lib = _lib()
+    n = STATIC_MAX_SETS + 1
+    keys = np.arange(n, dtype=np.uint64)
+    ids = np.zeros(n, dtype=np.uint32)
+    count = ctypes.c_int(-1)
+    rc = lib.gpx_edges_static(0, 4, 4, 4, 1, 1, 1, 0, n, keys.ctypes.data, ids.ctypes.data, None, None, 0,
+                              ctypes.byref(count))
+    assert rc == -3
+    assert 'at most 32768' in lib.gpx_edges_last_error(None).decode()
+    assert count.value == -1
