@@ -2362,23 +2362,31 @@ static int run_steps(gpx_ctx *ctx, int it0, int n, float *ms4)
     const bool split = split_eligible(ctx);
     const int *order = split ? all : passes_only;
     const int nf = split ? 6 : 3;
+    // TUNE_ITS iterations per form, in two rounds over the forms, so that
+    // no form is timed only early or only late in the run.
+    const int half = TUNE_ITS / 2, rounds = 2;
     EventSet ev;
-    ev.e.assign(nf + 1, nullptr);
+    ev.e.assign(rounds * nf + 1, nullptr);
     for (auto &x : ev.e) GPX_TRY(hipEventCreate(&x));
     ctx->split_choice = order[0];
     int rc = run_steps_form(ctx, it0, 2, nullptr);
-    for (int f = 0; f < nf && !rc; ++f) {
-        ctx->split_choice = order[f];
-        rc = check_hip(ctx, hipEventRecord(ev.e[f], ctx->stream));
-        if (!rc) rc = run_steps_form(ctx, it0 + 2 + f * TUNE_ITS, TUNE_ITS, nullptr);
+    for (int q = 0; q < rounds * nf && !rc; ++q) {
+        ctx->split_choice = order[q % nf];
+        rc = check_hip(ctx, hipEventRecord(ev.e[q], ctx->stream));
+        if (!rc) rc = run_steps_form(ctx, it0 + 2 + q * half, half, nullptr);
     }
-    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[nf], ctx->stream));
-    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[nf]));
+    if (!rc) rc = check_hip(ctx, hipEventRecord(ev.e[rounds * nf], ctx->stream));
+    if (!rc) rc = check_hip(ctx, hipEventSynchronize(ev.e[rounds * nf]));
     int best = 0;
     float t_best = 0;
     for (int f = 0; f < nf && !rc; ++f) {
         float t = 0;
-        rc = check_hip(ctx, hipEventElapsedTime(&t, ev.e[f], ev.e[f + 1]));
+        for (int r = 0; r < rounds && !rc; ++r) {
+            float tq = 0;
+            const int q = r * nf + f;
+            rc = check_hip(ctx, hipEventElapsedTime(&tq, ev.e[q], ev.e[q + 1]));
+            t += tq;
+        }
         if (!rc && (f == 0 || t < t_best)) {
             best = f;
             t_best = t;
