@@ -608,12 +608,17 @@ __device__ __forceinline__ bool merged_item(const StepArgs &a, int L, int &item)
     const int per = a.nseg + a.nseg_b;
     const int mid = a.m_nchunk * per;
     if (L < mid) {
+        // within a chunk, the frame items spread evenly among the interior
+        // items: position r is a frame item when floor((r+1) nb / per) >
+        // floor(r nb / per), so frame and interior rows advance together
         const int c = L / per, r = L - c * per;
-        if (r < a.nseg) {
-            item = c * a.nseg + r;
+        const long long nb = a.nseg_b;
+        const int f0 = (int)(r * nb / per), f1 = (int)((r + 1) * nb / per);
+        if (f1 == f0) {
+            item = c * a.nseg + (r - f1);
             return false;
         }
-        item = a.items_a1 + a.items_a2 + c * a.nseg_b + (r - a.nseg);
+        item = a.items_a1 + a.items_a2 + c * a.nseg_b + f0;
         return true;
     }
     item = a.items_a1 + (L - mid);
