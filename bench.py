@@ -355,7 +355,10 @@ def main():
     info = device_info(0)
     default_form = not (args.fused or args.overlap or args.passes or args.split or args.split_beside or args.merged
                         or args.split_merged)
+    t_setup = time.perf_counter()
     with HipSolver(G) as s:
+        s.synchronize()
+        t_setup = time.perf_counter() - t_setup     # create + upload of the model (PCIe)
         s.set_launch(args.xchunk, args.block)
         s.set_step(1 if args.fused else (2 if args.overlap else (0 if args.passes else (
                    3 if args.split else (5 if args.split_beside else (6 if args.merged else (
@@ -427,6 +430,10 @@ def main():
                      'interior_cells': interior},
         'device_bytes': mem,
         'host_build_s': round(t_build, 1),
+        # the model's create + upload (host arrays over PCIe), and the rate of
+        # a whole solve of the workload's iterations that includes it (not `value`)
+        'setup_s': round(t_setup, 3),
+        'pcie_inclusive_mcells_per_s': round(cells * G.iterations / (ms / K * 1e-3 * G.iterations + t_setup) / 1e6, 1),
     }
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(args.workload)
