@@ -237,6 +237,9 @@ __device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int str
             const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
             if (BND) load(i, cur);
             else load(i + 1 < c.xb ? i + 1 : i, nxt);
+            PhiPre pre;
+            pre.s = -1;
+            if (BND) phi_preload(lslab, jkmask & (pf >> PF_SLAB_H), i, j, k, pre);
             const float ex = cur.ex, ey1 = cur.ey1, ez1 = cur.ez1;
             const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz;
             const float ezj = cur.ezj, exj = cur.exj, eyk = cur.eyk, exk = cur.exk;
@@ -270,17 +273,18 @@ __device__ __forceinline__ void h_pass_run(const StepArgs &a, int first, int str
                         const SlabArgs &sl = lslab[s];
                         int p;
                         uint32_t q1, q2, o1, o2;
-                        slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                        float a1, b1, a2, b2;
+                        phi_get(sl, s, pre, i, j, k, p, q1, q2, o1, o2, a1, b1, a2, b2);
                         const float *R = lR + sl.roff;
                         if (sl.axis == 0) {
-                            hy = pml_correct(a.form, sl, R, p, hy, true, ls[m4], dEz_x / sl.d, sl.phi1, q1, o1);
-                            hz = pml_correct(a.form, sl, R, p, hz, false, ls[m5], dEy_x / sl.d, sl.phi2, q2, o2);
+                            hy = pml_correct_pre(a.form, sl, R, p, hy, true, ls[m4], dEz_x / sl.d, a1, b1, sl.phi1, q1, o1);
+                            hz = pml_correct_pre(a.form, sl, R, p, hz, false, ls[m5], dEy_x / sl.d, a2, b2, sl.phi2, q2, o2);
                         } else if (sl.axis == 1) {
-                            hx = pml_correct(a.form, sl, R, p, hx, false, ls[m3], dEz_y / sl.d, sl.phi1, q1, o1);
-                            hz = pml_correct(a.form, sl, R, p, hz, true, ls[m5], dEx_y / sl.d, sl.phi2, q2, o2);
+                            hx = pml_correct_pre(a.form, sl, R, p, hx, false, ls[m3], dEz_y / sl.d, a1, b1, sl.phi1, q1, o1);
+                            hz = pml_correct_pre(a.form, sl, R, p, hz, true, ls[m5], dEx_y / sl.d, a2, b2, sl.phi2, q2, o2);
                         } else {
-                            hx = pml_correct(a.form, sl, R, p, hx, true, ls[m3], dEy_z / sl.d, sl.phi1, q1, o1);
-                            hy = pml_correct(a.form, sl, R, p, hy, false, ls[m4], dEx_z / sl.d, sl.phi2, q2, o2);
+                            hx = pml_correct_pre(a.form, sl, R, p, hx, true, ls[m3], dEy_z / sl.d, a1, b1, sl.phi1, q1, o1);
+                            hy = pml_correct_pre(a.form, sl, R, p, hy, false, ls[m4], dEx_z / sl.d, a2, b2, sl.phi2, q2, o2);
                         }
                     }
                 }
@@ -439,6 +443,9 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
             const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
             if (BND || P > 0) load(i, cur);
             else load(i + 1 < c.xb ? i + 1 : i, nxt);
+            PhiPre pre;
+            pre.s = -1;
+            if (BND) phi_preload(lslab, jkmask & (pf >> PF_SLAB_E), i, j, k, pre);
             const float hx = cur.hx, hy = cur.hy, hz = cur.hz;
             float ex = cur.ex, ey = cur.ey, ez = cur.ez;
             const float hzj = cur.hzj, hxj = cur.hxj, hyk = cur.hyk, hxk = cur.hxk;
@@ -502,17 +509,18 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
                         const SlabArgs &sl = lslab[s];
                         int p;
                         uint32_t q1, q2, o1, o2;
-                        slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+                        float a1, b1, a2, b2;
+                        phi_get(sl, s, pre, i, j, k, p, q1, q2, o1, o2, a1, b1, a2, b2);
                         const float *R = lR + sl.roff;
                         if (sl.axis == 0) {
-                            ey = pml_correct(a.form, sl, R, p, ey, false, ls[m1], dHz_x / sl.d, sl.phi1, q1, o1);
-                            ez = pml_correct(a.form, sl, R, p, ez, true, ls[m2], dHy_x / sl.d, sl.phi2, q2, o2);
+                            ey = pml_correct_pre(a.form, sl, R, p, ey, false, ls[m1], dHz_x / sl.d, a1, b1, sl.phi1, q1, o1);
+                            ez = pml_correct_pre(a.form, sl, R, p, ez, true, ls[m2], dHy_x / sl.d, a2, b2, sl.phi2, q2, o2);
                         } else if (sl.axis == 1) {
-                            ex = pml_correct(a.form, sl, R, p, ex, true, ls[m0], dHz_y / sl.d, sl.phi1, q1, o1);
-                            ez = pml_correct(a.form, sl, R, p, ez, false, ls[m2], dHx_y / sl.d, sl.phi2, q2, o2);
+                            ex = pml_correct_pre(a.form, sl, R, p, ex, true, ls[m0], dHz_y / sl.d, a1, b1, sl.phi1, q1, o1);
+                            ez = pml_correct_pre(a.form, sl, R, p, ez, false, ls[m2], dHx_y / sl.d, a2, b2, sl.phi2, q2, o2);
                         } else {
-                            ex = pml_correct(a.form, sl, R, p, ex, false, ls[m0], dHy_z / sl.d, sl.phi1, q1, o1);
-                            ey = pml_correct(a.form, sl, R, p, ey, true, ls[m1], dHx_z / sl.d, sl.phi2, q2, o2);
+                            ex = pml_correct_pre(a.form, sl, R, p, ex, false, ls[m0], dHy_z / sl.d, a1, b1, sl.phi1, q1, o1);
+                            ey = pml_correct_pre(a.form, sl, R, p, ey, true, ls[m1], dHx_z / sl.d, a2, b2, sl.phi2, q2, o2);
                         }
                     }
                 }

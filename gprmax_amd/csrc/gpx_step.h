@@ -116,12 +116,12 @@ __device__ __forceinline__ void st(T *base, uint32_t boff, T v)
 // counters).
 typedef __attribute__((address_space(1))) float gfloat_t;
 
+// Phi values p0 (order 0) and p1 (order 1, ORDER 2 only) already loaded.
 template <int FORM, int ORDER>
-__device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p, float F,
-                                       bool plus_sign, float c, float dd, const float *pin_,
+__device__ __forceinline__ float pml_v(const SlabArgs &s, const float *R, int p, float F,
+                                       bool plus_sign, float c, float dd, float p0, float p1,
                                        float *pout_, bool wr, uint32_t i1, uint32_t off)
 {
-    const gfloat_t *pin = (const gfloat_t *)pin_;
     gfloat_t *pout = (gfloat_t *)pout_;
     const int ot = ORDER * s.t;
     float corr;
@@ -129,7 +129,7 @@ __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p,
         if (ORDER == 1) {  // pml_updates_*_HORIPML_ext.pyx, order1_*
             const float RA01 = R[p] - 1.0f;
             const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
-            const float ph = pin[i1];
+            const float ph = p0;
             corr = c * (RA01 * dd + RB0 * ph);
             if (wr) pout[i1] = RE0 * ph - RF0 * dd;
         } else {           // order2_*
@@ -138,7 +138,6 @@ __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p,
             const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
             const float RF0 = R[3 * ot + p], RF1 = R[3 * ot + s.t + p];
             const float RA01 = RA0 * RA1 - 1.0f;
-            const float p0 = pin[i1], p1 = pin[i1 + off];
             corr = c * (RA01 * dd + RA1 * RB0 * p0 + RB1 * p1);
             if (wr) {
                 pout[i1 + off] = RE1 * p1 - RF1 * (RA0 * dd + RB0 * p0);
@@ -151,7 +150,7 @@ __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p,
             const float IRA1 = IRA - 1.0f;
             const float RB0 = R[ot + p], RE0 = R[2 * ot + p], RF0 = R[3 * ot + p];
             const float RC0 = IRA * RB0 * RF0;
-            const float ph = pin[i1];
+            const float ph = p0;
             corr = c * (IRA1 * dd - IRA * ph);
             if (wr) pout[i1] = RE0 * ph + RC0 * dd - RC0 * ph;
         } else {           // order2_*
@@ -160,7 +159,6 @@ __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p,
             const float RB0 = R[ot + p], RB1 = R[ot + s.t + p];
             const float RE0 = R[2 * ot + p], RE1 = R[2 * ot + s.t + p];
             const float RC0 = IRA * R[3 * ot + p], RC1 = IRA * R[3 * ot + s.t + p];
-            const float p0 = pin[i1], p1 = pin[i1 + off];
             const float psi = RB0 * p0 + RB1 * p1;
             corr = c * (IRA1 * dd - IRA * psi);
             if (wr) {
@@ -170,6 +168,16 @@ __device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p,
         }
     }
     return plus_sign ? F + corr : F - corr;
+}
+
+template <int FORM, int ORDER>
+__device__ __forceinline__ float pml_t(const SlabArgs &s, const float *R, int p, float F,
+                                       bool plus_sign, float c, float dd, const float *pin_,
+                                       float *pout_, bool wr, uint32_t i1, uint32_t off)
+{
+    const gfloat_t *pin = (const gfloat_t *)pin_;
+    const float p0 = pin[i1], p1 = ORDER == 2 ? pin[i1 + off] : 0.0f;
+    return pml_v<FORM, ORDER>(s, R, p, F, plus_sign, c, dd, p0, p1, pout_, wr, i1, off);
 }
 
 // The same with formulation and order chosen at run time.
@@ -185,11 +193,16 @@ __device__ __forceinline__ float pml_apply(int form, const SlabArgs &s, const fl
                         : pml_t<1, 2>(s, R, p, F, plus_sign, c, dd, pin, pout, wr, i1, off);
 }
 
-__device__ __forceinline__ float pml_correct(int form, const SlabArgs &s, const float *R, int p,
-                                             float F, bool plus_sign, float c, float dd,
-                                             float *phi, uint32_t i1, uint32_t off)
+// One in-place PML correction with the Phi values already loaded (phi_get).
+__device__ __forceinline__ float pml_correct_pre(int form, const SlabArgs &s, const float *R, int p,
+                                                 float F, bool plus_sign, float c, float dd, float p0,
+                                                 float p1, float *phi, uint32_t i1, uint32_t off)
 {
-    return pml_apply(form, s, R, p, F, plus_sign, c, dd, phi, phi, true, i1, off);
+    if (form == 0)
+        return s.order == 1 ? pml_v<0, 1>(s, R, p, F, plus_sign, c, dd, p0, p1, phi, true, i1, off)
+                            : pml_v<0, 2>(s, R, p, F, plus_sign, c, dd, p0, p1, phi, true, i1, off);
+    return s.order == 1 ? pml_v<1, 1>(s, R, p, F, plus_sign, c, dd, p0, p1, phi, true, i1, off)
+                        : pml_v<1, 2>(s, R, p, F, plus_sign, c, dd, p0, p1, phi, true, i1, off);
 }
 
 __device__ __forceinline__ bool in_box(const SlabArgs &s, int i, int j, int k)
@@ -210,6 +223,54 @@ __device__ __forceinline__ void slab_index(const SlabArgs &s, int i, int j, int 
     i2 = ((uint32_t)c[0] * s.d2[1] + c[1]) * s.d2[2] + c[2];
     o1 = (uint32_t)s.d1[0] * s.d1[1] * s.d1[2];
     o2 = (uint32_t)s.d2[0] * s.d2[1] * s.d2[2];
+}
+
+// Phi of the first slab in `smask` for cell (i, j, k), loaded as soon as the
+// cell's slabs are known: the loads then run beside the cell's field loads
+// instead of after the main update that the correction follows.
+struct PhiPre {
+    int s;             // the slab, -1 when none
+    int p;
+    uint32_t q1, q2, o1, o2;
+    float a1, b1, a2, b2;  // Phi1 orders 0 and 1, Phi2 orders 0 and 1
+};
+
+__device__ __forceinline__ void phi_preload(const SlabArgs *slab, uint32_t smask, int i, int j, int k, PhiPre &pp)
+{
+    pp.s = -1;
+    pp.a1 = pp.b1 = pp.a2 = pp.b2 = 0.0f;
+    if (!smask) return;
+    const int s = __ffs(smask) - 1;
+    const SlabArgs &sl = slab[s];
+    slab_index(sl, i, j, k, pp.p, pp.q1, pp.q2, pp.o1, pp.o2);
+    pp.s = s;
+    const gfloat_t *f1 = (const gfloat_t *)sl.phi1, *f2 = (const gfloat_t *)sl.phi2;
+    pp.a1 = f1[pp.q1];
+    pp.a2 = f2[pp.q2];
+    if (sl.order == 2) {
+        pp.b1 = f1[pp.q1 + pp.o1];
+        pp.b2 = f2[pp.q2 + pp.o2];
+    }
+}
+
+// Profile index, Phi indices and Phi values of slab `s` for cell (i, j, k):
+// from the preload when it holds that slab, else loaded here.
+__device__ __forceinline__ void phi_get(const SlabArgs &sl, int s, const PhiPre &pre, int i, int j, int k, int &p,
+                                        uint32_t &q1, uint32_t &q2, uint32_t &o1, uint32_t &o2, float &a1,
+                                        float &b1, float &a2, float &b2)
+{
+    if (s == pre.s) {
+        p = pre.p;
+        q1 = pre.q1; q2 = pre.q2; o1 = pre.o1; o2 = pre.o2;
+        a1 = pre.a1; b1 = pre.b1; a2 = pre.a2; b2 = pre.b2;
+        return;
+    }
+    slab_index(sl, i, j, k, p, q1, q2, o1, o2);
+    const gfloat_t *f1 = (const gfloat_t *)sl.phi1, *f2 = (const gfloat_t *)sl.phi2;
+    a1 = f1[q1];
+    a2 = f2[q2];
+    b1 = sl.order == 2 ? f1[q1 + o1] : 0.0f;
+    b2 = sl.order == 2 ? f2[q2 + o2] : 0.0f;
 }
 
 // Persistent grid: blocks are dealt to the 8 XCDs round robin, so block b is
