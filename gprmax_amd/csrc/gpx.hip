@@ -158,6 +158,7 @@ struct HLoad {
 struct ELoad {
     float hx, hy, hz, ex, ey, ez, hzj, hxj, hyk, hxk;
     uint32_t m0, m1, m2;
+    uint32_t s0, s1, s2;   // auxiliary-value slots of the three edges (dispersive models)
 };
 
 // update_magnetic (gprMax/fields_updates_ext.pyx:404-412), then each H-PML
@@ -382,16 +383,12 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
     const float *ls = reinterpret_cast<const float *>(lds4 + nm);
     const float *lR = ls + nm;
     const float *ldisp = lR + a.nR;
-    uint8_t *lflag = reinterpret_cast<uint8_t *>(const_cast<float *>(ldisp + nd));
     stage_coef(lds4, const_cast<float *>(ls), a.coef, nm);
     if (BND) {
         stage(const_cast<float *>(lR), a.R, a.nR);
         stage_slabs(lslab, a.slab, a.nslab);
     }
-    if (P > 0) {
-        stage(const_cast<float *>(ldisp), a.disp, nd);
-        for (int q = threadIdx.x; q < nm; q += blockDim.x) lflag[q] = a.dispflag[q];
-    }
+    if (P > 0) stage(const_cast<float *>(ldisp), a.disp, nd);
     __syncthreads();
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -434,6 +431,12 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
             L.m0 = bld_id<IDT>(rI0, tid, si);
             L.m1 = bld_id<IDT>(rI1, tid, si);
             L.m2 = bld_id<IDT>(rI2, tid, si);
+            if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
+                const size_t pe = (size_t)ip * Pl;
+                L.s0 = ld(a.T.slot[0] + pe, to);
+                L.s1 = ld(a.T.slot[1] + pe, to);
+                L.s2 = ld(a.T.slot[2] + pe, to);
+            }
         };
         ELoad cur, nxt;
         if (!BND && P == 0) load(c.xa, cur);
@@ -454,18 +457,22 @@ __device__ __forceinline__ void e_pass_run(const StepArgs &a, int first, int str
             const bool ry_ = !BND || (i >= 1 && i < nx && j < ny && k >= 1 && k < nz);
             const bool rz_ = !BND || (i >= 1 && i < nx && j >= 1 && j < ny && k < nz);
             // Auxiliary values change only on edges of materials with poles,
-            // so only those edges read and write them.
-            const bool dx_ = P > 0 && rx_ && lflag[m0];
-            const bool dy_ = P > 0 && ry_ && lflag[m1];
-            const bool dz_ = P > 0 && rz_ && lflag[m2];
+            // so only those edges read and write them.  Such an edge has a
+            // slot (T_NOSLOT otherwise): the slots load with the plane, not
+            // after the indices, so the auxiliary values follow one round
+            // trip later instead of two.
+            const bool td = P > 0 && (flags & (F_MAIN | F_DISPB));
+            const uint32_t sx = td ? cur.s0 : T_NOSLOT, sy = td ? cur.s1 : T_NOSLOT, sz = td ? cur.s2 : T_NOSLOT;
+            const bool dx_ = P > 0 && rx_ && sx != T_NOSLOT;
+            const bool dy_ = P > 0 && ry_ && sy != T_NOSLOT;
+            const bool dz_ = P > 0 && rz_ && sz != T_NOSLOT;
             float txr[P > 0 ? P : 1], txi[P > 0 ? P : 1];
             float tyr[P > 0 ? P : 1], tyi[P > 0 ? P : 1];
             float tzr[P > 0 ? P : 1], tzi[P > 0 ? P : 1];
-            uint32_t sx = 0, sy = 0, sz = 0;
-            if (P > 0 && (flags & (F_MAIN | F_DISPB))) {
-                if (dx_) sx = t_load<P, CPLX>(a.T, 0, po, to, txr, txi);
-                if (dy_) sy = t_load<P, CPLX>(a.T, 1, po, to, tyr, tyi);
-                if (dz_) sz = t_load<P, CPLX>(a.T, 2, po, to, tzr, tzi);
+            if (td) {
+                if (dx_) t_load_at<P, CPLX>(a.T, 0, sx, txr, txi);
+                if (dy_) t_load_at<P, CPLX>(a.T, 1, sy, tyr, tyi);
+                if (dz_) t_load_at<P, CPLX>(a.T, 2, sz, tzr, tzi);
             }
             // Differences of H used by the E update and the E-PML (backward).
             const float dHz_y = hz - hzj, dHy_z = hy - hyk, dHx_z = hx - hxk;

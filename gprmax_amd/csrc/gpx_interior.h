@@ -148,11 +148,43 @@ __device__ __forceinline__ void interior_run(const IntArgs &a, int item)
 
     // No software prefetch: loading the next plane ahead raised the VGPR count
     // from under 64 to over 90 (5 waves per SIMD) for no gain.
+    // Lanes that recompute a neighbour's new H load its inputs with the
+    // plane, so the loads overlap the plane's own instead of following the
+    // barrier.
+    const bool krec = tl == 0 || k == a.box[4];
     PL cur;
     for (int i = xa; i < xb; ++i) {
         const uint32_t so = (uint32_t)i * P4;
         const uint32_t pf = ld(a.planeflag, 4u * (uint32_t)i);
         load(i, cur);
+        float hxjm = 0.f, hzjm = 0.f, ezjm = 0.f, exjm = 0.f, eyjm = 0.f, eyjmk = 0.f, eyjm1 = 0.f;
+        uint32_t mj3 = 0, mj5 = 0;
+        if (jrec) {
+            const uint32_t sn = so + P4, si = (uint32_t)i * Pid;
+            hxjm = bld(rF, tjm, so + C(HX));
+            hzjm = bld(rF, tjm, so + C(HZ));
+            ezjm = bld(rF, tjm, so + C(EZ));
+            exjm = bld(rF, tjm, so + C(EX));
+            eyjm = bld(rF, tjm, so + C(EY));
+            eyjmk = bld(rF, tjm + 4u, so + C(EY));
+            eyjm1 = bld(rF, tjm, sn + C(EY));
+            mj3 = bld_id<IDT>(rI, tidjm, si + 3 * ci);
+            mj5 = bld_id<IDT>(rI, tidjm, si + 5 * ci);
+        }
+        float hx0k = 0.f, hy0k = 0.f, ezk = 0.f, ezjk = 0.f, eyk = 0.f, exk = 0.f, ez1k = 0.f;
+        uint32_t mk3 = 0, mk4 = 0;
+        if (krec) {
+            const uint32_t si = (uint32_t)i * Pid;
+            hx0k = bld(rF, tkm, so + C(HX));
+            hy0k = bld(rF, tkm, so + C(HY));
+            ezk = bld(rF, tkm, so + C(EZ));
+            ezjk = bld(rF, tkm + 4u * NZ1, so + C(EZ));
+            eyk = bld(rF, tkm, so + C(EY));
+            exk = bld(rF, tkm, so + C(EX));
+            ez1k = bld(rF, tkm, so + P4 + C(EZ));
+            mk3 = bld_id<IDT>(rI, tidkm, si + 3 * ci);
+            mk4 = bld_id<IDT>(rI, tidkm, si + 4 * ci);
+        }
 
         // new H of the cell
         float hx, hy, hz;
@@ -178,38 +210,26 @@ __device__ __forceinline__ void interior_run(const IntArgs &a, int item)
             hxj = lh[pb][0][tl - Kin];
             hzj = lh[pb][2][tl - Kin];
         } else {
-            const uint32_t sn = so + P4, si = (uint32_t)i * Pid;
-            const float hxjm = bld(rF, tjm, so + C(HX)), hzjm = bld(rF, tjm, so + C(HZ));
-            const float ezjm = bld(rF, tjm, so + C(EZ)), exjm = bld(rF, tjm, so + C(EX));
-            const float eyjm = bld(rF, tjm, so + C(EY)), eyjmk = bld(rF, tjm + 4u, so + C(EY));
-            const float eyjm1 = bld(rF, tjm, sn + C(EY));
-            const uint32_t m3 = bld_id<IDT>(rI, tidjm, si + 3 * ci), m5 = bld_id<IDT>(rI, tidjm, si + 5 * ci);
-            const float4 qx = cH[m3], qz = cH[m5];
+            const float4 qx = cH[mj3], qz = cH[mj5];
             hxj = qx.x * hxjm - qx.z * (ez - ezjm) + qx.w * (eyjmk - eyjm);
             hzj = qz.x * hzjm - qz.y * (eyjm1 - eyjm) + qz.z * (ex - exjm);
             if (pf & PF_SRCH) {
-                hxj = int_msrc(a, sH, i, j - 1, k, 0, m3, hxj, it);
-                hzj = int_msrc(a, sH, i, j - 1, k, 2, m5, hzj, it);
+                hxj = int_msrc(a, sH, i, j - 1, k, 0, mj3, hxj, it);
+                hzj = int_msrc(a, sH, i, j - 1, k, 2, mj5, hzj, it);
             }
         }
         // new Hx and Hy at k-1: from the previous lane, or recomputed
         float hxk, hyk;
-        if (!(tl == 0 || k == a.box[4])) {
+        if (!krec) {
             hxk = lh[pb][0][tl - 1];
             hyk = lh[pb][1][tl - 1];
         } else {
-            const float hx0 = bld(rF, tkm, so + C(HX)), hy0 = bld(rF, tkm, so + C(HY));
-            const float ezk = bld(rF, tkm, so + C(EZ)), ezjk = bld(rF, tkm + 4u * NZ1, so + C(EZ));
-            const float eyk = bld(rF, tkm, so + C(EY)), exk = bld(rF, tkm, so + C(EX));
-            const float ez1k = bld(rF, tkm, so + P4 + C(EZ));
-            const uint32_t si = (uint32_t)i * Pid;
-            const uint32_t m3 = bld_id<IDT>(rI, tidkm, si + 3 * ci), m4 = bld_id<IDT>(rI, tidkm, si + 4 * ci);
-            const float4 qx = cH[m3], qy = cH[m4];
-            hxk = qx.x * hx0 - qx.z * (ezjk - ezk) + qx.w * (ey - eyk);
-            hyk = qy.x * hy0 - qy.w * (ex - exk) + qy.y * (ez1k - ezk);
+            const float4 qx = cH[mk3], qy = cH[mk4];
+            hxk = qx.x * hx0k - qx.z * (ezjk - ezk) + qx.w * (ey - eyk);
+            hyk = qy.x * hy0k - qy.w * (ex - exk) + qy.y * (ez1k - ezk);
             if (pf & PF_SRCH) {
-                hxk = int_msrc(a, sH, i, j, k - 1, 0, m3, hxk, it);
-                hyk = int_msrc(a, sH, i, j, k - 1, 1, m4, hyk, it);
+                hxk = int_msrc(a, sH, i, j, k - 1, 0, mk3, hxk, it);
+                hyk = int_msrc(a, sH, i, j, k - 1, 1, mk4, hyk, it);
             }
         }
 

@@ -371,6 +371,27 @@ struct TArgs {
     long long n[3];
 };
 
+// Slot of an edge whose material has no poles.
+constexpr uint32_t T_NOSLOT = 0xffffffffu;
+
+// Loads the P values of component c in slot s.
+template <int P, bool CPLX>
+__device__ __forceinline__ void t_load_at(const TArgs &t, int c, uint32_t s, float *tr, float *ti)
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const size_t e = (size_t)p * (size_t)t.n[c] + s;
+        if (CPLX) {
+            const float2 v = ld(reinterpret_cast<const float2 *>(t.T[c]) + e, 0u);
+            tr[p] = v.x;
+            ti[p] = v.y;
+        } else {
+            tr[p] = ld((const float *)t.T[c] + e, 0u);
+            ti[p] = 0.f;
+        }
+    }
+}
+
 // Loads the P values of edge (c, po + to / 4) and returns its slot.
 template <int P, bool CPLX>
 __device__ __forceinline__ uint32_t t_load(const TArgs &t, int c, size_t po, uint32_t to, float *tr, float *ti)
