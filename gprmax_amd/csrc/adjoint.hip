@@ -445,12 +445,12 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             const int oc = cidx(a, i, j, k), CY = a.nz, CX = a.ny * a.nz;
             const float *CB = a.CB;
             auto XI = [&](int c, int off, int coff) { return CB[oc + coff] * a.E[c][o + off]; };
+            // Every load comes before the first store: the compiler cannot
+            // move a load above a store through these pointers, and a load
+            // after a store costs the point a second memory round trip.
             const float dh0 = XI(1, 0, 0) - XI(1, 1, 1) - XI(2, 0, 0) + XI(2, NZ1, CY);
             const float dh1 = XI(2, 0, 0) - XI(2, PL, CX) - XI(0, 0, 0) + XI(0, 1, 1);
             const float dh2 = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
-            a.H[0][o] += dh0;
-            a.H[1][o] += dh1;
-            a.H[2][o] += dh2;
             const float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
             const float CA = a.CA[oc];
             const float curl[3] = {
@@ -462,8 +462,13 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                 gca += lE[c] * a.Es[c][o];
                 gcb += lE[c] * curl[c];
             }
-            a.gCA[oc] += gca;
-            a.gCB[oc] += gcb;
+            const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
+            const float gca0 = a.gCA[oc], gcb0 = a.gCB[oc];
+            a.H[0][o] = h0 + dh0;
+            a.H[1][o] = h1 + dh1;
+            a.H[2][o] = h2 + dh2;
+            a.gCA[oc] = gca0 + gca;
+            a.gCB[oc] = gcb0 + gcb;
             float giea = 0.f;
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
             float out[3] = {CA * lE[0], CA * lE[1], CA * lE[2]};
@@ -497,18 +502,36 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             }
         }
         const int o = g;
-        a.H[0][o] += dh[0];
-        a.H[1][o] += dh[1];
-        a.H[2][o] += dh[2];
-
+        // Loads first (see the fast path), then the stores.
         float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
         float out[3] = {lE[0], lE[1], lE[2]};
+        const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
-        if (cell) {  // edges outside the cells are never updated
-            const int oc = cidx(a, i, j, k);
+        float gca = 0.f, gcb = 0.f, gca0 = 0.f, gcb0 = 0.f;
+        int oc = 0;
+        float CA = 0.f, IEA = 0.f;
+        if (cell) {
+            oc = cidx(a, i, j, k);
             const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o,
                                      k > 0 ? o - 1 : o};
-            const float CA = a.CA[oc], IEA = a.IEA[oc];
+            CA = a.CA[oc];
+            IEA = a.IEA[oc];
+            const float curl[3] = {
+                (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
+                (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
+                (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
+            for (int c = 0; c < 3; ++c) {
+                if (!e_upd(a, c, i, j, k)) continue;
+                gca += lE[c] * a.Es[c][o];
+                gcb += lE[c] * curl[c];
+            }
+            gca0 = a.gCA[oc];
+            gcb0 = a.gCB[oc];
+        }
+        a.H[0][o] = h0 + dh[0];
+        a.H[1][o] = h1 + dh[1];
+        a.H[2][o] = h2 + dh[2];
+        if (cell) {  // edges outside the cells are never updated
             for (int s = 0; s < a.nslab; ++s) {
                 const InvSlab &S = a.slab[s];
                 if (!in_e(S, i, j, k)) continue;
@@ -524,21 +547,15 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                     mo[q[w]] = RE0 * mu[q[w]] + (float)sign * IEA * RB0 * lE[comp];
                 }
             }
-            const float curl[3] = {
-                (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
-                (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
-                (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
-            float gca = 0.f, gcb = 0.f, giea = 0.f;
+            float giea = 0.f;
             for (int c = 0; c < 3; ++c) {
                 if (!e_upd(a, c, i, j, k)) continue;
-                gca += lE[c] * a.Es[c][o];
-                gcb += lE[c] * curl[c];
                 out[c] = CA * lE[c];
                 if (a.P > 0) disp_reverse(a, c, o, oc, IEA, lE[c], out[c], giea);
             }
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
-            a.gCA[oc] += gca;
-            a.gCB[oc] += gcb;
+            a.gCA[oc] = gca0 + gca;
+            a.gCB[oc] = gcb0 + gcb;
             if (giea != 0.f) a.gIEA[oc] += giea;
         }
         a.En[0][o] = out[0];
@@ -620,9 +637,12 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
                 de[src] += (h_pml_ldd(a, S, w, im, jm, km) - h_pml_ldd(a, S, w, i, j, k)) / a.d;
             }
         }
-        a.E[0][g] += de[0];
-        a.E[1][g] += de[1];
-        a.E[2][g] += de[2];
+        // Loads first (lH here and lE for the update), then the stores.
+        const float lH[3] = {a.H[0][g], a.H[1][g], a.H[2][g]};
+        const float e0 = a.E[0][g], e1 = a.E[1][g], e2 = a.E[2][g];
+        a.E[0][g] = e0 + de[0];
+        a.E[1][g] = e1 + de[1];
+        a.E[2][g] = e2 + de[2];
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
             if (!in_h(S, i, j, k)) continue;
@@ -635,7 +655,7 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
                 h_pair(S.axis, w, comp, src, sign);
                 const float *mu = w == 0 ? S.PH1 : S.PH2;
                 float *mo = w == 0 ? S.QH1 : S.QH2;
-                mo[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * a.H[comp][g];
+                mo[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * lH[comp];
             }
         }
     }
