@@ -823,7 +823,7 @@ struct gpx_ctx {
     size_t next_bytes = 0;    // bytes of the second state and of the second H-PML Phi copies
     long long fstride = 0;    // floats between components: ntot rounded up to 64
     void *ID = nullptr;       // [6][ntot] as uint8, uint16 or uint32
-    int id_bytes = 4;         // 2 when every index < nmat fits 16 bits, else 4
+    int id_bytes = 2;         // material indices are uint16 on the device (nmat < 2^16, see gpx_create)
     float *cE = nullptr, *cH = nullptr;
     float *Rall[2] = {nullptr, nullptr};  // E, H: all slabs' R tables back to back
     std::vector<float> Rhost[2];
@@ -971,7 +971,10 @@ extern "C" int gpx_create(const gpx_desc *d, gpx_ctx **out)
         ctx->F[c] = ctx->Fmem[0] + c * ctx->fstride;
         ctx->Fn[c] = nullptr;
     }
-    ctx->id_bytes = d->nmat <= 65536 ? 2 : 4;
+    // The coefficient tables must fit LDS (as the reference's must fit 64 KB
+    // of constant memory, gprMax/model_build_run.py:512-516), which bounds
+    // nmat far below 2^16: every material index fits 16 bits.
+    ctx->id_bytes = 2;
     e = dalloc(ctx, (void **)&ctx->ID, 6 * (size_t)ctx->ntot * ctx->id_bytes, false);
     if (e != hipSuccess) goto fail_alloc;
     e = dalloc(ctx, (void **)&ctx->cE, (size_t)d->nmat * 5 * 4);
@@ -1176,12 +1179,8 @@ static int ensure_tslots(gpx_ctx *ctx)
             hipError_t e = dalloc(ctx, (void **)&ctx->tslot[c], (size_t)N * 4, false);
             if (e != hipSuccess) { rc = fail_hip(ctx, e, "slot array"); break; }
         }
-        if (ctx->id_bytes == 2)
-            hipLaunchKernelGGL(k_tslot_count<uint16_t>, dim3(nb), dim3(256), 0, ctx->stream,
-                               (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff);
-        else
-            hipLaunchKernelGGL(k_tslot_count<uint32_t>, dim3(nb), dim3(256), 0, ctx->stream,
-                               (const uint32_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff);
+        hipLaunchKernelGGL(k_tslot_count<uint16_t>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff);
         if ((rc = copy_out(ctx, h.data(), boff, (size_t)nb * 4))) break;
         unsigned long long acc = 0;
         for (int b = 0; b < nb; ++b) {
@@ -1190,12 +1189,8 @@ static int ensure_tslots(gpx_ctx *ctx)
             acc += v;
         }
         if ((rc = copy_in(ctx, boff, h.data(), (size_t)nb * 4))) break;
-        if (ctx->id_bytes == 2)
-            hipLaunchKernelGGL(k_tslot_fill<uint16_t>, dim3(nb), dim3(1024), 0, ctx->stream,
-                               (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff, ctx->tslot[c]);
-        else
-            hipLaunchKernelGGL(k_tslot_fill<uint32_t>, dim3(nb), dim3(1024), 0, ctx->stream,
-                               (const uint32_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff, ctx->tslot[c]);
+        hipLaunchKernelGGL(k_tslot_fill<uint16_t>, dim3(nb), dim3(1024), 0, ctx->stream,
+                           (const uint16_t *)ctx->ID + (size_t)c * N, ctx->dispflag, N, boff, ctx->tslot[c]);
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) { rc = fail_hip(ctx, e, "slot build"); break; }
         ctx->tn[c] = (long long)acc;
@@ -1224,13 +1219,8 @@ extern "C" int gpx_upload_id(gpx_ctx *ctx, const uint32_t *ID)
             return fail(ctx, GPX_EARG, "gpx_upload_id: material index " + std::to_string(ID[q]) +
                                            " >= nmat " + std::to_string(nmat));
     hipSetDevice(ctx->desc.device);
-    int rc;
-    if (ctx->id_bytes == 4) {
-        rc = copy_in(ctx, ctx->ID, ID, n * 4);
-    } else {
-        std::vector<uint16_t> c(ID, ID + n);
-        rc = copy_in(ctx, ctx->ID, c.data(), n * 2);
-    }
+    std::vector<uint16_t> c(ID, ID + n);   // every index < nmat < 2^16
+    int rc = copy_in(ctx, ctx->ID, c.data(), n * 2);
     if (!rc) {
         ctx->have_id = true;
         ctx->tslot_dirty = true;
@@ -1795,10 +1785,10 @@ template <bool BND>
 static pass_kernel pick_kernel(const gpx_ctx *ctx, int field)
 {
     if (field == 1)
-        return ctx->id_bytes == 2 ? k_h_pass<uint16_t, BND> : k_h_pass<uint32_t, BND>;
+        return k_h_pass<uint16_t, BND>;
     const int P = ctx->desc.maxpoles;
     const bool C = ctx->disp_complex != 0;
-    return ctx->id_bytes == 2 ? e_kernel<uint16_t, BND>(P, C) : e_kernel<uint32_t, BND>(P, C);
+    return e_kernel<uint16_t, BND>(P, C);
 }
 
 typedef void (*many_kernel)(const StepArgs *, PassMany);
@@ -1819,10 +1809,10 @@ template <bool BND>
 static many_kernel pick_many(const gpx_ctx *ctx, int field)
 {
     if (field == 1)
-        return ctx->id_bytes == 2 ? k_h_pass_many<uint16_t, BND> : k_h_pass_many<uint32_t, BND>;
+        return k_h_pass_many<uint16_t, BND>;
     const int P = ctx->desc.maxpoles;
     const bool C = ctx->disp_complex != 0;
-    return ctx->id_bytes == 2 ? e_many<uint16_t, BND>(P, C) : e_many<uint32_t, BND>(P, C);
+    return e_many<uint16_t, BND>(P, C);
 }
 
 // One pass = interior kernel (skipped when only PML work is asked for) then
@@ -1887,10 +1877,10 @@ static pass_kernel e_merged(int P, bool C)
 
 static pass_kernel pick_merged(const gpx_ctx *ctx, int field)
 {
-    if (field == 1) return ctx->id_bytes == 2 ? k_h_merged<uint16_t> : k_h_merged<uint32_t>;
+    if (field == 1) return k_h_merged<uint16_t>;
     const int P = ctx->desc.maxpoles;
     const bool C = ctx->disp_complex != 0;
-    return ctx->id_bytes == 2 ? e_merged<uint16_t>(P, C) : e_merged<uint32_t>(P, C);
+    return e_merged<uint16_t>(P, C);
 }
 
 // Items of the merged pass with chunks of xc planes (both kinds), and the
@@ -2199,7 +2189,7 @@ static hipError_t launch_interior(const gpx_ctx *ctx, int it)
     interior_args(ctx, it, a);
     const int B = INT_BLOCK;
     const int nxb = a.box[1] - a.box[0];
-    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior<uint16_t> : (const void *)k_interior<uint32_t>;
+    const void *k = (const void *)k_interior<uint16_t>;
     const size_t lds = (size_t)ctx->desc.nmat * 40;
     int xc = ctx->xchunk ? ctx->xchunk : 4;
     if (!ctx->xchunk) {
@@ -2211,8 +2201,7 @@ static hipError_t launch_interior(const gpx_ctx *ctx, int it)
     a.nitems = a.nseg * ((nxb + xc - 1) / xc);
     const int grid = (a.nitems + 7) / 8 * 8;
     a.per_xcd = grid / 8;
-    if (ctx->id_bytes == 2) hipLaunchKernelGGL(k_interior<uint16_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
-    else hipLaunchKernelGGL(k_interior<uint32_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
+    hipLaunchKernelGGL(k_interior<uint16_t>, dim3(grid), dim3(B), lds, ctx->stream, a);
     return hipGetLastError();
 }
 
@@ -2227,7 +2216,7 @@ static hipError_t launch_interior_hb(const gpx_ctx *ctx, int hflags, int it)
     size_t lds_b;
     base_args(ctx, 1, hb, block, lds_b, true);
     const int nxb = a.box[1] - a.box[0];
-    const void *k = ctx->id_bytes == 2 ? (const void *)k_interior_hb<uint16_t> : (const void *)k_interior_hb<uint32_t>;
+    const void *k = (const void *)k_interior_hb<uint16_t>;
     const size_t lds_i = (size_t)ctx->desc.nmat * 40;
     const size_t lds = lds_i > lds_b ? lds_i : lds_b;
     auto total = [&](int xc) {
@@ -2251,10 +2240,7 @@ static hipError_t launch_interior_hb(const gpx_ctx *ctx, int hflags, int it)
     hb.per_xcd = grid / 8;
     hb.flags = hflags;
     hb.it = it;
-    if (ctx->id_bytes == 2)
-        hipLaunchKernelGGL(k_interior_hb<uint16_t>, dim3(grid), dim3(INT_BLOCK), lds, ctx->stream, a, hb);
-    else
-        hipLaunchKernelGGL(k_interior_hb<uint32_t>, dim3(grid), dim3(INT_BLOCK), lds, ctx->stream, a, hb);
+    hipLaunchKernelGGL(k_interior_hb<uint16_t>, dim3(grid), dim3(INT_BLOCK), lds, ctx->stream, a, hb);
     return hipGetLastError();
 }
 

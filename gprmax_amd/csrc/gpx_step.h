@@ -921,21 +921,13 @@ StepKernels step_kernel_table(int id_bytes, int P, bool C)
     constexpr int K = STEP_CELLS_PER_THREAD;
 #define GPX_SK(IDT, PP, CC) \
     StepKernels{k_step<IDT, PP, CC, FORM, ORDER, K>, k_step_batch<IDT, PP, CC, FORM, ORDER, K>}
-    if (id_bytes == 2) {
-        switch (P) {
-        case 0: return GPX_SK(uint16_t, 0, false);
-        case 1: return C ? GPX_SK(uint16_t, 1, true) : GPX_SK(uint16_t, 1, false);
-        case 2: return C ? GPX_SK(uint16_t, 2, true) : GPX_SK(uint16_t, 2, false);
-        case 3: return C ? GPX_SK(uint16_t, 3, true) : GPX_SK(uint16_t, 3, false);
-        default: return C ? GPX_SK(uint16_t, 4, true) : GPX_SK(uint16_t, 4, false);
-        }
-    }
+    (void)id_bytes;  // material indices are always uint16 on the device
     switch (P) {
-    case 0: return GPX_SK(uint32_t, 0, false);
-    case 1: return C ? GPX_SK(uint32_t, 1, true) : GPX_SK(uint32_t, 1, false);
-    case 2: return C ? GPX_SK(uint32_t, 2, true) : GPX_SK(uint32_t, 2, false);
-    case 3: return C ? GPX_SK(uint32_t, 3, true) : GPX_SK(uint32_t, 3, false);
-    default: return C ? GPX_SK(uint32_t, 4, true) : GPX_SK(uint32_t, 4, false);
+    case 0: return GPX_SK(uint16_t, 0, false);
+    case 1: return C ? GPX_SK(uint16_t, 1, true) : GPX_SK(uint16_t, 1, false);
+    case 2: return C ? GPX_SK(uint16_t, 2, true) : GPX_SK(uint16_t, 2, false);
+    case 3: return C ? GPX_SK(uint16_t, 3, true) : GPX_SK(uint16_t, 3, false);
+    default: return C ? GPX_SK(uint16_t, 4, true) : GPX_SK(uint16_t, 4, false);
     }
 #undef GPX_SK
 }
