@@ -212,12 +212,13 @@ __global__ void k_inv_h(InvArgs a)
             const float RA01 = S.RH[p] - 1.0f, RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p],
                         RF0 = S.RH[3 * S.t + p];
             const int on = S.axis == 0 ? oi : (S.axis == 1 ? oj : ok);
+            const float ph2[2] = {S.PH1[q[0]], S.PH2[q[1]]};  // both loads before either store
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
                 h_pair(S.axis, w, comp, src, sign);
                 float *Phi = w == 0 ? S.PH1 : S.PH2;
                 const float dd = (a.E[src][on] - a.E[src][o]) / a.d;
-                const float ph = Phi[q[w]];
+                const float ph = ph2[w];
                 const float corr = a.cH * (RA01 * dd + RB0 * ph);
                 Phi[q[w]] = RE0 * ph - RF0 * dd;
                 h[comp] = sign > 0 ? h[comp] + corr : h[comp] - corr;
@@ -254,11 +255,7 @@ __global__ void k_inv_e(InvArgs a)
         const int o = g;
         const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
         float e[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
-        if (a.Es[0]) {
-            a.Es[0][o] = e[0];
-            a.Es[1][o] = e[1];
-            a.Es[2][o] = e[2];
-        }
+        const float en[3] = {e[0], e[1], e[2]};  // E^n, saved at the end (stores after every load)
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
         const int oc = cell ? cidx(a, i, j, k) : 0;
         const float CA = cell ? a.CA[oc] : 0.f, CB = cell ? a.CB[oc] : 0.f, IEA = cell ? a.IEA[oc] : 0.f;
@@ -280,8 +277,6 @@ __global__ void k_inv_e(InvArgs a)
                 }
                 phi[c] = ph;
             }
-            if (a.Ps[0])
-                for (int c = 0; c < 3; ++c) a.Ps[c][o] = phi[c];
         }
         if (upd[0])
             e[0] = CA * e[0] + CB * (a.H[2][o] - a.H[2][om[1]]) - CB * (a.H[1][o] - a.H[1][om[2]]);
@@ -300,12 +295,13 @@ __global__ void k_inv_e(InvArgs a)
             pidx(S, true, i, j, k, p, q[0], q[1]);
             const float RA01 = S.RE[p] - 1.0f, RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p],
                         RF0 = S.RE[3 * S.t + p];
+            const float ph2[2] = {S.PE1[q[0]], S.PE2[q[1]]};  // both loads before either store
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
                 e_pair(S.axis, w, comp, src, sign);
                 float *Phi = w == 0 ? S.PE1 : S.PE2;
                 const float dd = (a.H[src][o] - a.H[src][om[S.axis]]) / a.d;
-                const float ph = Phi[q[w]];
+                const float ph = ph2[w];
                 const float corr = IEA * (RA01 * dd + RB0 * ph);
                 Phi[q[w]] = RE0 * ph - RF0 * dd;
                 e[comp] = sign > 0 ? e[comp] + corr : e[comp] - corr;
@@ -324,6 +320,13 @@ __global__ void k_inv_e(InvArgs a)
         a.E[0][o] = e[0];
         a.E[1][o] = e[1];
         a.E[2][o] = e[2];
+        if (a.Es[0]) {
+            a.Es[0][o] = en[0];
+            a.Es[1][o] = en[1];
+            a.Es[2][o] = en[2];
+        }
+        if (disp && a.Ps[0])
+            for (int c = 0; c < 3; ++c) a.Ps[c][o] = phi[c];
     }
 }
 
