@@ -41,15 +41,21 @@ def randomise_tables(G, seed, scaled=False):
 
 def rich_model(seed=0, shape=(26, 22, 19), iterations=40, pml=4, order=1, formulation='HORIPML',
                poles=0, kind='debye', nmat=9, snapshots=True, cell=(1e-3, 1e-3, 1e-3)):
+    """`pml` is the thickness of every face, or a dict of per-face
+    thicknesses (x0, y0, z0, xmax, ymax, zmax; a missing or zero face is
+    open), as #pml_cells gives them."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = shape
+    faces = pml if isinstance(pml, dict) else None
+    if faces is not None:
+        pml = max(faces.values(), default=0)
     cfs = None
     if order == 2:
         a, b = CFS(), CFS()
         b.alpha.max = 0.05
         b.kappa.scalingprofile, b.kappa.min, b.kappa.max = 'quadratic', 1, 3
         cfs = [a, b]
-    bld = ModelBuilder(nx, ny, nz, *cell, iterations, pml_cells=pml,
+    bld = ModelBuilder(nx, ny, nz, *cell, iterations, pml_cells=faces if faces is not None else pml,
                        pml_formulation=formulation, cfs=cfs)
     mats = []
     for n in range(nmat):

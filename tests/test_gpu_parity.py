@@ -275,6 +275,39 @@ def test_run_from_random_state_anisotropic_random_tables(case, fused):
             assert np.array_equal(buf[c, 2:10, r], rx.outputs[name][2:10]), (r, name)
 
 
+# Per-face PML thicknesses, some faces open (#pml_cells x0 y0 z0 xmax ymax zmax).
+FACES = {
+    'uneven': {'x0': 3, 'y0': 5, 'z0': 2, 'xmax': 6, 'ymax': 4, 'zmax': 7},
+    'open': {'x0': 0, 'y0': 4, 'z0': 5, 'xmax': 3, 'ymax': 0, 'zmax': 0},
+    'none': {},
+}
+
+
+@pytest.mark.parametrize('fused', **FORMS)
+@pytest.mark.parametrize('faces', list(FACES))
+@pytest.mark.parametrize('case', ['horipml2', 'mripml1', 'debye3'])
+def test_run_from_random_state_uneven_pml(case, faces, fused):
+    """Every step form from a random state on a grid whose PML has a
+    different thickness on every face, some faces open, or no PML at all:
+    the interior box, the boundary frame and the merged block order follow
+    the slabs, and state and receiver samples equal the oracle's."""
+    G = rich_model(seed=41, shape=WIDE, iterations=12, snapshots=False, pml=FACES[faces], **CASES[case])
+    assert len(G.pmls) == sum(1 for v in FACES[faces].values() if v > 0)
+    _random_state(G, 43)
+    lib = oracle.load(np.float32)
+    with _hip()(G) as s:
+        s.set_step(fused)
+        _push_state(s, G)
+        s.run(2, 8)
+        for it in range(2, 10):
+            osolver.step(lib, G, it, 8)
+        _assert_state_equal(s, G, 'form {} faces {}'.format(fused, faces))
+        buf = s.download_rx()
+    for r, rx in enumerate(G.rxs):
+        for c, name in enumerate(COMPS):
+            assert np.array_equal(buf[c, 2:10, r], rx.outputs[name][2:10]), (r, name)
+
+
 PLANE_CASES = {
     'horipml1': dict(order=1),
     'mripml2': dict(order=2, formulation='MRIPML'),
