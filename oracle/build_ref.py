@@ -5,7 +5,8 @@ mounted, to pin the C restatement in ``oracle/fdtd_oracle.c`` and the host
 edge build bit for bit, and to time the reference's CPU kernels beside the C
 restatement.  Nothing here is copied: the .pyx sources are compiled where
 they lie; the generated C and the built modules go to ``oracle/_ref/``
-(git-ignored, and listed in .gpurunignore: it never travels to a GPU host).
+(git-ignored and never copied to another host: the GPU tests and the
+benchmark use only the C restatement).
 
 Two builds are made:
   exact/   -O3 -fopenmp -ffp-contract=off   (every product rounded, no FMA)

@@ -25,7 +25,7 @@ def test_header_and_binding_agree():
 def test_library_exports_every_declared_symbol():
     from gprmax_amd.hip import LIBRARY, library
     if not os.path.exists(LIBRARY):
-        pytest.fail('libgprmax_hip.so is not built (run __graft_entry__.build())')
+        pytest.fail('libgprmax_hip.so is not built (python -m gprmax_amd.build)')
     lib = library()
     for name in declared_functions():
         assert hasattr(lib, name), name
