@@ -108,7 +108,33 @@ struct InvArgs {
     int fast[6];
     int shell[6][6];                     // [lo x, y, z, hi x, y, z] of the shell boxes
     int mix_nf, mix_ns, mix_gf[2], mix_gs;  // block counts and grid sizes of the mixed launches
+    // Launches over every grid point: all_gk blocks of 128 points along k per
+    // (j, i) row, all_blocks blocks, padded to a multiple of 8 (all_per_xcd x 8).
+    int all_gk, all_blocks, all_per_xcd;
 };
+
+// One-dimensional launches: the hardware deals block b to XCD b mod 8, so block
+// b takes logical number (b mod 8) * per_xcd + b / 8 (per_xcd = launched blocks
+// / 8).  Each XCD then works through a contiguous run of logical blocks, i.e.
+// of neighbouring grid rows, and the rows a point's differences read (j +- 1,
+// i +- 1) are in the same XCD's L2 instead of another's.
+__device__ __forceinline__ int xcd_block(int per_xcd)
+{
+    const int b = blockIdx.x;
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+// Point (i, j, k) of a launch over every grid point; false past the grid.
+__device__ __forceinline__ bool all_point(const InvArgs &a, int &i, int &j, int &k)
+{
+    const int lb = xcd_block(a.all_per_xcd);
+    if (lb >= a.all_blocks) return false;
+    const int kb = lb % a.all_gk, r = lb / a.all_gk;
+    k = kb * blockDim.x + threadIdx.x;
+    j = r % (a.ny + 1);
+    i = r / (a.ny + 1);
+    return true;
+}
 
 // Grid offsets are 32-bit: gpx_inv_create requires fewer than 2^31 points.
 __device__ __forceinline__ int fidx(const InvArgs &a, int i, int j, int k)
@@ -191,9 +217,8 @@ __device__ __forceinline__ int axis_of(const InvSlab &s, int i, int j, int k)
 __global__ void k_inv_h(InvArgs a)
 {
     {
-        // launched over (k blocks, j, i): no index division
-        const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
-        if (k > a.nz) return;
+        int i, j, k;
+        if (!all_point(a, i, j, k) || k > a.nz) return;
         const int g = fidx(a, i, j, k);
         const int o = g, oi = fidx(a, i + 1, j, k), oj = fidx(a, i, j + 1, k), ok = o + 1;
         float h[3] = {a.H[0][o], a.H[1][o], a.H[2][o]};
@@ -248,9 +273,8 @@ __global__ void k_inv_rx(InvArgs a)
 __global__ void k_inv_e(InvArgs a)
 {
     {
-        // launched over (k blocks, j, i): no index division
-        const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
-        if (k > a.nz) return;
+        int i, j, k;
+        if (!all_point(a, i, j, k) || k > a.nz) return;
         const int g = fidx(a, i, j, k);
         const int o = g;
         const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
@@ -390,8 +414,8 @@ __device__ __forceinline__ void disp_reverse(const InvArgs &a, int c, int o, int
 // T^{n+1}).  Runs before the E gathers of the same iteration.
 __global__ void k_adj_disp(InvArgs a)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
-    if (k >= a.nz || j >= a.ny || i >= a.nx) return;
+    int i, j, k;
+    if (!all_point(a, i, j, k) || k >= a.nz || j >= a.ny || i >= a.nx) return;
     const int o = fidx(a, i, j, k), oc = cidx(a, i, j, k);
     for (int c = 0; c < 3; ++c) {
         if (!e_upd(a, c, i, j, k)) continue;
@@ -421,13 +445,15 @@ template <bool FAST>
 __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, int bz)
 {
     {
-        // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
+        // launched over (segments of a (j, k) plane, i) of the fast box (FAST),
+        // every lane a point of the plane, or over the shell boxes
         int i, j, k;
         if (FAST) {
-            k = a.fast[4] + bx * blockDim.x + threadIdx.x;
-            j = a.fast[2] + by;
-            i = a.fast[0] + bz;
-            if (k >= a.fast[5]) return;
+            const int fz = a.fast[5] - a.fast[4], t = bx * blockDim.x + threadIdx.x;
+            if (t >= (a.fast[3] - a.fast[2]) * fz) return;
+            k = a.fast[4] + t % fz;
+            j = a.fast[2] + t / fz;
+            i = a.fast[0] + by;
         } else {
             // the shell around the fast box, as up to six disjoint boxes
             // (blockIdx.y), each enumerated flat so every lane has a point
@@ -590,13 +616,15 @@ template <bool FAST>
 __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, int bz)
 {
     {
-        // launched over (k blocks, j, i) of the fast box (FAST) or of the grid
+        // launched over (segments of a (j, k) plane, i) of the fast box (FAST),
+        // every lane a point of the plane, or over the shell boxes
         int i, j, k;
         if (FAST) {
-            k = a.fast[4] + bx * blockDim.x + threadIdx.x;
-            j = a.fast[2] + by;
-            i = a.fast[0] + bz;
-            if (k >= a.fast[5]) return;
+            const int fz = a.fast[5] - a.fast[4], t = bx * blockDim.x + threadIdx.x;
+            if (t >= (a.fast[3] - a.fast[2]) * fz) return;
+            k = a.fast[4] + t % fz;
+            j = a.fast[2] + t / fz;
+            i = a.fast[0] + by;
         } else {
             // the shell around the fast box, as up to six disjoint boxes
             // (blockIdx.y), each enumerated flat so every lane has a point
@@ -985,6 +1013,9 @@ static int n_checkpoints(const gpx_inv *v) { return (v->desc.iterations + v->des
 static long long ck_stride(const gpx_inv *v) { return 6 * v->ntot + 3 * v->P * v->ntot + v->phi_total; }
 static long long seg_stride(const gpx_inv *v) { return (v->P ? 9 : 6) * v->ntot; }
 
+static int all_gk(const gpx_inv *v);
+static int all_blocks(const gpx_inv *v);
+
 static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
 {
     InvArgs a;
@@ -1031,18 +1062,32 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     for (int c = 0; c < 3 && v->P; ++c) a.T[c] = (adjoint ? v->LT : v->T) + (size_t)c * v->P * v->ntot;
     memcpy(a.fast, v->fast, sizeof(a.fast));
     memcpy(a.shell, v->shell, sizeof(a.shell));
+    a.all_gk = all_gk(v);
+    a.all_blocks = all_blocks(v);
+    a.all_per_xcd = (a.all_blocks + 7) / 8;
     return a;
 }
 
-// Launch shapes: 128 threads along k, one block row per (j, i).
+// Launch shapes: 128 threads along k, one block row per (j, i).  Launches over
+// every point are one-dimensional (all_point): all_gk blocks per row, the
+// block count padded to a multiple of 8.
+static int all_gk(const gpx_inv *v)
+{
+    return (v->desc.nz + 1 + 127) / 128;
+}
+static int all_blocks(const gpx_inv *v)
+{
+    return all_gk(v) * (v->desc.ny + 1) * (v->desc.nx + 1);
+}
 static dim3 grid_all(const gpx_inv *v)
 {
-    return dim3((v->desc.nz + 1 + 127) / 128, v->desc.ny + 1, v->desc.nx + 1);
+    return dim3((unsigned)((all_blocks(v) + 7) / 8 * 8));
 }
 static dim3 grid_fast(const gpx_inv *v)
 {
     const int fx = v->fast[1] - v->fast[0], fy = v->fast[3] - v->fast[2], fz = v->fast[5] - v->fast[4];
-    return dim3(fz > 0 ? (fz + 127) / 128 : 1, fy > 0 ? fy : 1, fx > 0 ? fx : 1);
+    const long long plane = fy > 0 && fz > 0 ? (long long)fy * fz : 1;
+    return dim3((unsigned)((plane + 127) / 128), fx > 0 ? fx : 1, 1);
 }
 static dim3 grid_shell(const gpx_inv *v)
 {
