@@ -204,8 +204,11 @@ def bench_adjoint(args):
     homogeneous start model, 8 z-directed Hertzian sources x 16 Ez receivers,
     L2 trace misfit, checkpoint interval K = 64 over 2000 iterations.
     Reports the cost of the gradient (forward + recompute + adjoint) against
-    one forward, and, with --fd-cells, central differences of the float32
-    forward at that many random interior cells (first source only)."""
+    one forward; then (--adj-keep 1) the same gradient with the forward
+    keeping every segment's stored states (about 154 GB of device memory),
+    which leaves forward + adjoint; and, with --fd-cells, central differences
+    of the float32 forward at that many random interior cells (first source
+    only)."""
     import numpy as np
     from gprmax_amd.inversion import InversionGrid, misfit_and_gradient
 
@@ -245,6 +248,23 @@ def bench_adjoint(args):
         'gradient_wall_s_all_sources': round(wall, 2), 'loss': loss,
         'grad_norm_er': float(np.linalg.norm(ger)), 'grad_norm_se': float(np.linalg.norm(gse)),
     }
+    if args.adj_keep:
+        # The same gradient with the forward keeping every segment's stored
+        # states in device memory, so the backward recomputes nothing.
+        nbytes = grid.keep_bytes_all()
+        grid.set_keep(nbytes)
+        t0 = time.perf_counter()
+        loss_k, ger_k, gse_k = misfit_and_gradient(grid, er0, se0, sources, receivers, observed, waves)
+        wall_k = time.perf_counter() - t0
+        ms_fk, ms_bk = grid.timings()
+        grid.set_keep(0)
+        line['kept_states'] = {
+            'device_gb': round(nbytes / 1e9, 1),
+            'forward_ms': round(ms_fk, 2), 'backward_ms': round(ms_bk, 2),
+            'ratio_to_plain_forward': round((ms_fk + ms_bk) / ms_f, 3),
+            'gradient_wall_s_all_sources': round(wall_k, 2),
+            'gradient_identical': bool(loss_k == loss and np.array_equal(ger_k, ger) and np.array_equal(gse_k, gse)),
+        }
     if args.fd_cells:
         idx = np.argwhere(grid.interior)
         picks = idx[rng.choice(len(idx), args.fd_cells, replace=False)]
@@ -325,6 +345,8 @@ def main():
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
     ap.add_argument('--fd-cells', type=int, default=0, help='finite-difference check cells (adjoint workload)')
+    ap.add_argument('--adj-keep', type=int, default=1,
+                    help='adjoint workload: also time the gradient with every segment kept (1) or not (0)')
     ap.add_argument('--calibrate', action='store_true',
                     help='only run a device copy of --calibrate-mib MiB (counter calibration under rocprofv3)')
     ap.add_argument('--calibrate-mib', type=int, default=1024)

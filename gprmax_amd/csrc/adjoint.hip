@@ -771,6 +771,13 @@ struct gpx_inv {
     int shell[6][6] = {};              // see InvArgs
     int shell_max = 0;                 // points of the largest shell box
     float ms_fwd = 0, ms_bwd = 0;
+    // Segments whose stored states the forward keeps (gpx_inv_set_keep): the
+    // last nkeep segments, each K x seg_stride floats, in `keep`; the backward
+    // then skips their recompute.
+    double keep_bytes = 0;
+    float *keep = nullptr;
+    long long keep_floats = 0;     // size of `keep`
+    int nkeep = 0;                 // segments kept by the last forward
     std::string err;
 };
 
@@ -871,7 +878,7 @@ extern "C" int gpx_inv_destroy(gpx_inv *v)
     hipSetDevice(v->desc.device);
     if (v->stream) hipStreamSynchronize(v->stream);
     for (float *p : {v->F, v->L, v->L2, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid, v->dc,
-                     v->T, v->LT})
+                     v->T, v->LT, v->keep})
         if (p) hipFree(p);
     if (v->rxc) hipFree(v->rxc);
     for (auto &S : v->slabs) {
@@ -1120,9 +1127,16 @@ static int checkpoint(gpx_inv *v, int c, bool save)
     return 0;
 }
 
-static void forward_step(gpx_inv *v, int it, bool rx)
+// One forward iteration; with `slot`, the kernels also store E^n, H^{n+1}
+// (and phi^n) there, as the recompute does.
+static void forward_step(gpx_inv *v, int it, bool rx, float *slot = nullptr)
 {
     InvArgs a = inv_args(v, false, it);
+    for (int q = 0; slot && q < 3; ++q) {
+        a.Es[q] = slot + q * v->ntot;
+        a.Hs[q] = slot + (3 + q) * v->ntot;
+        if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
+    }
     if (rx && v->nrx) hipLaunchKernelGGL(k_inv_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
     hipLaunchKernelGGL(k_inv_h, grid_all(v), dim3(128), 0, v->stream, a);
     hipLaunchKernelGGL(k_inv_e, grid_all(v), dim3(128), 0, v->stream, a);
@@ -1158,6 +1172,30 @@ extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, c
     }
     if (nrx) INV_TRY(hipMemcpy(v->rxc, rx, (size_t)nrx * 3 * 4, hipMemcpyHostToDevice));
     if (!v->ck) INV_TRY(hipMalloc((void **)&v->ck, (size_t)n_checkpoints(v) * ck_stride(v) * 4));
+    // the segments whose states this forward keeps
+    const int nseg = n_checkpoints(v);
+    const long long seg_floats = (long long)g.K * seg_stride(v);
+    const double fit = v->keep_bytes / ((double)seg_floats * 4.0);
+    const int nkeep = fit >= nseg ? nseg : (int)fit;
+    if ((long long)nkeep * seg_floats != v->keep_floats) {
+        if (v->keep) hipFree(v->keep);
+        v->keep = nullptr;
+        v->keep_floats = 0;
+        v->nkeep = 0;
+        v->have_forward = false;
+        if (nkeep > 0) {
+            const hipError_t e = hipMalloc((void **)&v->keep, (size_t)nkeep * seg_floats * 4);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                v->keep = nullptr;
+                return ifail(v, (int)e, "gpx_inv_forward: cannot allocate " +
+                                              std::to_string((double)nkeep * seg_floats * 4 / 1e9) +
+                                              " GB for the kept segments (gpx_inv_set_keep)");
+            }
+            v->keep_floats = (long long)nkeep * seg_floats;
+        }
+    }
+    v->nkeep = nkeep;
     INV_TRY(hipMemsetAsync(v->F, 0, 6 * v->ntot * 4, v->stream));
     if (v->P) INV_TRY(hipMemsetAsync(v->T, 0, (size_t)3 * v->P * v->ntot * 4, v->stream));
     for (auto &S : v->slabs)
@@ -1167,8 +1205,15 @@ extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, c
     INV_TRY(hipEventCreate(&e1));
     INV_TRY(hipEventRecord(e0, v->stream));
     for (int it = 0; it < g.iterations; ++it) {
+        const int c = it / g.K;
+        if (c >= nseg - nkeep) {  // a kept segment: its states, no checkpoint
+            float *slot = v->keep + (long long)(c - (nseg - nkeep)) * seg_floats +
+                          (long long)(it - c * g.K) * seg_stride(v);
+            forward_step(v, it, true, slot);
+            continue;
+        }
         if (it % g.K == 0)
-            if (int rc = checkpoint(v, it / g.K, true)) return rc;
+            if (int rc = checkpoint(v, c, true)) return rc;
         forward_step(v, it, true);
     }
     INV_TRY(hipGetLastError());
@@ -1206,12 +1251,16 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     INV_TRY(hipEventRecord(e0, v->stream));
     const dim3 G = grid_all(v), GF = grid_fast(v), GS = grid_shell(v);
     const bool nofast = fast_empty(v);
-    for (int c = n_checkpoints(v) - 1; c >= 0; --c) {
+    const int nseg = n_checkpoints(v);
+    for (int c = nseg - 1; c >= 0; --c) {
         const int n0 = c * g.K, n1 = n0 + g.K < g.iterations ? n0 + g.K : g.iterations;
-        // recompute the segment, storing E^n and H^{n+1}
-        if (int rc = checkpoint(v, c, false)) return rc;
-        for (int it = n0; it < n1; ++it) {
-            float *slot = v->seg + (long long)(it - n0) * seg_stride(v);
+        const bool kept = c >= nseg - v->nkeep;
+        float *base = kept ? v->keep + (long long)(c - (nseg - v->nkeep)) * g.K * seg_stride(v) : v->seg;
+        // recompute the segment, storing E^n and H^{n+1} (kept segments have them)
+        if (!kept)
+            if (int rc = checkpoint(v, c, false)) return rc;
+        for (int it = n0; it < n1 && !kept; ++it) {
+            float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, false, it);
             for (int q = 0; q < 3; ++q) {  // the kernels save E^n, H^{n+1} (and phi^n) here
                 a.Es[q] = slot + q * v->ntot;
@@ -1223,7 +1272,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
         }
         // adjoint sweep over the segment
         for (int it = n1 - 1; it >= n0; --it) {
-            float *slot = v->seg + (long long)(it - n0) * seg_stride(v);
+            float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, true, it);
             for (int q = 0; q < 3; ++q) {
                 a.Es[q] = slot + q * v->ntot;
@@ -1268,6 +1317,13 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     INV_TRY(hipMemcpy(gCA, v->grad, v->ncell * 4, hipMemcpyDeviceToHost));
     INV_TRY(hipMemcpy(gCB, v->grad + v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
     INV_TRY(hipMemcpy(gIEA, v->grad + 2 * v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int gpx_inv_set_keep(gpx_inv *v, double bytes)
+{
+    if (!v || !(bytes >= 0)) return ifail(v, GPX_EARG, "gpx_inv_set_keep: bad argument");
+    v->keep_bytes = bytes;
     return 0;
 }
 

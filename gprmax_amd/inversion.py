@@ -47,6 +47,7 @@ SIGNATURES = {
     'gpx_inv_set_dispersive': ([_P, _I, _P, _P, _P], _I),
     'gpx_inv_forward': ([_P, _I, _I, _I, _F, _P, _I, _P, _P], _I),
     'gpx_inv_backward': ([_P, _P, _P, _P, _P], _I),
+    'gpx_inv_set_keep': ([_P, ctypes.c_double], _I),
     'gpx_inv_timings': ([_P, ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
 }
 
@@ -230,6 +231,19 @@ class InversionGrid(InversionGeometry):
         ger[~self.interior] = 0
         gse[~self.interior] = 0
         return ger, gse
+
+    def set_keep(self, nbytes):
+        """Let the forward keep the stored states of its last checkpoint
+        segments in up to `nbytes` of device memory, so that the backward
+        skips their recompute (0, the default: recompute every segment).
+        The gradient does not change."""
+        self._check(self.lib.gpx_inv_set_keep(self.v, float(nbytes)))
+
+    def keep_bytes_all(self):
+        """Device bytes that keep every segment: (6, or 9 with poles) floats
+        per grid point and iteration."""
+        per = 9 if self.debye is not None else 6
+        return 4.0 * per * (self.nx + 1) * (self.ny + 1) * (self.nz + 1) * self.K * (-(-self.iterations // self.K))
 
     def timings(self):
         f, b = ctypes.c_float(), ctypes.c_float()

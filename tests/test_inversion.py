@@ -224,3 +224,31 @@ def test_gpu_debye_adjoint_matches_float64_autograd(poles):
     for ours, ref in ((ger, ger_ref), (gse, gse_ref)):
         err = np.linalg.norm(ours[m] - ref[m]) / np.linalg.norm(ref[m])
         assert err < 1e-3, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('poles', [0, 3])
+def test_gpu_kept_segments_give_the_recomputed_gradient(poles):
+    """The forward keeping the stored states of none, some (2 of 5, the last
+    one partial) or all of its checkpoint segments: traces, loss and both
+    gradients equal those of the all-recompute backward bit for bit, also
+    when the kept set shrinks between forwards."""
+    from gprmax_amd.inversion import InversionGrid, misfit_and_gradient
+    geo, waves, er_t, se_t, er0, se0, src, rx = _setup()
+    if poles:
+        de, tau = _debye_params(SHAPE, 30 + poles, P=poles)
+        geo.set_debye(de, tau)
+    obs = _observed(geo, waves, er_t, se_t, src, rx)
+    grid = InversionGrid(SHAPE, 2e-3, geo.iterations, pml=PML, er_pml=7.0, K=16)
+    try:
+        if poles:
+            grid.set_debye(de, tau)
+        L0, ger0, gse0 = misfit_and_gradient(grid, er0, se0, src, rx, obs, waves)
+        seg = grid.keep_bytes_all() / 5          # 70 iterations, K = 16: 5 segments
+        for nbytes in (grid.keep_bytes_all(), 2.5 * seg, 0):
+            grid.set_keep(nbytes)
+            L, ger, gse = misfit_and_gradient(grid, er0, se0, src, rx, obs, waves)
+            assert L == L0, nbytes
+            assert np.array_equal(ger, ger0) and np.array_equal(gse, gse0), nbytes
+    finally:
+        grid.close()
