@@ -515,20 +515,19 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
         // Ey(i,j,k) = ... + CB (Hx(i,j,k) - Hx(i,j,k-1)) - CB (Hz(i,j,k) - Hz(i-1,j,k))
         // Ez(i,j,k) = ... + CB (Hy(i,j,k) - Hy(i-1,j,k)) - CB (Hx(i,j,k) - Hx(i,j-1,k))
         float dh[3];
-        dh[0] = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
-        dh[1] = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
-        dh[2] = xi(a, 0, i, j, k) - xi(a, 0, i, j + 1, k) - xi(a, 1, i, j, k) + xi(a, 1, i + 1, j, k);
-        // E-PML: dd = (S(e) - S(e - axis)) / d uses H at the edge and at edge - axis,
-        // so H here feeds the edge here (+) and the edge one step up the axis (-).
-        for (int s = 0; s < a.nslab; ++s) {
-            const InvSlab &S = a.slab[s];
-            if (!in_box3(S.gE_lo, S.gE_hi, i, j, k)) continue;
-            const int ip = i + (S.axis == 0), jp = j + (S.axis == 1), kp = k + (S.axis == 2);
-            for (int w = 0; w < 2; ++w) {
-                int comp, src, sign;
-                e_pair(S.axis, w, comp, src, sign);
-                dh[src] += (e_pml_ldd(a, S, w, i, j, k) - e_pml_ldd(a, S, w, ip, jp, kp)) / a.d;
-            }
+        if (i >= 1 && i + 2 <= a.nx && j >= 1 && j + 2 <= a.ny && k >= 1 && k + 2 <= a.nz) {
+            // every edge the curl terms read is updated: xi without its range
+            // checks, as on the fast path (same terms, same order)
+            const int o = g, NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
+            const int oc = cidx(a, i, j, k), CY = a.nz, CX = a.ny * a.nz;
+            auto XI = [&](int c, int off, int coff) { return a.CB[oc + coff] * a.E[c][o + off]; };
+            dh[0] = XI(1, 0, 0) - XI(1, 1, 1) - XI(2, 0, 0) + XI(2, NZ1, CY);
+            dh[1] = XI(2, 0, 0) - XI(2, PL, CX) - XI(0, 0, 0) + XI(0, 1, 1);
+            dh[2] = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
+        } else {
+            dh[0] = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
+            dh[1] = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
+            dh[2] = xi(a, 0, i, j, k) - xi(a, 0, i, j + 1, k) - xi(a, 1, i, j, k) + xi(a, 1, i + 1, j, k);
         }
         const int o = g;
         // Loads first (see the fast path), then the stores.
@@ -557,25 +556,42 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             gca0 = a.gCA[oc];
             gcb0 = a.gCB[oc];
         }
+        // E-PML: dd = (S(e) - S(e - axis)) / d uses H at the edge and at edge - axis,
+        // so H here feeds the edge here (+) and the edge one step up the axis (-).
+        // The edge's own term (e_pml_ldd of the edge, with its lE and IEA, 0
+        // outside the cells) shares the slab index, profile values and Phi
+        // adjoints with the update of the Phi adjoints, mE^n = RE mE^{n+1} +
+        // (dE/dPhiE)^T lE, written into the Q arrays (edges of cells only;
+        // edges outside the cells are never updated).
+        for (int s = 0; s < a.nslab; ++s) {
+            const InvSlab &S = a.slab[s];
+            if (!in_box3(S.gE_lo, S.gE_hi, i, j, k)) continue;
+            const bool own = in_e(S, i, j, k);
+            int p = 0;
+            int q[2] = {0, 0};
+            float mu[2] = {0.f, 0.f}, RA01 = 0.f, RB0 = 0.f, RE0 = 0.f, RF0 = 0.f;
+            if (own) {
+                pidx(S, true, i, j, k, p, q[0], q[1]);
+                RA01 = S.RE[p] - 1.0f;
+                RB0 = S.RE[S.t + p];
+                RE0 = S.RE[2 * S.t + p];
+                RF0 = S.RE[3 * S.t + p];
+                mu[0] = S.PE1[q[0]];
+                mu[1] = S.PE2[q[1]];
+            }
+            const int ip = i + (S.axis == 0), jp = j + (S.axis == 1), kp = k + (S.axis == 2);
+            for (int w = 0; w < 2; ++w) {
+                int comp, src, sign;
+                e_pair(S.axis, w, comp, src, sign);
+                const float ldd = own ? (float)sign * IEA * RA01 * lE[comp] - RF0 * mu[w] : 0.f;
+                dh[src] += (ldd - e_pml_ldd(a, S, w, ip, jp, kp)) / a.d;
+                if (own && cell) (w == 0 ? S.QE1 : S.QE2)[q[w]] = RE0 * mu[w] + (float)sign * IEA * RB0 * lE[comp];
+            }
+        }
         a.H[0][o] = h0 + dh[0];
         a.H[1][o] = h1 + dh[1];
         a.H[2][o] = h2 + dh[2];
         if (cell) {  // edges outside the cells are never updated
-            for (int s = 0; s < a.nslab; ++s) {
-                const InvSlab &S = a.slab[s];
-                if (!in_e(S, i, j, k)) continue;
-                int p;
-                int q[2];
-                pidx(S, true, i, j, k, p, q[0], q[1]);
-                const float RB0 = S.RE[S.t + p], RE0 = S.RE[2 * S.t + p];
-                for (int w = 0; w < 2; ++w) {
-                    int comp, src, sign;
-                    e_pair(S.axis, w, comp, src, sign);
-                    const float *mu = w == 0 ? S.PE1 : S.PE2;
-                    float *mo = w == 0 ? S.QE1 : S.QE2;
-                    mo[q[w]] = RE0 * mu[q[w]] + (float)sign * IEA * RB0 * lE[comp];
-                }
-            }
             float giea = 0.f;
             for (int c = 0; c < 3; ++c) {
                 if (!e_upd(a, c, i, j, k)) continue;
@@ -653,42 +669,55 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
         // Hy(i,j,k) = Hy - DB (Ex(i,j,k+1) - Ex(i,j,k)) + DB (Ez(i+1,j,k) - Ez(i,j,k))
         // Hz(i,j,k) = Hz - DB (Ey(i+1,j,k) - Ey(i,j,k)) + DB (Ex(i,j+1,k) - Ex(i,j,k))
         float de[3];
-        de[0] = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
-        de[1] = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
-        de[2] = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
-        // H-PML: dd = (S(h + axis) - S(h)) / d, so E here feeds the cell here (-)
-        // and the cell one step down the axis (+).
-        for (int s = 0; s < a.nslab; ++s) {
-            const InvSlab &S = a.slab[s];
-            if (!in_box3(S.gH_lo, S.gH_hi, i, j, k)) continue;
-            const int im = i - (S.axis == 0), jm = j - (S.axis == 1), km = k - (S.axis == 2);
-            for (int w = 0; w < 2; ++w) {
-                int comp, src, sign;
-                h_pair(S.axis, w, comp, src, sign);
-                de[src] += (h_pml_ldd(a, S, w, im, jm, km) - h_pml_ldd(a, S, w, i, j, k)) / a.d;
-            }
+        if (i >= 1 && i + 1 <= a.nx && j >= 1 && j + 1 <= a.ny && k >= 1 && k + 1 <= a.nz) {
+            // every cell the curl terms read is updated: eta without its range
+            // checks, as on the fast path (same terms, same order)
+            const int o = g, NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
+            auto ETA = [&](int c, int off) { return a.DB * a.H[c][o + off]; };
+            de[0] = ETA(1, 0) - ETA(1, -1) - ETA(2, 0) + ETA(2, -NZ1);
+            de[1] = ETA(2, 0) - ETA(2, -PL) - ETA(0, 0) + ETA(0, -1);
+            de[2] = ETA(0, 0) - ETA(0, -NZ1) - ETA(1, 0) + ETA(1, -PL);
+        } else {
+            de[0] = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
+            de[1] = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
+            de[2] = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
         }
         // Loads first (lH here and lE for the update), then the stores.
         const float lH[3] = {a.H[0][g], a.H[1][g], a.H[2][g]};
         const float e0 = a.E[0][g], e1 = a.E[1][g], e2 = a.E[2][g];
-        a.E[0][g] = e0 + de[0];
-        a.E[1][g] = e1 + de[1];
-        a.E[2][g] = e2 + de[2];
+        // H-PML: dd = (S(h + axis) - S(h)) / d, so E here feeds the cell here (-)
+        // and the cell one step down the axis (+).  The cell's own term
+        // (h_pml_ldd of the cell, with its lH) shares the slab index, profile
+        // values and Phi adjoints with the update of the Phi adjoints, mH^n =
+        // RE mH^{n+1} + (dH/dPhiH)^T lH, written into the Q arrays.
         for (int s = 0; s < a.nslab; ++s) {
             const InvSlab &S = a.slab[s];
-            if (!in_h(S, i, j, k)) continue;
-            int p;
-            int q[2];
-            pidx(S, false, i, j, k, p, q[0], q[1]);
-            const float RB0 = S.RH[S.t + p], RE0 = S.RH[2 * S.t + p];
+            if (!in_box3(S.gH_lo, S.gH_hi, i, j, k)) continue;
+            const bool own = in_h(S, i, j, k);
+            int p = 0;
+            int q[2] = {0, 0};
+            float mu[2] = {0.f, 0.f}, RA01 = 0.f, RB0 = 0.f, RE0 = 0.f, RF0 = 0.f;
+            if (own) {
+                pidx(S, false, i, j, k, p, q[0], q[1]);
+                RA01 = S.RH[p] - 1.0f;
+                RB0 = S.RH[S.t + p];
+                RE0 = S.RH[2 * S.t + p];
+                RF0 = S.RH[3 * S.t + p];
+                mu[0] = S.PH1[q[0]];
+                mu[1] = S.PH2[q[1]];
+            }
+            const int im = i - (S.axis == 0), jm = j - (S.axis == 1), km = k - (S.axis == 2);
             for (int w = 0; w < 2; ++w) {
                 int comp, src, sign;
                 h_pair(S.axis, w, comp, src, sign);
-                const float *mu = w == 0 ? S.PH1 : S.PH2;
-                float *mo = w == 0 ? S.QH1 : S.QH2;
-                mo[q[w]] = RE0 * mu[q[w]] + (float)sign * a.cH * RB0 * lH[comp];
+                const float ldd = own ? (float)sign * a.cH * RA01 * lH[comp] - RF0 * mu[w] : 0.f;
+                de[src] += (h_pml_ldd(a, S, w, im, jm, km) - ldd) / a.d;
+                if (own) (w == 0 ? S.QH1 : S.QH2)[q[w]] = RE0 * mu[w] + (float)sign * a.cH * RB0 * lH[comp];
             }
         }
+        a.E[0][g] = e0 + de[0];
+        a.E[1][g] = e1 + de[1];
+        a.E[2][g] = e2 + de[2];
     }
 }
 
