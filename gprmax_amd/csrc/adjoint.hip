@@ -31,7 +31,7 @@
 // linearly.  Its adjoint, applied from n = N-1 down to 0 to the adjoint state
 // (lE, lH, mE, mH) = dL/ds^{n+1}, is two gather kernels:
 //   k_adj_e : lH += (dE^{n+1}/dH^{n+1})^T lE  (curl and E-PML terms); gradients
-//             from lE and the stored E^n, H^{n+1}; mE = (dE/dPhiE)^T lE + RE mE;
+//             from lE and the stored E^n and curl H^{n+1}; mE = (dE/dPhiE)^T lE + RE mE;
 //             lE = CA lE
 //   k_adj_h : lE += (dH^{n+1}/dE^n)^T lH      (curl and H-PML terms);
 //             mH = (dH/dPhiH)^T lH + RE mH; lH = DA lH = lH
@@ -39,7 +39,7 @@
 // are double-buffered (read one copy, write the other), so each kernel's
 // gathers see the values of the previous sub-step only.  Forward states
 // come from checkpoints every K iterations; each segment is recomputed
-// forward, storing E^n and H^{n+1} for every iteration of it, before the
+// forward, storing E^n and curl H^{n+1} for every iteration of it, before the
 // adjoint sweeps back over it.
 
 #include <hip/hip_runtime.h>
@@ -83,8 +83,8 @@ struct InvArgs {
     float d, DB, cH;                     // cell size; DB = dt/(mu0 d); cH = dt/mu0
     float *E[3], *H[3];                  // fields (forward) or adjoint fields
     float *En[3];                        // adjoint: lE^n written here (lE^{n+1} is read from E)
-    float *Es[3], *Hs[3];                // forward: where to save E^n and H^{n+1} (or null);
-                                         // adjoint: the saved E^n and H^{n+1}
+    float *Es[3], *Cs[3];                // forward: where to save E^n and the curl of H^{n+1}
+                                         // (or null); adjoint: the saved E^n and curl
     const float *CA, *CB, *IEA;          // per cell [nx][ny][nz]
     float *gCA, *gCB, *gIEA;             // gradient accumulators (adjoint)
     int nslab;
@@ -252,11 +252,6 @@ __global__ void k_inv_h(InvArgs a)
         a.H[0][o] = h[0];
         a.H[1][o] = h[1];
         a.H[2][o] = h[2];
-        if (a.Hs[0]) {
-            a.Hs[0][o] = h[0];
-            a.Hs[1][o] = h[1];
-            a.Hs[2][o] = h[2];
-        }
     }
 }
 
@@ -280,6 +275,12 @@ __global__ void k_inv_e(InvArgs a)
         const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o, k > 0 ? o - 1 : o};
         float e[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
         const float en[3] = {e[0], e[1], e[2]};  // E^n, saved at the end (stores after every load)
+        // The differences of H^{n+1} of the E update (om clamps to the point
+        // itself at the low faces, so every load is inside the grid).
+        const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
+        const float d2y = h2 - a.H[2][om[1]], d1z = h1 - a.H[1][om[2]];
+        const float d0z = h0 - a.H[0][om[2]], d2x = h2 - a.H[2][om[0]];
+        const float d1x = h1 - a.H[1][om[0]], d0y = h0 - a.H[0][om[1]];
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
         const int oc = cell ? cidx(a, i, j, k) : 0;
         const float CA = cell ? a.CA[oc] : 0.f, CB = cell ? a.CB[oc] : 0.f, IEA = cell ? a.IEA[oc] : 0.f;
@@ -302,12 +303,9 @@ __global__ void k_inv_e(InvArgs a)
                 phi[c] = ph;
             }
         }
-        if (upd[0])
-            e[0] = CA * e[0] + CB * (a.H[2][o] - a.H[2][om[1]]) - CB * (a.H[1][o] - a.H[1][om[2]]);
-        if (upd[1])
-            e[1] = CA * e[1] + CB * (a.H[0][o] - a.H[0][om[2]]) - CB * (a.H[2][o] - a.H[2][om[0]]);
-        if (upd[2])
-            e[2] = CA * e[2] + CB * (a.H[1][o] - a.H[1][om[0]]) - CB * (a.H[0][o] - a.H[0][om[1]]);
+        if (upd[0]) e[0] = CA * e[0] + CB * d2y - CB * d1z;
+        if (upd[1]) e[1] = CA * e[1] + CB * d0z - CB * d2x;
+        if (upd[2]) e[2] = CA * e[2] + CB * d1x - CB * d0y;
         if (disp)
             for (int c = 0; c < 3; ++c)
                 if (upd[c]) e[c] = e[c] - IEA * phi[c];
@@ -348,6 +346,11 @@ __global__ void k_inv_e(InvArgs a)
             a.Es[0][o] = en[0];
             a.Es[1][o] = en[1];
             a.Es[2][o] = en[2];
+        }
+        if (a.Cs[0]) {  // the curl of H^{n+1} in the form dL/dCB reads it
+            a.Cs[0][o] = d2y - d1z;
+            a.Cs[1][o] = d0z - d2x;
+            a.Cs[2][o] = d1x - d0y;
         }
         if (disp && a.Ps[0])
             for (int c = 0; c < 3; ++c) a.Ps[c][o] = phi[c];
@@ -437,7 +440,7 @@ __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k
 //   lH    += (dE^{n+1}/dH^{n+1})^T lE^{n+1}      (curl and E-PML terms, a gather)
 //   lE^n   = CA lE^{n+1} on updated edges (else unchanged), into En
 //   mE^n   = RE mE^{n+1} + (dE^{n+1}/dPhiE^n)^T lE^{n+1}, into the Q arrays
-//   gradients of the cell from lE^{n+1}, the saved E^n and H^{n+1}.
+//   gradients of the cell from lE^{n+1}, the saved E^n and curl H^{n+1}.
 // lE^{n+1} and mE^{n+1} are only read, so neighbours see consistent values.
 // FAST = true runs over the points of the fast box only (every lane of a wave
 // takes the short path); FAST = false over all other points.
@@ -482,10 +485,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             const float dh2 = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
             const float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
             const float CA = a.CA[oc];
-            const float curl[3] = {
-                (a.Hs[2][o] - a.Hs[2][o - NZ1]) - (a.Hs[1][o] - a.Hs[1][o - 1]),
-                (a.Hs[0][o] - a.Hs[0][o - 1]) - (a.Hs[2][o] - a.Hs[2][o - PL]),
-                (a.Hs[1][o] - a.Hs[1][o - PL]) - (a.Hs[0][o] - a.Hs[0][o - NZ1])};
+            const float curl[3] = {a.Cs[0][o], a.Cs[1][o], a.Cs[2][o]};
             float gca = 0.f, gcb = 0.f;
             for (int c = 0; c < 3; ++c) {
                 gca += lE[c] * a.Es[c][o];
@@ -544,10 +544,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                                      k > 0 ? o - 1 : o};
             CA = a.CA[oc];
             IEA = a.IEA[oc];
-            const float curl[3] = {
-                (a.Hs[2][o] - a.Hs[2][om[1]]) - (a.Hs[1][o] - a.Hs[1][om[2]]),
-                (a.Hs[0][o] - a.Hs[0][om[2]]) - (a.Hs[2][o] - a.Hs[2][om[0]]),
-                (a.Hs[1][o] - a.Hs[1][om[0]]) - (a.Hs[0][o] - a.Hs[0][om[1]])};
+            const float curl[3] = {a.Cs[0][o], a.Cs[1][o], a.Cs[2][o]};
             for (int c = 0; c < 3; ++c) {
                 if (!e_upd(a, c, i, j, k)) continue;
                 gca += lE[c] * a.Es[c][o];
@@ -1027,7 +1024,7 @@ extern "C" int gpx_inv_set_dispersive(gpx_inv *v, int P, const float *ee, const 
             *p = nullptr;
         }
         v->P = P;
-        // the segment store keeps phi^n of the three E components beside E^n, H^{n+1}
+        // the segment store keeps phi^n of the three E components beside E^n, curl H^{n+1}
         INV_TRY(hipMalloc((void **)&v->seg, (size_t)v->desc.K * (P ? 9 : 6) * v->ntot * 4));
         if (P) {
             INV_TRY(hipMalloc((void **)&v->dc, (size_t)3 * P * v->ncell * 4));
@@ -1156,14 +1153,14 @@ static int checkpoint(gpx_inv *v, int c, bool save)
     return 0;
 }
 
-// One forward iteration; with `slot`, the kernels also store E^n, H^{n+1}
+// One forward iteration; with `slot`, the kernels also store E^n, curl H^{n+1}
 // (and phi^n) there, as the recompute does.
 static void forward_step(gpx_inv *v, int it, bool rx, float *slot = nullptr)
 {
     InvArgs a = inv_args(v, false, it);
     for (int q = 0; slot && q < 3; ++q) {
         a.Es[q] = slot + q * v->ntot;
-        a.Hs[q] = slot + (3 + q) * v->ntot;
+        a.Cs[q] = slot + (3 + q) * v->ntot;
         if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
     }
     if (rx && v->nrx) hipLaunchKernelGGL(k_inv_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
@@ -1285,15 +1282,15 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
         const int n0 = c * g.K, n1 = n0 + g.K < g.iterations ? n0 + g.K : g.iterations;
         const bool kept = c >= nseg - v->nkeep;
         float *base = kept ? v->keep + (long long)(c - (nseg - v->nkeep)) * g.K * seg_stride(v) : v->seg;
-        // recompute the segment, storing E^n and H^{n+1} (kept segments have them)
+        // recompute the segment, storing E^n and curl H^{n+1} (kept segments have them)
         if (!kept)
             if (int rc = checkpoint(v, c, false)) return rc;
         for (int it = n0; it < n1 && !kept; ++it) {
             float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, false, it);
-            for (int q = 0; q < 3; ++q) {  // the kernels save E^n, H^{n+1} (and phi^n) here
+            for (int q = 0; q < 3; ++q) {  // the kernels save E^n, curl H^{n+1} (and phi^n) here
                 a.Es[q] = slot + q * v->ntot;
-                a.Hs[q] = slot + (3 + q) * v->ntot;
+                a.Cs[q] = slot + (3 + q) * v->ntot;
                 if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
             }
             hipLaunchKernelGGL(k_inv_h, G, dim3(128), 0, v->stream, a);
@@ -1305,7 +1302,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
             InvArgs a = inv_args(v, true, it);
             for (int q = 0; q < 3; ++q) {
                 a.Es[q] = slot + q * v->ntot;
-                a.Hs[q] = slot + (3 + q) * v->ntot;
+                a.Cs[q] = slot + (3 + q) * v->ntot;
                 if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
                 a.E[q] = lE[cur] + q * v->ntot;
                 a.En[q] = lE[cur ^ 1] + q * v->ntot;

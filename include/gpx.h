@@ -308,8 +308,8 @@ int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, const float 
 /* Adjoint of the last forward: resid float32 [nrx][iterations] = dL/d(trace);
  * outputs float32 [nx][ny][nz].  Recomputes each checkpoint segment. */
 int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, float *gCB, float *gIEA);
-/* Device memory the forward may spend keeping the stored states (E^n,
- * H^{n+1}, and phi^n with poles: 6 or 9 x (nx+1)(ny+1)(nz+1) floats per
+/* Device memory the forward may spend keeping the stored states (E^n, the
+ * curl of H^{n+1}, and phi^n with poles: 6 or 9 x (nx+1)(ny+1)(nz+1) floats per
  * iteration) of its last checkpoint segments, so that the backward does not
  * recompute them (default 0: every segment is recomputed).  As many whole
  * segments as fit are kept, from the last one down; the gradient is the same
