@@ -83,8 +83,9 @@ struct InvArgs {
     float d, DB, cH;                     // cell size; DB = dt/(mu0 d); cH = dt/mu0
     float *E[3], *H[3];                  // fields (forward) or adjoint fields
     float *En[3];                        // adjoint: lE^n written here (lE^{n+1} is read from E)
-    float *Es[3], *Cs[3];                // forward: where to save E^n and the curl of H^{n+1}
-                                         // (or null); adjoint: the saved E^n and curl
+    float *EC;                           // forward: where to save, per point, E^n (3 floats)
+                                         // then the curl of H^{n+1} (3), as three float2
+                                         // (or null); adjoint: the saved values
     const float *CA, *CB, *IEA;          // per cell [nx][ny][nz]
     float *gCA, *gCB, *gIEA;             // gradient accumulators (adjoint)
     int nslab;
@@ -342,15 +343,11 @@ __global__ void k_inv_e(InvArgs a)
         a.E[0][o] = e[0];
         a.E[1][o] = e[1];
         a.E[2][o] = e[2];
-        if (a.Es[0]) {
-            a.Es[0][o] = en[0];
-            a.Es[1][o] = en[1];
-            a.Es[2][o] = en[2];
-        }
-        if (a.Cs[0]) {  // the curl of H^{n+1} in the form dL/dCB reads it
-            a.Cs[0][o] = d2y - d1z;
-            a.Cs[1][o] = d0z - d2x;
-            a.Cs[2][o] = d1x - d0y;
+        if (a.EC) {  // E^n, then the curl of H^{n+1} in the form dL/dCB reads it
+            float2 *q = reinterpret_cast<float2 *>(a.EC) + 3 * (size_t)o;
+            q[0] = make_float2(en[0], en[1]);
+            q[1] = make_float2(en[2], d2y - d1z);
+            q[2] = make_float2(d0z - d2x, d1x - d0y);
         }
         if (disp && a.Ps[0])
             for (int c = 0; c < 3; ++c) a.Ps[c][o] = phi[c];
@@ -485,10 +482,12 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
             const float dh2 = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
             const float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
             const float CA = a.CA[oc];
-            const float curl[3] = {a.Cs[0][o], a.Cs[1][o], a.Cs[2][o]};
+            const float2 *sq = reinterpret_cast<const float2 *>(a.EC) + 3 * (size_t)o;
+            const float2 s0 = sq[0], s1 = sq[1], s2 = sq[2];
+            const float En[3] = {s0.x, s0.y, s1.x}, curl[3] = {s1.y, s2.x, s2.y};
             float gca = 0.f, gcb = 0.f;
             for (int c = 0; c < 3; ++c) {
-                gca += lE[c] * a.Es[c][o];
+                gca += lE[c] * En[c];
                 gcb += lE[c] * curl[c];
             }
             const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
@@ -544,10 +543,12 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                                      k > 0 ? o - 1 : o};
             CA = a.CA[oc];
             IEA = a.IEA[oc];
-            const float curl[3] = {a.Cs[0][o], a.Cs[1][o], a.Cs[2][o]};
+            const float2 *sq = reinterpret_cast<const float2 *>(a.EC) + 3 * (size_t)o;
+            const float2 s0 = sq[0], s1 = sq[1], s2 = sq[2];
+            const float En[3] = {s0.x, s0.y, s1.x}, curl[3] = {s1.y, s2.x, s2.y};
             for (int c = 0; c < 3; ++c) {
                 if (!e_upd(a, c, i, j, k)) continue;
-                gca += lE[c] * a.Es[c][o];
+                gca += lE[c] * En[c];
                 gcb += lE[c] * curl[c];
             }
             gca0 = a.gCA[oc];
@@ -885,7 +886,7 @@ extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
     if (e == hipSuccess) e = hipMalloc((void **)&v->F, 6 * v->ntot * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->L, 6 * v->ntot * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->L2, 3 * v->ntot * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&v->seg, (size_t)d->K * 6 * v->ntot * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&v->seg, (size_t)d->K * ((6 * v->ntot + 3) / 4 * 4) * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->par, 3 * v->ncell * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->grad, 3 * v->ncell * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->waves, (size_t)d->iterations * 4);
@@ -1025,7 +1026,7 @@ extern "C" int gpx_inv_set_dispersive(gpx_inv *v, int P, const float *ee, const 
         }
         v->P = P;
         // the segment store keeps phi^n of the three E components beside E^n, curl H^{n+1}
-        INV_TRY(hipMalloc((void **)&v->seg, (size_t)v->desc.K * (P ? 9 : 6) * v->ntot * 4));
+        INV_TRY(hipMalloc((void **)&v->seg, (size_t)v->desc.K * (((P ? 9 : 6) * v->ntot + 3) / 4 * 4) * 4));
         if (P) {
             INV_TRY(hipMalloc((void **)&v->dc, (size_t)3 * P * v->ncell * 4));
             INV_TRY(hipMalloc((void **)&v->T, (size_t)3 * P * v->ntot * 4));
@@ -1044,7 +1045,10 @@ extern "C" int gpx_inv_set_dispersive(gpx_inv *v, int P, const float *ee, const 
 
 static int n_checkpoints(const gpx_inv *v) { return (v->desc.iterations + v->desc.K - 1) / v->desc.K; }
 static long long ck_stride(const gpx_inv *v) { return 6 * v->ntot + 3 * v->P * v->ntot + v->phi_total; }
-static long long seg_stride(const gpx_inv *v) { return (v->P ? 9 : 6) * v->ntot; }
+// Floats per iteration in the segment store: E^n and curl H^{n+1} interleaved
+// (6 per point), then phi^n of the three E components with poles; rounded up
+// to 4 floats so that every slot is 16-byte aligned.
+static long long seg_stride(const gpx_inv *v) { return ((v->P ? 9 : 6) * v->ntot + 3) / 4 * 4; }
 
 static int all_gk(const gpx_inv *v);
 static int all_blocks(const gpx_inv *v);
@@ -1158,11 +1162,8 @@ static int checkpoint(gpx_inv *v, int c, bool save)
 static void forward_step(gpx_inv *v, int it, bool rx, float *slot = nullptr)
 {
     InvArgs a = inv_args(v, false, it);
-    for (int q = 0; slot && q < 3; ++q) {
-        a.Es[q] = slot + q * v->ntot;
-        a.Cs[q] = slot + (3 + q) * v->ntot;
-        if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
-    }
+    if (slot) a.EC = slot;
+    for (int q = 0; slot && v->P && q < 3; ++q) a.Ps[q] = slot + (6 + q) * v->ntot;
     if (rx && v->nrx) hipLaunchKernelGGL(k_inv_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
     hipLaunchKernelGGL(k_inv_h, grid_all(v), dim3(128), 0, v->stream, a);
     hipLaunchKernelGGL(k_inv_e, grid_all(v), dim3(128), 0, v->stream, a);
@@ -1288,11 +1289,8 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
         for (int it = n0; it < n1 && !kept; ++it) {
             float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, false, it);
-            for (int q = 0; q < 3; ++q) {  // the kernels save E^n, curl H^{n+1} (and phi^n) here
-                a.Es[q] = slot + q * v->ntot;
-                a.Cs[q] = slot + (3 + q) * v->ntot;
-                if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
-            }
+            a.EC = slot;  // the kernels save E^n, curl H^{n+1} (and phi^n) here
+            for (int q = 0; v->P && q < 3; ++q) a.Ps[q] = slot + (6 + q) * v->ntot;
             hipLaunchKernelGGL(k_inv_h, G, dim3(128), 0, v->stream, a);
             hipLaunchKernelGGL(k_inv_e, G, dim3(128), 0, v->stream, a);
         }
@@ -1300,9 +1298,8 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
         for (int it = n1 - 1; it >= n0; --it) {
             float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, true, it);
+            a.EC = slot;
             for (int q = 0; q < 3; ++q) {
-                a.Es[q] = slot + q * v->ntot;
-                a.Cs[q] = slot + (3 + q) * v->ntot;
                 if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
                 a.E[q] = lE[cur] + q * v->ntot;
                 a.En[q] = lE[cur ^ 1] + q * v->ntot;

@@ -240,10 +240,11 @@ class InversionGrid(InversionGeometry):
         self._check(self.lib.gpx_inv_set_keep(self.v, float(nbytes)))
 
     def keep_bytes_all(self):
-        """Device bytes that keep every segment: (6, or 9 with poles) floats
-        per grid point and iteration."""
+        """Device bytes that keep every segment: per iteration 6 (9 with
+        poles) floats per grid point, rounded up to a multiple of 4 floats."""
         per = 9 if self.debye is not None else 6
-        return 4.0 * per * (self.nx + 1) * (self.ny + 1) * (self.nz + 1) * self.K * (-(-self.iterations // self.K))
+        floats = (per * (self.nx + 1) * (self.ny + 1) * (self.nz + 1) + 3) // 4 * 4
+        return 4.0 * floats * self.K * (-(-self.iterations // self.K))
 
     def timings(self):
         f, b = ctypes.c_float(), ctypes.c_float()
