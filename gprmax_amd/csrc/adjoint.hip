@@ -87,7 +87,8 @@ struct InvArgs {
                                          // then the curl of H^{n+1} (3), as three float2
                                          // (or null); adjoint: the saved values
     const float *CA, *CB, *IEA;          // per cell [nx][ny][nz]
-    float *gCA, *gCB, *gIEA;             // gradient accumulators (adjoint)
+    float2 *gCAB;                        // gradient accumulators (adjoint): dL/dCA, dL/dCB per cell
+    float *gIEA;                         // and dL/dIEA
     int nslab;
     InvSlab slab[INV_MAX_SLABS];
     int si, sj, sk;                      // source edge (Ez), -1 when none
@@ -491,12 +492,12 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                 gcb += lE[c] * curl[c];
             }
             const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
-            const float gca0 = a.gCA[oc], gcb0 = a.gCB[oc];
+            const float2 gab = a.gCAB[oc];
+            const float gca0 = gab.x, gcb0 = gab.y;
             a.H[0][o] = h0 + dh0;
             a.H[1][o] = h1 + dh1;
             a.H[2][o] = h2 + dh2;
-            a.gCA[oc] = gca0 + gca;
-            a.gCB[oc] = gcb0 + gcb;
+            a.gCAB[oc] = make_float2(gca0 + gca, gcb0 + gcb);
             float giea = 0.f;
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
             float out[3] = {CA * lE[0], CA * lE[1], CA * lE[2]};
@@ -551,8 +552,9 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                 gca += lE[c] * En[c];
                 gcb += lE[c] * curl[c];
             }
-            gca0 = a.gCA[oc];
-            gcb0 = a.gCB[oc];
+            const float2 gab = a.gCAB[oc];
+            gca0 = gab.x;
+            gcb0 = gab.y;
         }
         // E-PML: dd = (S(e) - S(e - axis)) / d uses H at the edge and at edge - axis,
         // so H here feeds the edge here (+) and the edge one step up the axis (-).
@@ -597,8 +599,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
                 if (a.P > 0) disp_reverse(a, c, o, oc, IEA, lE[c], out[c], giea);
             }
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
-            a.gCA[oc] = gca0 + gca;
-            a.gCB[oc] = gcb0 + gcb;
+            a.gCAB[oc] = make_float2(gca0 + gca, gcb0 + gcb);
             if (giea != 0.f) a.gIEA[oc] += giea;
         }
         a.En[0][o] = out[0];
@@ -773,7 +774,7 @@ struct gpx_inv {
     float *ck = nullptr;           // checkpoints: nck x (6 ntot + phi)
     float *seg = nullptr;          // segment store: K x 6 x ntot
     float *par = nullptr;          // CA, CB, IEA: 3 x ncell
-    float *grad = nullptr;         // gCA, gCB, gIEA: 3 x ncell
+    float *grad = nullptr;         // (gCA, gCB) interleaved per cell, then gIEA: 3 x ncell
     float *waves = nullptr;
     int P = 0;                     // Debye poles
     float *dc = nullptr;           // ee, eqt, zt: 3 x P x ncell
@@ -1072,8 +1073,7 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     a.CA = v->par;
     a.CB = v->par + v->ncell;
     a.IEA = v->par + 2 * v->ncell;
-    a.gCA = v->grad;
-    a.gCB = v->grad + v->ncell;
+    a.gCAB = reinterpret_cast<float2 *>(v->grad);
     a.gIEA = v->grad + 2 * v->ncell;
     a.nslab = (int)v->slabs.size();
     for (int s = 0; s < a.nslab; ++s) {
@@ -1337,8 +1337,12 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     INV_TRY(hipEventElapsedTime(&v->ms_bwd, e0, e1));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    INV_TRY(hipMemcpy(gCA, v->grad, v->ncell * 4, hipMemcpyDeviceToHost));
-    INV_TRY(hipMemcpy(gCB, v->grad + v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
+    std::vector<float> ab((size_t)2 * v->ncell);
+    INV_TRY(hipMemcpy(ab.data(), v->grad, (size_t)2 * v->ncell * 4, hipMemcpyDeviceToHost));
+    for (long long c = 0; c < v->ncell; ++c) {
+        gCA[c] = ab[2 * c];
+        gCB[c] = ab[2 * c + 1];
+    }
     INV_TRY(hipMemcpy(gIEA, v->grad + 2 * v->ncell, v->ncell * 4, hipMemcpyDeviceToHost));
     return 0;
 }
