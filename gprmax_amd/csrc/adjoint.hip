@@ -34,8 +34,10 @@
 //             from lE and the stored E^n and curl H^{n+1}; mE = (dE/dPhiE)^T lE + RE mE;
 //             lE = CA lE
 //   k_adj_h : lE += (dH^{n+1}/dE^n)^T lH      (curl and H-PML terms);
-//             mH = (dH/dPhiH)^T lH + RE mH; lH = DA lH = lH
-// then the receiver residual of iteration n is added to lE.  lE, mE and mH
+//             mH = (dH/dPhiH)^T lH + RE mH; lH = DA lH = lH; and the
+//             receiver residual of iteration n is added to lEz at each
+//             receiver (the thread of the point adds it after its update).
+// lE, mE and mH
 // are double-buffered (read one copy, write the other), so each kernel's
 // gathers see the values of the previous sub-step only.  Forward states
 // come from checkpoints every K iterations; each segment is recomputed
@@ -102,6 +104,7 @@ struct InvArgs {
     float *Ps[3];                        // forward: where to save phi^n (or null); adjoint: phi^n
     int nrx;
     const int *rxc;                      // [nrx][3]
+    const uint8_t *rxrow;                // adjoint: per (i, j) row of points, 1 when a receiver is on it
     float *traces;                       // [nrx][iterations]
     const float *resid;                  // [nrx][iterations], dL/d(trace)
     // Grid points [fast[0], fast[1]) x [fast[2], fast[3]) x [fast[4], fast[5])
@@ -427,6 +430,18 @@ __global__ void k_adj_disp(InvArgs a)
     }
 }
 
+// lEz^n at point (i, j, k) after the H half: v plus dL/d(trace sample of
+// iteration it) of every receiver on the point, in receiver order.
+__device__ __forceinline__ float adj_rx_add(const InvArgs &a, int i, int j, int k, float v)
+{
+    if (!a.rxrow[i * (a.ny + 1) + j]) return v;
+    for (int r = 0; r < a.nrx; ++r) {
+        const int *c = a.rxc + 3 * r;
+        if (c[0] == i && c[1] == j && c[2] == k) v += a.resid[(long long)r * a.iterations + a.it];
+    }
+    return v;
+}
+
 // CB of the cell of E edge (i, j, k) times lE_c there, when that edge is updated.
 __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k)
 {
@@ -659,9 +674,10 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
             const float de0 = ETA(1, 0) - ETA(1, -1) - ETA(2, 0) + ETA(2, -NZ1);
             const float de1 = ETA(2, 0) - ETA(2, -PL) - ETA(0, 0) + ETA(0, -1);
             const float de2 = ETA(0, 0) - ETA(0, -NZ1) - ETA(1, 0) + ETA(1, -PL);
+            const float e2 = a.E[2][o] + de2;
             a.E[0][o] += de0;
             a.E[1][o] += de1;
-            a.E[2][o] += de2;
+            a.E[2][o] = adj_rx_add(a, i, j, k, e2);
             return;
         }
         // Hx(i,j,k) = Hx - DB (Ez(i,j+1,k) - Ez(i,j,k)) + DB (Ey(i,j,k+1) - Ey(i,j,k))
@@ -716,7 +732,7 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
         }
         a.E[0][g] = e0 + de[0];
         a.E[1][g] = e1 + de[1];
-        a.E[2][g] = e2 + de[2];
+        a.E[2][g] = adj_rx_add(a, i, j, k, e2 + de[2]);
     }
 }
 
@@ -752,14 +768,6 @@ __global__ void k_adj_h_mix(InvArgs a)
     else adj_h_point<true>(a, q % a.mix_gf[0], (q / a.mix_gf[0]) % a.mix_gf[1], q / (a.mix_gf[0] * a.mix_gf[1]));
 }
 
-// lEz at the receivers += dL/d(trace sample of iteration it).
-__global__ void k_adj_rx(InvArgs a)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.nrx) return;
-    const int *c = a.rxc + 3 * r;
-    a.E[2][fidx(a, c[0], c[1], c[2])] += a.resid[(long long)r * a.iterations + a.it];
-}
 
 }  // namespace
 
@@ -781,6 +789,7 @@ struct gpx_inv {
     float *T = nullptr;            // T of Ex, Ey, Ez: 3 x P x ntot
     float *LT = nullptr;           // its adjoint
     int *rxc = nullptr;
+    uint8_t *rxrow = nullptr;      // [nx+1][ny+1]: 1 on the rows of points that hold a receiver
     float *traces = nullptr, *resid = nullptr;
     int nrx = 0, si = -1, sj = -1, sk = -1;
     float sfac = 0;
@@ -909,6 +918,7 @@ extern "C" int gpx_inv_destroy(gpx_inv *v)
                      v->T, v->LT, v->keep})
         if (p) hipFree(p);
     if (v->rxc) hipFree(v->rxc);
+    if (v->rxrow) hipFree(v->rxrow);
     for (auto &S : v->slabs) {
         for (float *p : S.phi) if (p) hipFree(p);
         for (float *p : S.mu) if (p) hipFree(p);
@@ -1091,6 +1101,7 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     a.iterations = g.iterations;
     a.nrx = v->nrx;
     a.rxc = v->rxc;
+    a.rxrow = v->rxrow;
     a.traces = v->traces;
     a.resid = v->resid;
     a.ncell = v->ncell;
@@ -1198,6 +1209,13 @@ extern "C" int gpx_inv_forward(gpx_inv *v, int si, int sj, int sk, float sfac, c
         INV_TRY(hipMalloc((void **)&v->resid, (size_t)(nrx > 0 ? nrx : 1) * g.iterations * 4));
     }
     if (nrx) INV_TRY(hipMemcpy(v->rxc, rx, (size_t)nrx * 3 * 4, hipMemcpyHostToDevice));
+    {  // the rows of points that hold a receiver, for the adjoint H half
+        const size_t rows = (size_t)(g.nx + 1) * (g.ny + 1);
+        if (!v->rxrow) INV_TRY(hipMalloc((void **)&v->rxrow, rows));
+        std::vector<uint8_t> flag(rows, 0);
+        for (int r = 0; r < nrx; ++r) flag[(size_t)rx[3 * r] * (g.ny + 1) + rx[3 * r + 1]] = 1;
+        INV_TRY(hipMemcpy(v->rxrow, flag.data(), rows, hipMemcpyHostToDevice));
+    }
     if (!v->ck) INV_TRY(hipMalloc((void **)&v->ck, (size_t)n_checkpoints(v) * ck_stride(v) * 4));
     // the segments whose states this forward keeps
     const int nseg = n_checkpoints(v);
@@ -1327,7 +1345,6 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
                 for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
                 hipLaunchKernelGGL(k_adj_h_mix, GM, dim3(128), 0, v->stream, a);
             }
-            if (v->nrx) hipLaunchKernelGGL(k_adj_rx, dim3((v->nrx + 63) / 64), dim3(64), 0, v->stream, a);
             cur ^= 1;
         }
         INV_TRY(hipGetLastError());
