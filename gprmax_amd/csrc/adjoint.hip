@@ -458,7 +458,7 @@ __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k
 // FAST = true runs over the points of the fast box only (every lane of a wave
 // takes the short path); FAST = false over all other points.
 template <bool FAST>
-__device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, int bz)
+__device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
 {
     {
         // launched over (segments of a (j, k) plane, i) of the fast box (FAST),
@@ -626,7 +626,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by, in
 template <bool FAST>
 __global__ void k_adj_e(InvArgs a)
 {
-    adj_e_point<FAST>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    adj_e_point<FAST>(a, blockIdx.x, blockIdx.y);
 }
 
 // DB times lH_c at H cell (i, j, k), when that cell is updated.
@@ -643,7 +643,7 @@ __device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int 
 // FAST = true runs over the points of the fast box only (every lane of a wave
 // takes the short path); FAST = false over all other points.
 template <bool FAST>
-__device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, int bz)
+__device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by)
 {
     {
         // launched over (segments of a (j, k) plane, i) of the fast box (FAST),
@@ -739,7 +739,7 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by, in
 template <bool FAST>
 __global__ void k_adj_h(InvArgs a)
 {
-    adj_h_point<FAST>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    adj_h_point<FAST>(a, blockIdx.x, blockIdx.y);
 }
 
 // The fast-box and shell blocks of one half-step in one launch: the shell
@@ -757,15 +757,15 @@ __device__ __forceinline__ bool mix_shell(const InvArgs &a, int b, int &idx)
 __global__ void k_adj_e_mix(InvArgs a)
 {
     int q;
-    if (mix_shell(a, blockIdx.x, q)) adj_e_point<false>(a, q % a.mix_gs, q / a.mix_gs, 0);
-    else adj_e_point<true>(a, q % a.mix_gf[0], (q / a.mix_gf[0]) % a.mix_gf[1], q / (a.mix_gf[0] * a.mix_gf[1]));
+    if (mix_shell(a, blockIdx.x, q)) adj_e_point<false>(a, q % a.mix_gs, q / a.mix_gs);
+    else adj_e_point<true>(a, q % a.mix_gf[0], q / a.mix_gf[0]);
 }
 
 __global__ void k_adj_h_mix(InvArgs a)
 {
     int q;
-    if (mix_shell(a, blockIdx.x, q)) adj_h_point<false>(a, q % a.mix_gs, q / a.mix_gs, 0);
-    else adj_h_point<true>(a, q % a.mix_gf[0], (q / a.mix_gf[0]) % a.mix_gf[1], q / (a.mix_gf[0] * a.mix_gf[1]));
+    if (mix_shell(a, blockIdx.x, q)) adj_h_point<false>(a, q % a.mix_gs, q / a.mix_gs);
+    else adj_h_point<true>(a, q % a.mix_gf[0], q / a.mix_gf[0]);
 }
 
 
