@@ -2,8 +2,9 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one iteration of the time loop (fused H pass + fused E pass) over
-the whole grid.  W untimed iterations run first, then exactly K iterations are
+A step is one iteration of the time loop (the H and E updates with their PML
+corrections, sources and receivers) over the whole grid, in the step form the
+library picks.  W untimed iterations run first, then exactly K iterations are
 timed with HIP events on the solver's stream, bracketed by synchronisation.
 The default workload is the 300x300x200 heterogeneous half-space (256 soil
 materials, PEC cylinder, magnetic dipole, nine receivers).
@@ -12,7 +13,10 @@ Prints one JSON line: Mcells/s (NX*NY*NZ*K / T), the roofline of the dominant
 kernel (its algorithmic bytes per launch / its mean launch duration from HIP
 events, against 8 TB/s HBM; HBM traffic per launch from the committed counter
 summary), and the CPU oracle timed on the host cores on a bounded sample of the
-same workload.
+same workload.  On the default workload the line also carries the metric's
+other two parts under `forms`: the batched B-scan against sequential plain
+solves (16 of the 96 traces) and the adjoint gradient against one forward
+(`--forms 0` leaves them out; `--workload bscan` / `adjoint` runs them alone).
 """
 
 import argparse
@@ -93,6 +97,12 @@ def measured_traffic(workload, kernel):
 
 
 def bench_bscan(args):
+    line = bscan_line(args, args.traces)
+    print(json.dumps(line), flush=True)
+    return 0 if line['receivers_identical'] else 1
+
+
+def bscan_line(args, traces):
     """Uneven B-scan: `--traces` models (sides 128-256 cells, 1 mm, 10-cell
     PML, random soil), each `--bscan-its` iterations, built on the host first
     (outside both timings).  Both legs are timed the same way, from the first
@@ -111,7 +121,7 @@ def bench_bscan(args):
     from gprmax_amd.workloads import bscan_trace
 
     models = []
-    for t in range(args.traces):
+    for t in range(traces):
         models.append(bscan_trace(t, iterations=args.bscan_its))
         print('built trace {} {}'.format(t, models[-1].title), file=sys.stderr, flush=True)
     cells = sum(G.nx * G.ny * G.nz for G in models)
@@ -148,7 +158,7 @@ def bench_bscan(args):
         'n_gpus': 1, 'steps': args.bscan_its, 'warmup': 0,
         'higher_is_better': True, 'scaling': 'strong', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': 'uneven B-scan, {} traces, sides 128-256 cells'.format(args.traces),
+        'config': {'workload': 'uneven B-scan, {} traces, sides 128-256 cells'.format(traces),
                    'cells': cells, 'iterations': args.bscan_its},
         'sequential_wall_s': round(plain_wall, 3), 'sequential_device_s': round(plain_dev, 3),
         'batched_wall_s': round(batch_wall, 3), 'batched_upload_s': round(t_up, 3),
@@ -159,8 +169,7 @@ def bench_bscan(args):
         'plain_step_form': 'fused' if args.fused else ('two-pass' if args.passes else 'default (timed choice)'),
         'receivers_identical': same,
     }
-    print(json.dumps(line), flush=True)
-    return 0 if same else 1
+    return line
 
 
 def bench_edges(args):
@@ -199,6 +208,12 @@ def bench_edges(args):
 
 
 def bench_adjoint(args):
+    line = adjoint_line(args)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def adjoint_line(args):
     """Config of the differentiable solve: 160x160x120 cells at 2 mm, 10-cell
     PML, smooth true model (er in [4, 10], sigma in [1e-3, 2e-2]),
     homogeneous start model, 8 z-directed Hertzian sources x 16 Ez receivers,
@@ -285,8 +300,7 @@ def bench_adjoint(args):
         line['fd_rel_err_median'] = float(np.median(errs))
         line['fd_rel_err_max'] = float(np.max(errs))
     grid.close()
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 def step_form_label(fused, form, choice, default_form):
@@ -342,6 +356,9 @@ def main():
     ap.add_argument('--split-beside', action='store_true',
                     help='the split step with the H boundary pass beside the interior kernel (form 5)')
     ap.add_argument('--traces', type=int, default=8, help='B-scan traces (workload bscan)')
+    ap.add_argument('--forms', type=int, default=1,
+                    help='headline workload: also run the B-scan and adjoint workloads and report them (1) or not (0)')
+    ap.add_argument('--forms-traces', type=int, default=16, help='B-scan traces of that report')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
     ap.add_argument('--fd-cells', type=int, default=0, help='finite-difference check cells (adjoint workload)')
@@ -457,6 +474,19 @@ def main():
         'setup_s': round(t_setup, 3),
         'pcie_inclusive_mcells_per_s': round(cells * G.iterations / (ms / K * 1e-3 * G.iterations + t_setup) / 1e6, 1),
     }
+    if args.forms and args.workload == 'heterogeneous_halfspace':
+        # The metric's other two parts, from their own workloads in the same
+        # process: the batched B-scan (a 16-trace sample of the 96-trace
+        # workload) and the adjoint gradient.
+        b = bscan_line(args, args.forms_traces)
+        ad = adjoint_line(args)
+        line['forms'] = {
+            'bscan': {'value': b['value'], 'unit': b['unit'], 'traces': args.forms_traces,
+                      'device_ratio': b['device_ratio'], 'sequential_wall_s': b['sequential_wall_s'],
+                      'batched_wall_s': b['batched_wall_s'], 'receivers_identical': b['receivers_identical']},
+            'adjoint': {'value': ad['value'], 'unit': ad['unit'], 'forward_ms': ad['forward_ms'],
+                        'backward_ms': ad['backward_ms'], 'kept_states': ad.get('kept_states')},
+        }
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(args.workload)
     print(json.dumps(line), flush=True)
