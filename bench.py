@@ -145,7 +145,7 @@ def bscan_line(args, traces):
     with HipBatch(solvers) as b:
         tile, xchunk, ms_tune = b.tune(steps=3)
         t_tune = time.perf_counter() - t0 - t_up
-        ms = b.run(0, args.bscan_its, graph=True)
+        ms = b.run(0, args.bscan_its, graph=bool(args.bscan_graph))
         rx = [s.download_rx() for s in solvers]
     for s in solvers:
         s.close()
@@ -360,6 +360,7 @@ def main():
                     help='headline workload: also run the B-scan and adjoint workloads and report them (1) or not (0)')
     ap.add_argument('--forms-traces', type=int, default=16, help='B-scan traces of that report')
     ap.add_argument('--bscan-its', type=int, default=1500, help='iterations per B-scan trace')
+    ap.add_argument('--bscan-graph', type=int, default=1, help='B-scan batch: iterations in HIP graphs (1) or launched directly (0)')
     ap.add_argument('--adj-its', type=int, default=2000, help='iterations of the adjoint workload')
     ap.add_argument('--fd-cells', type=int, default=0, help='finite-difference check cells (adjoint workload)')
     ap.add_argument('--adj-keep', type=int, default=1,
