@@ -84,7 +84,11 @@ struct InvArgs {
     long long ntot;
     float d, DB, cH;                     // cell size; DB = dt/(mu0 d); cH = dt/mu0
     float *E[3], *H[3];                  // fields (forward) or adjoint fields
-    float *En[3];                        // adjoint: lE^n written here (lE^{n+1} is read from E)
+    // Adjoint fields, one float4 per grid point: lE of the sub-step (read),
+    // lE^n (written by the E half, completed by the H half) and lH.  The w of
+    // lE and lE^n is CB of the point's cell (0 where the point is no cell),
+    // carried along so that the gathers' CB lE terms come with one load.
+    float4 *lE, *lEn, *lH;
     float *EC;                           // forward: where to save, per point, E^n (3 floats)
                                          // then the curl of H^{n+1} (3), as three float2
                                          // (or null); adjoint: the saved values
@@ -139,6 +143,12 @@ __device__ __forceinline__ bool all_point(const InvArgs &a, int &i, int &j, int 
     j = r % (a.ny + 1);
     i = r / (a.ny + 1);
     return true;
+}
+
+// Component c (0, 1, 2) of an adjoint field's float4.
+__device__ __forceinline__ float comp3(const float4 &v, int c)
+{
+    return c == 0 ? v.x : (c == 1 ? v.y : v.z);
 }
 
 // Grid offsets are 32-bit: gpx_inv_create requires fewer than 2^31 points.
@@ -373,7 +383,7 @@ __device__ __forceinline__ float e_pml_ldd(const InvArgs &a, const InvSlab &S, i
     pidx(S, true, i, j, k, p, q[0], q[1]);
     const float RA01 = S.RE[p] - 1.0f, RF0 = S.RE[3 * S.t + p];
     const float IEA = (i < a.nx && j < a.ny && k < a.nz) ? a.IEA[cidx(a, i, j, k)] : 0.f;
-    const float lF = a.E[comp][fidx(a, i, j, k)];
+    const float lF = comp3(a.lE[fidx(a, i, j, k)], comp);
     const float mu = (w == 0 ? S.PE1 : S.PE2)[q[w]];
     return (float)sign * IEA * RA01 * lF - RF0 * mu;
 }
@@ -387,7 +397,7 @@ __device__ __forceinline__ float h_pml_ldd(const InvArgs &a, const InvSlab &S, i
     int q[2];
     pidx(S, false, i, j, k, p, q[0], q[1]);
     const float RA01 = S.RH[p] - 1.0f, RF0 = S.RH[3 * S.t + p];
-    const float lF = a.H[comp][fidx(a, i, j, k)];
+    const float lF = comp3(a.lH[fidx(a, i, j, k)], comp);
     const float mu = (w == 0 ? S.PH1 : S.PH2)[q[w]];
     return (float)sign * a.cH * RA01 * lF - RF0 * mu;
 }
@@ -421,13 +431,29 @@ __global__ void k_adj_disp(InvArgs a)
     int i, j, k;
     if (!all_point(a, i, j, k) || k >= a.nz || j >= a.ny || i >= a.nx) return;
     const int o = fidx(a, i, j, k), oc = cidx(a, i, j, k);
+    const float4 v = a.lE[o];
+    float l3[3] = {v.x, v.y, v.z};
     for (int c = 0; c < 3; ++c) {
         if (!e_upd(a, c, i, j, k)) continue;
-        float l = a.E[c][o];
+        float l = l3[c];
         for (int p = 0; p < a.P; ++p)
             l = l - a.dc[(long long)(2 * a.P + p) * a.ncell + oc] * a.T[c][(long long)p * a.ntot + o];
-        a.E[c][o] = l;
+        l3[c] = l;
     }
+    a.lE[o] = make_float4(l3[0], l3[1], l3[2], v.w);
+}
+
+// The adjoint fields at the start of the backward: zero, with CB of the
+// point's cell (0 where the point is no cell) in the w of both lE copies.
+__global__ void k_adj_init(InvArgs a)
+{
+    int i, j, k;
+    if (!all_point(a, i, j, k) || k > a.nz) return;
+    const int o = fidx(a, i, j, k);
+    const float cb = (i < a.nx && j < a.ny && k < a.nz) ? a.CB[cidx(a, i, j, k)] : 0.f;
+    a.lE[o] = make_float4(0.f, 0.f, 0.f, cb);
+    a.lEn[o] = make_float4(0.f, 0.f, 0.f, cb);
+    a.lH[o] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // lEz^n at point (i, j, k) after the H half: v plus dL/d(trace sample of
@@ -446,7 +472,8 @@ __device__ __forceinline__ float adj_rx_add(const InvArgs &a, int i, int j, int 
 __device__ __forceinline__ float xi(const InvArgs &a, int c, int i, int j, int k)
 {
     if (!e_upd(a, c, i, j, k)) return 0.f;
-    return a.CB[cidx(a, i, j, k)] * a.E[c][fidx(a, i, j, k)];
+    const float4 v = a.lE[fidx(a, i, j, k)];  // w: CB of the cell
+    return v.w * comp3(v, c);
 }
 
 // Reverse of the E half of iteration n, one thread per grid point:
@@ -487,16 +514,17 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
             // updated and no PML term exists here.
             const int o = g;
             const int NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
-            const int oc = cidx(a, i, j, k), CY = a.nz, CX = a.ny * a.nz;
-            const float *CB = a.CB;
-            auto XI = [&](int c, int off, int coff) { return CB[oc + coff] * a.E[c][o + off]; };
+            const int oc = cidx(a, i, j, k);
+            // CB lE at the point and at its k+1, j+1 and i+1 neighbours (the
+            // w of each is CB of that point's cell).
             // Every load comes before the first store: the compiler cannot
             // move a load above a store through these pointers, and a load
             // after a store costs the point a second memory round trip.
-            const float dh0 = XI(1, 0, 0) - XI(1, 1, 1) - XI(2, 0, 0) + XI(2, NZ1, CY);
-            const float dh1 = XI(2, 0, 0) - XI(2, PL, CX) - XI(0, 0, 0) + XI(0, 1, 1);
-            const float dh2 = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
-            const float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+            const float4 q0 = a.lE[o], qk = a.lE[o + 1], qj = a.lE[o + NZ1], qi = a.lE[o + PL];
+            const float dh0 = q0.w * q0.y - qk.w * qk.y - q0.w * q0.z + qj.w * qj.z;
+            const float dh1 = q0.w * q0.z - qi.w * qi.z - q0.w * q0.x + qk.w * qk.x;
+            const float dh2 = q0.w * q0.x - qj.w * qj.x - q0.w * q0.y + qi.w * qi.y;
+            const float lE[3] = {q0.x, q0.y, q0.z};
             const float CA = a.CA[oc];
             const float2 *sq = reinterpret_cast<const float2 *>(a.EC) + 3 * (size_t)o;
             const float2 s0 = sq[0], s1 = sq[1], s2 = sq[2];
@@ -506,12 +534,10 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
                 gca += lE[c] * En[c];
                 gcb += lE[c] * curl[c];
             }
-            const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
+            const float4 hq = a.lH[o];
             const float2 gab = a.gCAB[oc];
             const float gca0 = gab.x, gcb0 = gab.y;
-            a.H[0][o] = h0 + dh0;
-            a.H[1][o] = h1 + dh1;
-            a.H[2][o] = h2 + dh2;
+            a.lH[o] = make_float4(hq.x + dh0, hq.y + dh1, hq.z + dh2, hq.w);
             a.gCAB[oc] = make_float2(gca0 + gca, gcb0 + gcb);
             float giea = 0.f;
             if (i == a.si && j == a.sj && k == a.sk) giea += -lE[2] * a.waves[a.it] * a.sfac;
@@ -521,9 +547,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
                 for (int c = 0; c < 3; ++c) disp_reverse(a, c, o, oc, IEA, lE[c], out[c], giea);
             }
             if (giea != 0.f) a.gIEA[oc] += giea;
-            a.En[0][o] = out[0];
-            a.En[1][o] = out[1];
-            a.En[2][o] = out[2];
+            a.lEn[o] = make_float4(out[0], out[1], out[2], q0.w);
             return;
         }
         // Ex(i,j,k) = ... + CB (Hz(i,j,k) - Hz(i,j-1,k)) - CB (Hy(i,j,k) - Hy(i,j,k-1))
@@ -534,11 +558,10 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
             // every edge the curl terms read is updated: xi without its range
             // checks, as on the fast path (same terms, same order)
             const int o = g, NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
-            const int oc = cidx(a, i, j, k), CY = a.nz, CX = a.ny * a.nz;
-            auto XI = [&](int c, int off, int coff) { return a.CB[oc + coff] * a.E[c][o + off]; };
-            dh[0] = XI(1, 0, 0) - XI(1, 1, 1) - XI(2, 0, 0) + XI(2, NZ1, CY);
-            dh[1] = XI(2, 0, 0) - XI(2, PL, CX) - XI(0, 0, 0) + XI(0, 1, 1);
-            dh[2] = XI(0, 0, 0) - XI(0, NZ1, CY) - XI(1, 0, 0) + XI(1, PL, CX);
+            const float4 q0 = a.lE[o], qk = a.lE[o + 1], qj = a.lE[o + NZ1], qi = a.lE[o + PL];
+            dh[0] = q0.w * q0.y - qk.w * qk.y - q0.w * q0.z + qj.w * qj.z;
+            dh[1] = q0.w * q0.z - qi.w * qi.z - q0.w * q0.x + qk.w * qk.x;
+            dh[2] = q0.w * q0.x - qj.w * qj.x - q0.w * q0.y + qi.w * qi.y;
         } else {
             dh[0] = xi(a, 1, i, j, k) - xi(a, 1, i, j, k + 1) - xi(a, 2, i, j, k) + xi(a, 2, i, j + 1, k);
             dh[1] = xi(a, 2, i, j, k) - xi(a, 2, i + 1, j, k) - xi(a, 0, i, j, k) + xi(a, 0, i, j, k + 1);
@@ -546,9 +569,10 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
         }
         const int o = g;
         // Loads first (see the fast path), then the stores.
-        float lE[3] = {a.E[0][o], a.E[1][o], a.E[2][o]};
+        const float4 le = a.lE[o];
+        float lE[3] = {le.x, le.y, le.z};
         float out[3] = {lE[0], lE[1], lE[2]};
-        const float h0 = a.H[0][o], h1 = a.H[1][o], h2 = a.H[2][o];
+        const float4 hq = a.lH[o];
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
         float gca = 0.f, gcb = 0.f, gca0 = 0.f, gcb0 = 0.f;
         int oc = 0;
@@ -603,9 +627,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
                 if (own && cell) (w == 0 ? S.QE1 : S.QE2)[q[w]] = RE0 * mu[w] + (float)sign * IEA * RB0 * lE[comp];
             }
         }
-        a.H[0][o] = h0 + dh[0];
-        a.H[1][o] = h1 + dh[1];
-        a.H[2][o] = h2 + dh[2];
+        a.lH[o] = make_float4(hq.x + dh[0], hq.y + dh[1], hq.z + dh[2], hq.w);
         if (cell) {  // edges outside the cells are never updated
             float giea = 0.f;
             for (int c = 0; c < 3; ++c) {
@@ -617,9 +639,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
             a.gCAB[oc] = make_float2(gca0 + gca, gcb0 + gcb);
             if (giea != 0.f) a.gIEA[oc] += giea;
         }
-        a.En[0][o] = out[0];
-        a.En[1][o] = out[1];
-        a.En[2][o] = out[2];
+        a.lEn[o] = make_float4(out[0], out[1], out[2], le.w);
     }
 }
 
@@ -633,7 +653,7 @@ __global__ void k_adj_e(InvArgs a)
 __device__ __forceinline__ float eta(const InvArgs &a, int c, int i, int j, int k)
 {
     if (!h_upd(a, c, i, j, k)) return 0.f;
-    return a.DB * a.H[c][fidx(a, i, j, k)];
+    return a.DB * comp3(a.lH[fidx(a, i, j, k)], c);
 }
 
 // Reverse of the H half of iteration n (lH = dL/dH^{n+1}, complete):
@@ -670,14 +690,14 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by)
         if (FAST) {
             const int o = g;
             const int NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
-            auto ETA = [&](int c, int off) { return a.DB * a.H[c][o + off]; };
-            const float de0 = ETA(1, 0) - ETA(1, -1) - ETA(2, 0) + ETA(2, -NZ1);
-            const float de1 = ETA(2, 0) - ETA(2, -PL) - ETA(0, 0) + ETA(0, -1);
-            const float de2 = ETA(0, 0) - ETA(0, -NZ1) - ETA(1, 0) + ETA(1, -PL);
-            const float e2 = a.E[2][o] + de2;
-            a.E[0][o] += de0;
-            a.E[1][o] += de1;
-            a.E[2][o] = adj_rx_add(a, i, j, k, e2);
+            // DB lH at the point and at its k-1, j-1 and i-1 neighbours
+            const float4 h0 = a.lH[o], hk = a.lH[o - 1], hj = a.lH[o - NZ1], hi = a.lH[o - PL];
+            const float DB = a.DB;
+            const float de0 = DB * h0.y - DB * hk.y - DB * h0.z + DB * hj.z;
+            const float de1 = DB * h0.z - DB * hi.z - DB * h0.x + DB * hk.x;
+            const float de2 = DB * h0.x - DB * hj.x - DB * h0.y + DB * hi.y;
+            const float4 e = a.lE[o];
+            a.lE[o] = make_float4(e.x + de0, e.y + de1, adj_rx_add(a, i, j, k, e.z + de2), e.w);
             return;
         }
         // Hx(i,j,k) = Hx - DB (Ez(i,j+1,k) - Ez(i,j,k)) + DB (Ey(i,j,k+1) - Ey(i,j,k))
@@ -688,18 +708,20 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by)
             // every cell the curl terms read is updated: eta without its range
             // checks, as on the fast path (same terms, same order)
             const int o = g, NZ1 = a.nz + 1, PL = (a.ny + 1) * NZ1;
-            auto ETA = [&](int c, int off) { return a.DB * a.H[c][o + off]; };
-            de[0] = ETA(1, 0) - ETA(1, -1) - ETA(2, 0) + ETA(2, -NZ1);
-            de[1] = ETA(2, 0) - ETA(2, -PL) - ETA(0, 0) + ETA(0, -1);
-            de[2] = ETA(0, 0) - ETA(0, -NZ1) - ETA(1, 0) + ETA(1, -PL);
+            const float4 h0 = a.lH[o], hk = a.lH[o - 1], hj = a.lH[o - NZ1], hi = a.lH[o - PL];
+            const float DB = a.DB;
+            de[0] = DB * h0.y - DB * hk.y - DB * h0.z + DB * hj.z;
+            de[1] = DB * h0.z - DB * hi.z - DB * h0.x + DB * hk.x;
+            de[2] = DB * h0.x - DB * hj.x - DB * h0.y + DB * hi.y;
         } else {
             de[0] = eta(a, 1, i, j, k) - eta(a, 1, i, j, k - 1) - eta(a, 2, i, j, k) + eta(a, 2, i, j - 1, k);
             de[1] = eta(a, 2, i, j, k) - eta(a, 2, i - 1, j, k) - eta(a, 0, i, j, k) + eta(a, 0, i, j, k - 1);
             de[2] = eta(a, 0, i, j, k) - eta(a, 0, i, j - 1, k) - eta(a, 1, i, j, k) + eta(a, 1, i - 1, j, k);
         }
         // Loads first (lH here and lE for the update), then the stores.
-        const float lH[3] = {a.H[0][g], a.H[1][g], a.H[2][g]};
-        const float e0 = a.E[0][g], e1 = a.E[1][g], e2 = a.E[2][g];
+        const float4 lh = a.lH[g];
+        const float lH[3] = {lh.x, lh.y, lh.z};
+        const float4 ev = a.lE[g];
         // H-PML: dd = (S(h + axis) - S(h)) / d, so E here feeds the cell here (-)
         // and the cell one step down the axis (+).  The cell's own term
         // (h_pml_ldd of the cell, with its lH) shares the slab index, profile
@@ -730,9 +752,7 @@ __device__ __forceinline__ void adj_h_point(const InvArgs &a, int bx, int by)
                 if (own) (w == 0 ? S.QH1 : S.QH2)[q[w]] = RE0 * mu[w] + (float)sign * a.cH * RB0 * lH[comp];
             }
         }
-        a.E[0][g] = e0 + de[0];
-        a.E[1][g] = e1 + de[1];
-        a.E[2][g] = adj_rx_add(a, i, j, k, e2 + de[2]);
+        a.lE[g] = make_float4(ev.x + de[0], ev.y + de[1], adj_rx_add(a, i, j, k, ev.z + de[2]), ev.w);
     }
 }
 
@@ -777,8 +797,8 @@ struct gpx_inv {
     long long ntot = 0, ncell = 0;
     hipStream_t stream = nullptr;
     float *F = nullptr;            // E, H: 6 x ntot
-    float *L = nullptr;            // adjoint lE, lH: 6 x ntot
-    float *L2 = nullptr;           // the second copy of lE: 3 x ntot
+    float4 *LE[2] = {nullptr, nullptr};  // adjoint lE, two copies, and lH: one float4
+    float4 *LH = nullptr;                // per grid point each (InvArgs::lE)
     float *ck = nullptr;           // checkpoints: nck x (6 ntot + phi)
     float *seg = nullptr;          // segment store: K x 6 x ntot
     float *par = nullptr;          // CA, CB, IEA: 3 x ncell
@@ -894,8 +914,8 @@ extern "C" int gpx_inv_create(const gpx_inv_desc *d, gpx_inv **out)
     v->ncell = (long long)d->nx * d->ny * d->nz;
     hipError_t e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&v->F, 6 * v->ntot * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&v->L, 6 * v->ntot * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&v->L2, 3 * v->ntot * 4);
+    for (float4 **p : {&v->LE[0], &v->LE[1], &v->LH})
+        if (e == hipSuccess) e = hipMalloc((void **)p, v->ntot * 16);
     if (e == hipSuccess) e = hipMalloc((void **)&v->seg, (size_t)d->K * ((6 * v->ntot + 3) / 4 * 4) * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->par, 3 * v->ncell * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&v->grad, 3 * v->ncell * 4);
@@ -914,7 +934,9 @@ extern "C" int gpx_inv_destroy(gpx_inv *v)
     if (!v) return 0;
     hipSetDevice(v->desc.device);
     if (v->stream) hipStreamSynchronize(v->stream);
-    for (float *p : {v->F, v->L, v->L2, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid, v->dc,
+    for (float4 *p : {v->LE[0], v->LE[1], v->LH})
+        if (p) hipFree(p);
+    for (float *p : {v->F, v->ck, v->seg, v->par, v->grad, v->waves, v->traces, v->resid, v->dc,
                      v->T, v->LT, v->keep})
         if (p) hipFree(p);
     if (v->rxc) hipFree(v->rxc);
@@ -1075,10 +1097,9 @@ static InvArgs inv_args(const gpx_inv *v, bool adjoint, int it)
     const double mu0 = 1.25663706127e-6;
     a.DB = (float)((1.0 / g.d) * (g.dt / mu0));
     a.cH = (float)(g.dt / mu0);
-    float *base = adjoint ? v->L : v->F;
-    for (int c = 0; c < 3; ++c) {
-        a.E[c] = base + c * v->ntot;
-        a.H[c] = base + (3 + c) * v->ntot;
+    for (int c = 0; c < 3 && !adjoint; ++c) {  // the adjoint fields are set by the backward
+        a.E[c] = v->F + c * v->ntot;
+        a.H[c] = v->F + (3 + c) * v->ntot;
     }
     a.CA = v->par;
     a.CB = v->par + v->ncell;
@@ -1281,8 +1302,14 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     const gpx_inv_desc &g = v->desc;
     hipSetDevice(g.device);
     if (v->nrx) INV_TRY(hipMemcpy(v->resid, resid, (size_t)v->nrx * g.iterations * 4, hipMemcpyHostToDevice));
-    INV_TRY(hipMemsetAsync(v->L, 0, 6 * v->ntot * 4, v->stream));
-    INV_TRY(hipMemsetAsync(v->L2, 0, 3 * v->ntot * 4, v->stream));
+    {  // zero adjoint fields, CB of the cells in the w of both lE copies
+        InvArgs a = inv_args(v, true, 0);
+        a.lE = v->LE[0];
+        a.lEn = v->LE[1];
+        a.lH = v->LH;
+        hipLaunchKernelGGL(k_adj_init, grid_all(v), dim3(128), 0, v->stream, a);
+        INV_TRY(hipGetLastError());
+    }
     INV_TRY(hipMemsetAsync(v->grad, 0, 3 * v->ncell * 4, v->stream));
     if (v->P) INV_TRY(hipMemsetAsync(v->LT, 0, (size_t)3 * v->P * v->ntot * 4, v->stream));
     for (auto &S : v->slabs)
@@ -1290,7 +1317,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
             INV_TRY(hipMemsetAsync(S.mu[q], 0, S.n[q] * 4, v->stream));
             INV_TRY(hipMemsetAsync(S.mu2[q], 0, S.n[q] * 4, v->stream));
         }
-    float *lE[2] = {v->L, v->L2};  // current and next copy of lE
+    float4 *lEb[2] = {v->LE[0], v->LE[1]};  // current and next copy of lE
     int cur = 0;
     hipEvent_t e0, e1;
     INV_TRY(hipEventCreate(&e0));
@@ -1319,11 +1346,10 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
             float *slot = base + (long long)(it - n0) * seg_stride(v);
             InvArgs a = inv_args(v, true, it);
             a.EC = slot;
-            for (int q = 0; q < 3; ++q) {
-                if (v->P) a.Ps[q] = slot + (6 + q) * v->ntot;
-                a.E[q] = lE[cur] + q * v->ntot;
-                a.En[q] = lE[cur ^ 1] + q * v->ntot;
-            }
+            for (int q = 0; v->P && q < 3; ++q) a.Ps[q] = slot + (6 + q) * v->ntot;
+            a.lE = lEb[cur];
+            a.lEn = lEb[cur ^ 1];
+            a.lH = v->LH;
             if (v->P) hipLaunchKernelGGL(k_adj_disp, G, dim3(128), 0, v->stream, a);
             for (int s = 0; s < a.nslab; ++s) {
                 const auto &S = v->slabs[s];
@@ -1334,7 +1360,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
             }
             if (nofast) {
                 hipLaunchKernelGGL(k_adj_e<false>, GS, dim3(ADJ_BLOCK), 0, v->stream, a);
-                for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
+                a.lE = a.lEn;  // lE^n, completed by the H half
                 hipLaunchKernelGGL(k_adj_h<false>, GS, dim3(ADJ_BLOCK), 0, v->stream, a);
             } else {
                 a.mix_nf = (int)(GF.x * GF.y * GF.z);
@@ -1344,7 +1370,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
                 a.mix_gs = (int)GS.x;
                 const dim3 GM(a.mix_nf + a.mix_ns);
                 hipLaunchKernelGGL(k_adj_e_mix, GM, dim3(ADJ_BLOCK), 0, v->stream, a);
-                for (int q = 0; q < 3; ++q) a.E[q] = a.En[q];  // lE^n, completed by the H half
+                a.lE = a.lEn;  // lE^n, completed by the H half
                 hipLaunchKernelGGL(k_adj_h_mix, GM, dim3(ADJ_BLOCK), 0, v->stream, a);
             }
             cur ^= 1;
