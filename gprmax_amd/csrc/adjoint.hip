@@ -86,8 +86,9 @@ struct InvArgs {
     float *E[3], *H[3];                  // fields (forward) or adjoint fields
     // Adjoint fields, one float4 per grid point: lE of the sub-step (read),
     // lE^n (written by the E half, completed by the H half) and lH.  The w of
-    // lE and lE^n is CB of the point's cell (0 where the point is no cell),
-    // carried along so that the gathers' CB lE terms come with one load.
+    // lE and lE^n is CB of the point's cell, that of lH its CA (0 where the
+    // point is no cell), carried along so that the E gather's CB lE terms and
+    // CA come with the loads it makes anyway.
     float4 *lE, *lEn, *lH;
     float *EC;                           // forward: where to save, per point, E^n (3 floats)
                                          // then the curl of H^{n+1} (3), as three float2
@@ -444,16 +445,18 @@ __global__ void k_adj_disp(InvArgs a)
 }
 
 // The adjoint fields at the start of the backward: zero, with CB of the
-// point's cell (0 where the point is no cell) in the w of both lE copies.
+// point's cell in the w of both lE copies and its CA in the w of lH (0 where
+// the point is no cell).
 __global__ void k_adj_init(InvArgs a)
 {
     int i, j, k;
     if (!all_point(a, i, j, k) || k > a.nz) return;
     const int o = fidx(a, i, j, k);
-    const float cb = (i < a.nx && j < a.ny && k < a.nz) ? a.CB[cidx(a, i, j, k)] : 0.f;
+    const bool cell = i < a.nx && j < a.ny && k < a.nz;
+    const float cb = cell ? a.CB[cidx(a, i, j, k)] : 0.f, ca = cell ? a.CA[cidx(a, i, j, k)] : 0.f;
     a.lE[o] = make_float4(0.f, 0.f, 0.f, cb);
     a.lEn[o] = make_float4(0.f, 0.f, 0.f, cb);
-    a.lH[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.lH[o] = make_float4(0.f, 0.f, 0.f, ca);
 }
 
 // lEz^n at point (i, j, k) after the H half: v plus dL/d(trace sample of
@@ -525,7 +528,8 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
             const float dh1 = q0.w * q0.z - qi.w * qi.z - q0.w * q0.x + qk.w * qk.x;
             const float dh2 = q0.w * q0.x - qj.w * qj.x - q0.w * q0.y + qi.w * qi.y;
             const float lE[3] = {q0.x, q0.y, q0.z};
-            const float CA = a.CA[oc];
+            const float4 hq = a.lH[o];  // w: CA of the cell
+            const float CA = hq.w;
             const float2 *sq = reinterpret_cast<const float2 *>(a.EC) + 3 * (size_t)o;
             const float2 s0 = sq[0], s1 = sq[1], s2 = sq[2];
             const float En[3] = {s0.x, s0.y, s1.x}, curl[3] = {s1.y, s2.x, s2.y};
@@ -534,7 +538,6 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
                 gca += lE[c] * En[c];
                 gcb += lE[c] * curl[c];
             }
-            const float4 hq = a.lH[o];
             const float2 gab = a.gCAB[oc];
             const float gca0 = gab.x, gcb0 = gab.y;
             a.lH[o] = make_float4(hq.x + dh0, hq.y + dh1, hq.z + dh2, hq.w);
@@ -572,7 +575,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
         const float4 le = a.lE[o];
         float lE[3] = {le.x, le.y, le.z};
         float out[3] = {lE[0], lE[1], lE[2]};
-        const float4 hq = a.lH[o];
+        const float4 hq = a.lH[o];  // w: CA of the cell (0 where the point is no cell)
         const bool cell = i < a.nx && j < a.ny && k < a.nz;
         float gca = 0.f, gcb = 0.f, gca0 = 0.f, gcb0 = 0.f;
         int oc = 0;
@@ -581,7 +584,7 @@ __device__ __forceinline__ void adj_e_point(const InvArgs &a, int bx, int by)
             oc = cidx(a, i, j, k);
             const int om[3] = {i > 0 ? fidx(a, i - 1, j, k) : o, j > 0 ? fidx(a, i, j - 1, k) : o,
                                      k > 0 ? o - 1 : o};
-            CA = a.CA[oc];
+            CA = hq.w;
             IEA = a.IEA[oc];
             const float2 *sq = reinterpret_cast<const float2 *>(a.EC) + 3 * (size_t)o;
             const float2 s0 = sq[0], s1 = sq[1], s2 = sq[2];
@@ -798,7 +801,7 @@ struct gpx_inv {
     hipStream_t stream = nullptr;
     float *F = nullptr;            // E, H: 6 x ntot
     float4 *LE[2] = {nullptr, nullptr};  // adjoint lE, two copies, and lH: one float4
-    float4 *LH = nullptr;                // per grid point each (InvArgs::lE)
+    float4 *LH = nullptr;                // per grid point each (InvArgs::lE, lH)
     float *ck = nullptr;           // checkpoints: nck x (6 ntot + phi)
     float *seg = nullptr;          // segment store: K x 6 x ntot
     float *par = nullptr;          // CA, CB, IEA: 3 x ncell
@@ -1302,7 +1305,7 @@ extern "C" int gpx_inv_backward(gpx_inv *v, const float *resid, float *gCA, floa
     const gpx_inv_desc &g = v->desc;
     hipSetDevice(g.device);
     if (v->nrx) INV_TRY(hipMemcpy(v->resid, resid, (size_t)v->nrx * g.iterations * 4, hipMemcpyHostToDevice));
-    {  // zero adjoint fields, CB of the cells in the w of both lE copies
+    {  // zero adjoint fields, CB and CA of the cells in the w components
         InvArgs a = inv_args(v, true, 0);
         a.lE = v->LE[0];
         a.lEn = v->LE[1];
