@@ -62,7 +62,7 @@ def ref_edge_build():
     """The reference's compiled edge build (gprMax/yee_cell_build_ext.pyx),
     from oracle/_ref.  The module takes its Material class from
     gprMax.materials at import; it only constructs materials and sets their
-    er, se, mr, sm and type, so it is handed this build's Material class,
+    er, se, mr, sm and type, so it is handed gprmax_amd's Material class,
     whose attributes are the reference's."""
     if not os.path.isdir(REF_TREE):
         pytest.skip('reference kernels not built (oracle/build_ref.py)')

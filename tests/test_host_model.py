@@ -310,7 +310,7 @@ class _RefGrid:
 @pytest.mark.parametrize('seed', [0, 1, 2])
 def test_edge_build_matches_the_reference_compiled_build(seed, ref_edge_build):
     """The reference's own yee_cell_build_ext (compiled from its .pyx into
-    oracle/_ref) and this build's vectorised host build on the same grid:
+    oracle/_ref) and the vectorised host build of gprmax_amd.grid on the same grid:
     identical edge indices, and identical averaged materials created in the
     same order with the same names and constitutive values.  The rigid flags
     go to the reference as its 12 (E) and 6 (H) edge slots per cell, all set
