@@ -3170,17 +3170,32 @@ static void batch_pass_launch(const gpx_batch *b, const PassPlan &p, const PassG
     }
 }
 
+// A group of one model (a model larger than the group budget, or the last
+// one left) shares its launches with nothing, so it runs as a plain solve of
+// that model instead: run_steps with the model's own step form (timed on its
+// first long run, as gpx_run does), on the model's stream, ordered after the
+// batch stream's earlier work and before its later work.  The step forms
+// give the same bits as the passes.
+static bool group_alone(const PassGroup &g) { return g.gn == 1; }
+
 static int batch_pass_steps(gpx_batch *b, const PassPlan &p, int it0, int n, int graph, float *ms)
 {
     for (gpx_ctx *c : b->ctx) GPX_BTRY(hipStreamSynchronize(c->stream));
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, eb, ec;
     GPX_BTRY(hipEventCreate(&e0));
     GPX_BTRY(hipEventCreate(&e1));
+    GPX_BTRY(hipEventCreateWithFlags(&eb, hipEventDisableTiming));
+    GPX_BTRY(hipEventCreateWithFlags(&ec, hipEventDisableTiming));
     std::vector<hipGraph_t> gr;
     std::vector<hipGraphExec_t> ge;
     int rc = 0;
     if (graph) {
         for (const PassGroup &g : p.groups) {
+            if (group_alone(g)) {
+                gr.push_back(nullptr);
+                ge.push_back(nullptr);
+                continue;
+            }
             hipGraph_t h = nullptr;
             hipGraphExec_t x = nullptr;
             hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
@@ -3199,8 +3214,19 @@ static int batch_pass_steps(gpx_batch *b, const PassPlan &p, int it0, int n, int
     }
     if (!rc) {
         hipError_t e = hipEventRecord(e0, b->stream);
-        for (size_t gi = 0; e == hipSuccess && gi < p.groups.size(); ++gi) {
-            if (graph) {
+        for (size_t gi = 0; e == hipSuccess && !rc && gi < p.groups.size(); ++gi) {
+            if (group_alone(p.groups[gi])) {
+                gpx_ctx *c = b->ctx[p.groups[gi].g0];
+                e = hipEventRecord(eb, b->stream);
+                if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, eb, 0);
+                if (e != hipSuccess) break;
+                if (int r = run_steps(c, it0, n, nullptr)) {
+                    rc = bfail(b, r, std::string("gpx_batch: ") + gpx_last_error(c));
+                    break;
+                }
+                e = hipEventRecord(ec, c->stream);
+                if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, ec, 0);
+            } else if (graph) {
                 e = hipGraphLaunch(ge[gi], b->stream);
             } else {
                 for (int s = 0; e == hipSuccess && s < n; ++s) {
@@ -3209,17 +3235,24 @@ static int batch_pass_steps(gpx_batch *b, const PassPlan &p, int it0, int n, int
                 }
             }
         }
-        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float t = 0;
-        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-        if (e != hipSuccess) rc = bfail(b, (int)e, std::string("gpx_batch: ") + hipGetErrorString(e));
-        else if (ms) *ms = t;
+        if (!rc) {
+            if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e != hipSuccess) rc = bfail(b, (int)e, std::string("gpx_batch: ") + hipGetErrorString(e));
+            else if (ms) *ms = t;
+        }
     }
-    for (hipGraphExec_t x : ge) hipGraphExecDestroy(x);
-    for (hipGraph_t h : gr) hipGraphDestroy(h);
+    for (gpx_ctx *c : b->ctx) hipStreamSynchronize(c->stream);
+    for (hipGraphExec_t x : ge)
+        if (x) hipGraphExecDestroy(x);
+    for (hipGraph_t h : gr)
+        if (h) hipGraphDestroy(h);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    hipEventDestroy(eb);
+    hipEventDestroy(ec);
     return rc;
 }
 
@@ -3269,6 +3302,7 @@ extern "C" int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xch
     hipSetDevice(b->device);
     float best = -1.f;
     int bt = 0, bx = 0;
+    bool all_alone = true;
     {
         PassPlan p;
         if (int rc = batch_prepare_passes(b, p)) return rc;
@@ -3276,8 +3310,12 @@ extern "C" int gpx_batch_tune(gpx_batch *b, int steps, int *tile_cells, int *xch
         if (int rc = batch_pass_steps(b, p, 0, 1, 0, nullptr)) return rc;  // first launch
         if (int rc = batch_pass_steps(b, p, 0, steps, 0, &ms)) return rc;
         best = ms / steps;
+        for (const PassGroup &g : p.groups) all_alone = all_alone && group_alone(g);
     }
-    for (int tc : batch_tiles(b)) {
+    // Models too large to share a group each fill the GPU on their own and
+    // run as plain solves; the fused step over all models is not timed for
+    // them (it trails the plain forms of such models by a third).
+    for (int tc : all_alone ? std::vector<int>() : batch_tiles(b)) {
         BatchPlan p;
         if (int rc = batch_prepare(b, tc, 0, p)) return rc;
         const int base = p.xc;
