@@ -384,3 +384,80 @@ class HipBatch:
         ms = ctypes.c_float()
         self._check(self.lib.gpx_batch_run(self.b, int(it0), int(n), int(bool(graph)), ctypes.byref(ms)))
         return ms.value
+
+
+def _state_bytes(G):
+    """Device bytes of a model's fields and material indices (six float32
+    components and six uint16 indices per grid point), the measure the batch
+    groups models by."""
+    return 36.0 * (G.nx + 1) * (G.ny + 1) * (G.nz + 1)
+
+
+def solve_bscan(grids, device=0, graph=True, group_bytes=0, tune_steps=3):
+    """Solve every model of a B-scan (one grid per trace, any sizes, the same
+    iteration count) from the zero state; returns (the receiver buffer of
+    each grid, as HipSolver.download_rx gives it; device ms of the runs).
+
+    Consecutive models form groups as the batch forms them (state bytes up to
+    ``group_bytes``, 0: 192 MiB): a group of several runs as one HipBatch
+    (tuned, shared launches, ``graph``), a group of one as that model's plain
+    solve with its own step form.  While a group runs, a second host thread
+    builds and uploads the next one, so each group's set-up hides behind the
+    device work of the group before it; a group's models are freed once their
+    receivers are read."""
+    from concurrent.futures import ThreadPoolExecutor
+    grids = list(grids)
+    if not grids:
+        return [], 0.0
+    budget = float(group_bytes) if group_bytes else 192.0 * 2 ** 20
+    groups, acc = [], 0.0
+    for n, G in enumerate(grids):
+        b = _state_bytes(G)
+        if not groups or acc + b > budget:
+            groups.append([])
+            acc = 0.0
+        groups[-1].append(n)
+        acc += b
+
+    def create(idx):
+        made = []
+        try:
+            for n in idx:
+                made.append(HipSolver(grids[n], device))
+        except Exception:
+            for s in made:
+                s.close()
+            raise
+        return made
+
+    out = [None] * len(grids)
+    device_ms = 0.0
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        nxt = pool.submit(create, groups[0])
+        try:
+            for gi, idx in enumerate(groups):
+                solvers = nxt.result()
+                nxt = pool.submit(create, groups[gi + 1]) if gi + 1 < len(groups) else None
+                try:
+                    if len(solvers) == 1:
+                        ms, _, _ = solvers[0].run_timed(0, grids[idx[0]].iterations)
+                    else:
+                        with HipBatch(solvers) as b:
+                            if group_bytes:
+                                b.set_group_bytes(group_bytes)
+                            b.tune(steps=tune_steps)
+                            ms = b.run(0, grids[idx[0]].iterations, graph=graph)
+                    device_ms += ms
+                    for n, s in zip(idx, solvers):
+                        out[n] = s.download_rx()
+                finally:
+                    for s in solvers:
+                        s.close()
+        finally:
+            if nxt is not None:  # a group built ahead of a failure
+                try:
+                    for s in nxt.result():
+                        s.close()
+                except Exception:
+                    pass
+    return out, device_ms

@@ -496,6 +496,21 @@ def test_batch_matches_single_runs(tile):
             s.close()
 
 
+@pytest.mark.parametrize('group_bytes', [0, 1], ids=['one-group', 'each-alone'])
+def test_bscan_solve_matches_single_runs(group_bytes):
+    """The pipelined B-scan solve (groups built on a second host thread while
+    the group before runs): the three models in one group (shared launches)
+    or each alone (plain solves with their own step forms, timed on the
+    30-iteration run) give every model the receiver traces of its own run."""
+    from gprmax_amd.hip import solve_bscan
+    models = _batch_models(30)
+    ref = _single_runs(models, 30)
+    rx, ms = solve_bscan(models, group_bytes=group_bytes, tune_steps=2)
+    assert ms > 0 and len(rx) == len(models)
+    for got, (_, want) in zip(rx, ref):
+        assert np.array_equal(got, want)
+
+
 def test_batch_autotuner_picks_a_shape_and_resets_the_state():
     from gprmax_amd.hip import HipBatch
     models = _batch_models(16)
