@@ -393,6 +393,22 @@ def _state_bytes(G):
     return 36.0 * (G.nx + 1) * (G.ny + 1) * (G.nz + 1)
 
 
+def bscan_groups(grids, group_bytes=0):
+    """Consecutive grids in groups of at most ``group_bytes`` of state (0:
+    192 MiB), as lists of indices; a grid larger than the budget is a group
+    of its own."""
+    budget = float(group_bytes) if group_bytes else 192.0 * 2 ** 20
+    groups, acc = [], 0.0
+    for n, G in enumerate(grids):
+        b = _state_bytes(G)
+        if not groups or acc + b > budget:
+            groups.append([])
+            acc = 0.0
+        groups[-1].append(n)
+        acc += b
+    return groups
+
+
 def solve_bscan(grids, device=0, graph=True, group_bytes=0, tune_steps=3):
     """Solve every model of a B-scan (one grid per trace, any sizes, the same
     iteration count) from the zero state; returns (the receiver buffer of
@@ -409,15 +425,7 @@ def solve_bscan(grids, device=0, graph=True, group_bytes=0, tune_steps=3):
     grids = list(grids)
     if not grids:
         return [], 0.0
-    budget = float(group_bytes) if group_bytes else 192.0 * 2 ** 20
-    groups, acc = [], 0.0
-    for n, G in enumerate(grids):
-        b = _state_bytes(G)
-        if not groups or acc + b > budget:
-            groups.append([])
-            acc = 0.0
-        groups[-1].append(n)
-        acc += b
+    groups = bscan_groups(grids, group_bytes)
 
     def create(idx):
         made = []

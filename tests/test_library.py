@@ -92,3 +92,17 @@ def test_static_edge_build_refuses_too_many_sets_before_touching_a_device():
     assert rc == -3
     assert 'at most 32768' in lib.gpx_edges_last_error(None).decode()
     assert count.value == -1
+
+
+def test_bscan_groups_follow_the_state_budget():
+    """solve_bscan's groups: consecutive grids while their state (36 B per
+    grid point) fits the budget, a grid past it alone, order kept."""
+    from types import SimpleNamespace
+    from gprmax_amd.hip import bscan_groups
+    small = SimpleNamespace(nx=99, ny=99, nz=99)      # 1e6 points, 36e6 B
+    large = SimpleNamespace(nx=255, ny=255, nz=255)   # 16.8e6 points, 604e6 B
+    grids = [small, small, small, large, small, small, large, large]
+    assert bscan_groups(grids) == [[0, 1, 2], [3], [4, 5], [6], [7]]
+    assert bscan_groups(grids, group_bytes=80e6) == [[0, 1], [2], [3], [4, 5], [6], [7]]
+    assert bscan_groups(grids, group_bytes=1) == [[n] for n in range(8)]
+    assert bscan_groups([]) == []
